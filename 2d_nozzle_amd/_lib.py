@@ -1,0 +1,95 @@
+"""ctypes binding of ``libnozzle_hip.so`` (the C ABI in ``include/nozzle_hip.h``).
+
+The library is built in-tree by ``build.py`` (``hipcc --offload-arch=gfx950``)
+into ``2d_nozzle_amd/_lib/``.  There is no fallback: if the library is
+missing or fails to load, ``load()`` raises, and so does every entry point
+of the device backend.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libnozzle_hip.so")
+
+NZ_OK = 0
+
+
+class TableOffsets(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in (
+        "phi_vol", "dxi", "deta", "axi", "aeta", "phi_face", "lift", "w_vol", "w_face",
+        "mean_w", "minv_ref", "pad")]
+
+
+class Problem(C.Structure):
+    """Mirror of ``nz_problem``; field order and types must match the header."""
+
+    _fields_ = [
+        ("n_elem", C.c_int64), ("n_edge", C.c_int64),
+        ("nface", C.c_int32), ("nbf", C.c_int32), ("nqv", C.c_int32), ("nqf", C.c_int32),
+        ("affine", C.c_int32), ("straight", C.c_int32),
+        ("geo", C.c_void_p), ("area", C.c_void_p), ("minv", C.c_void_p), ("mean_w", C.c_void_p),
+        ("edge_geo", C.c_void_p), ("fe_nbr", C.c_void_p), ("fe_info", C.c_void_p),
+        ("tables", C.c_void_p),
+        ("off", TableOffsets),
+        ("gamma", C.c_double), ("efix", C.c_double), ("low_mach", C.c_double),
+        ("flux_id", C.c_int32), ("limiter", C.c_int32),
+        ("rho_t_bf", C.c_double), ("p_t_bf", C.c_double), ("band_bf", C.c_double),
+        ("at2", C.c_double), ("at", C.c_double), ("rho_t", C.c_double),
+        ("ca", C.c_double), ("sa", C.c_double), ("p_back", C.c_double),
+        ("rho_floor", C.c_double), ("p_floor", C.c_double), ("fraction", C.c_double),
+        ("lebesgue", C.c_double),
+        ("bisect_steps", C.c_int32), ("pad0", C.c_int32),
+        ("wnorm", C.c_double * 4),
+    ]
+
+
+# name -> (restype, argtypes); every exported symbol of the header
+P = C.c_void_p
+SIGNATURES = {
+    "nz_scratch_doubles": (C.c_int64, [C.POINTER(Problem)]),
+    "nz_residual": (C.c_int, [C.POINTER(Problem), P, P, P, P]),
+    "nz_rate": (C.c_int, [C.POINTER(Problem), P, P, P, P]),
+    "nz_inverse_mass": (C.c_int, [C.POINTER(Problem), P, P, P]),
+    "nz_stage": (C.c_int, [C.POINTER(Problem), P, P, P, P, C.c_double, C.c_double, C.c_double,
+                           C.c_int, P, P, P, P]),
+    "nz_norm_finish": (C.c_int, [C.POINTER(Problem), P, P, P]),
+    "nz_norm": (C.c_int, [C.POINTER(Problem), P, P, P, P]),
+    "nz_limit": (C.c_int, [C.POINTER(Problem), P, P, P, P]),
+    "nz_time_step": (C.c_int, [C.POINTER(Problem), P, C.c_double, P, P]),
+    "nz_last_error": (C.c_char_p, []),
+}
+
+_LIB: C.CDLL | None = None
+
+
+def load() -> C.CDLL:
+    """Load the HIP library (once).  Raises ``ImportError`` if it is absent."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"the HIP backend library is not built ({LIB_PATH} missing); "
+            "run `python -c 'import __graft_entry__ as g; g.build()'` or "
+            "`python 2d_nozzle_amd/build.py`"
+        )
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib
+
+
+class HipError(RuntimeError):
+    pass
+
+
+def check(rc: int, what: str) -> None:
+    if rc != NZ_OK:
+        msg = load().nz_last_error().decode(errors="replace")
+        raise HipError(f"{what} failed ({rc}): {msg}")

@@ -1,0 +1,395 @@
+"""The ``hip`` backend: the reference ``Backend`` interface on MI355X kernels.
+
+``HipBackend(ops, flow, opts)`` keeps the constructor, attributes and methods
+of the reference's backend ABC (``src/backends/base.py:81-201``) and of its
+compiled CPU backend (``src/backends/numba_backend.py:50-337``):
+
+* primitives ``residual(U) -> (R, wave)``, ``inverse_mass(R)``, ``limit(U)``,
+  ``time_step(wave)``, ``norm(A)``, ``asarray``, ``to_numpy``;
+* integrators ``rate``, ``euler_step``, ``ssprk3_step``, ``make_step`` and
+  ``run(U, n_steps, scheme) -> (U, res_last)``;
+* state ``name``, ``scheme``, ``n_limited``, ``n_mean_repaired``,
+  ``n_limit_calls``, ``limiter_activity()``, ``component_weight``.
+
+Arrays go in and come out in the same kind: NumPy in, NumPy out (so the
+reference's march and tests work unchanged), or ``torch`` device tensors in,
+device tensors out (no host round trip).  ``run`` copies the state in once and
+out once and never aliases its own buffers in what it returns
+(``numba_backend.py:283-303``).
+
+Every numerical call goes through ``libnozzle_hip.so``; there is no CPU path.
+An unknown flux raises ``NotImplementedError`` and an unknown scheme
+``ValueError``, as in the reference.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib
+from .config import (SCHEMES, component_weights, recommended_scheme, reference_floors,
+                     step_coefficient)
+from .operators import element_metrics
+
+FLUX_IDS = {"roe": 0, "hllc": 1, "ausmup": 2}
+
+
+def _torch():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("the hip backend needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+def _tables(rd) -> tuple[np.ndarray, dict]:
+    """Reference tables, concatenated; offsets in doubles."""
+    phi, dxi, deta, w = rd.phi_vol, rd.dphi_dxi_vol, rd.dphi_deta_vol, rd.w_vol
+    mass_ref = (phi * w[None, :]) @ phi.T
+    minv_ref = np.linalg.inv(mass_ref)
+    parts = {
+        "phi_vol": phi,
+        "dxi": dxi,
+        "deta": deta,
+        "axi": minv_ref @ (dxi * w[None, :]),
+        "aeta": minv_ref @ (deta * w[None, :]),
+        "phi_face": rd.phi_face,
+        "lift": np.einsum("ij,sfjq->sfiq", minv_ref, rd.phi_face),
+        "w_vol": w,
+        "w_face": rd.w_face,
+        "mean_w": (phi @ w) / w.sum(),
+        "minv_ref": minv_ref,
+    }
+    off, chunks, pos = {}, [], 0
+    for k, v in parts.items():
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+        off[k] = pos
+        chunks.append(v)
+        pos += v.size
+    return np.concatenate(chunks), off
+
+
+class HipBackend:
+    """MI355X execution of the DG residual and the pseudo-time march."""
+
+    name = "hip"
+
+    def __init__(self, ops, flow, opts, *, device: int | None = None):
+        torch = _torch()
+        self._torch = torch
+        self.lib = _lib.load()
+        self.ops, self.flow, self.opts = ops, flow, opts
+        self.n_limited = 0
+        self.n_mean_repaired = 0
+        self.n_limit_calls = 0
+        self.component_weight = np.asarray(component_weights(flow)).reshape(1, 1, 4)
+        rd = ops.ref
+        self.scheme = recommended_scheme(rd.order) if opts.scheme is None else opts.scheme
+        flux = getattr(flow, "flux", "roe")
+        if flux not in FLUX_IDS:
+            raise NotImplementedError(
+                f"the hip kernels implement {sorted(FLUX_IDS)}, so flux={flux!r} would silently "
+                "give a different answer here; add a kernel for it"
+            )
+        if opts.limiter not in ("none", "positivity"):
+            raise NotImplementedError(
+                f"limiter={opts.limiter!r} has no device kernel yet; use 'positivity' or 'none'"
+            )
+        self.device = torch.device("cuda", torch.cuda.current_device() if device is None else device)
+        f64 = dict(dtype=torch.float64, device=self.device)
+        topo, ed = ops.topology, ops.topology.edges
+        kind, q_geo = topo.kind, topo.geometry_order
+        E, N, nqv, nqf, nface = ops.n_elem, rd.n_basis, rd.n_qvol, rd.n_qface, topo.n_faces
+        self.E, self.N = E, N
+        self.affine = kind == "tri" and q_geo == 1
+        self.straight = q_geo == 1
+        ni = ops.n_interior
+        F = ni + ops.n_boundary
+        if F >= 2**28 or E >= 2**31:
+            raise ValueError("mesh too large for the 32-bit face maps")
+
+        # -- geometry
+        X = np.asarray(ops.node_coords, dtype=np.float64)
+        det, xix, xiy, etx, ety = element_metrics(rd, X[topo.elem_nodes])
+        if self.affine:
+            geo = np.stack([xix[:, 0], xiy[:, 0], etx[:, 0], ety[:, 0], det[:, 0]], axis=-1)
+        else:
+            geo = np.stack([xix, xiy, etx, ety, rd.w_vol[None, :] * det], axis=-1)
+        minv = mean_w = None
+        if not self.affine:
+            minv = getattr(ops, "inv_mass", None)
+            if minv is None:
+                wdet = rd.w_vol[None, :] * det
+                minv = np.linalg.inv(np.einsum("iq,jq,eq->eij", rd.phi_vol, rd.phi_vol, wdet))
+            mean_w = np.asarray(ops.mean_weights)
+        nrm = np.asarray(ops.edge_normal)
+        jac = np.asarray(ops.edge_jac)
+        if self.straight:
+            edge_geo = np.stack([nrm[:, 0, 0], nrm[:, 0, 1], jac[:, 0]], axis=-1)
+        else:
+            edge_geo = np.stack([nrm[..., 0], nrm[..., 1], jac], axis=-1)
+
+        # -- maps: neighbour (or -tag) and (edge << 3 | side << 2 | other face)
+        fe, fs = np.asarray(ed.face_edge), np.asarray(ed.face_side)
+        nbr = np.asarray(ed.face_neighbour).astype(np.int64).copy()
+        bnd = fe >= ni
+        nbr[bnd] = -np.asarray(ed.bedge_tag)[fe[bnd] - ni]
+        oface = np.zeros_like(fe)
+        inner = ~bnd
+        iface = np.asarray(ed.iedge_face)
+        oface[inner] = np.where(fs[inner] == 0, iface[fe[inner], 1], iface[fe[inner], 0])
+        info = (fe << 3) | (fs << 2) | oface
+
+        tables, off = _tables(rd)
+
+        def dev(a, dtype=None):
+            if a is None:
+                return None
+            t = torch.from_numpy(np.ascontiguousarray(a))
+            return t.to(device=self.device, dtype=dtype or t.dtype)
+
+        self._keep = {
+            "geo": dev(geo, torch.float64),
+            "area": dev(np.asarray(ops.elem_area), torch.float64),
+            "minv": dev(minv, torch.float64),
+            "mean_w": dev(mean_w, torch.float64),
+            "edge_geo": dev(edge_geo, torch.float64),
+            "fe_nbr": dev(nbr.astype(np.int32)),
+            "fe_info": dev(info.astype(np.int32)),
+            "tables": dev(tables),
+        }
+
+        def ptr(k):
+            t = self._keep[k]
+            return C.c_void_p(t.data_ptr() if t is not None else 0)
+
+        at2 = flow.gamma * flow.Rgas * flow.total_temperature
+        rho_floor, p_floor = reference_floors(flow)
+        w = component_weights(flow)
+        P = _lib.Problem()
+        P.n_elem, P.n_edge = E, F
+        P.nface, P.nbf, P.nqv, P.nqf = nface, N, nqv, nqf
+        P.affine, P.straight = int(self.affine), int(self.straight)
+        for k in ("geo", "area", "minv", "mean_w", "edge_geo", "fe_nbr", "fe_info", "tables"):
+            setattr(P, k, ptr(k))
+        for k, v in off.items():
+            setattr(P.off, k, v)
+        P.gamma, P.efix = flow.gamma, flow.entropy_fix
+        P.low_mach = getattr(flow, "hllc_low_mach", 0.0)
+        P.flux_id = FLUX_IDS[flux]
+        self.limiting = opts.limiter != "none" and rd.order > 0
+        P.limiter = 1 if self.limiting else 0
+        P.rho_t_bf = float(flow.stagnation_density) if getattr(flow, "backflow", False) else 0.0
+        P.p_t_bf, P.band_bf = flow.total_pressure, 0.05
+        P.at2, P.at = at2, math.sqrt(at2)
+        P.rho_t = flow.gamma * flow.total_pressure / at2
+        P.ca, P.sa = math.cos(flow.inflow_angle), math.sin(flow.inflow_angle)
+        P.p_back = flow.back_pressure_ratio * flow.total_pressure
+        P.rho_floor, P.p_floor, P.fraction = rho_floor, p_floor, 1e-8
+        # Lebesgue constant of the basis over the probe set (numba_backend.py:115-120)
+        P.lebesgue = float(max(np.abs(rd.phi_vol).sum(axis=0).max(),
+                               np.abs(rd.phi_face).sum(axis=2).max()))
+        P.bisect_steps = 12
+        for s in range(4):
+            P.wnorm[s] = w[s]
+        self.P = P
+        self._pp = C.byref(P)
+
+        # -- working set, allocated once (numba_backend.py:122-134)
+        shape = (E, N, 4)
+        self._S = [torch.empty(shape, **f64) for _ in range(4)]
+        self._F = torch.empty(shape, **f64)
+        self._wave = torch.empty(E, **f64)
+        self._dt = torch.empty(E, **f64)
+        nscr = int(self.lib.nz_scratch_doubles(self._pp))
+        if nscr < 0:
+            _lib.check(-1, "nz_scratch_doubles")
+        self._scratch = torch.empty(nscr, **f64)
+        self._res = torch.empty(1, **f64)
+        self._counters = torch.zeros(2, dtype=torch.int64, device=self.device)
+        self._flags = torch.zeros(E, dtype=torch.uint8, device=self.device)
+
+    # ------------------------------------------------------------ helpers
+    def _stream(self):
+        return C.c_void_p(self._torch.cuda.current_stream(self.device).cuda_stream)
+
+    def _in(self, A):
+        """(device tensor, was_numpy)."""
+        torch = self._torch
+        if isinstance(A, torch.Tensor):
+            t = A.to(device=self.device, dtype=torch.float64)
+            return t.contiguous(), False
+        return torch.from_numpy(np.ascontiguousarray(A, dtype=np.float64)).to(self.device), True
+
+    def _out(self, t, as_numpy: bool):
+        return t.cpu().numpy() if as_numpy else t
+
+    @staticmethod
+    def _p(t) -> C.c_void_p:
+        return C.c_void_p(t.data_ptr())
+
+    def _chk(self, rc, what):
+        _lib.check(rc, what)
+
+    # ------------------------------------------------------------ state
+    def limiter_activity(self) -> float:
+        if self.n_limit_calls == 0:
+            return float("nan")
+        return self.n_limited / self.n_limit_calls
+
+    def positivity_scale(self) -> float:
+        """``base.py:123-134``: extra dt factor while a limiter is on."""
+        pc = getattr(self.opts, "positivity_cfl", 0.0)
+        if pc > 0.0 and self.opts.limiter != "none" and self.ops.ref.order > 0:
+            return float(pc)
+        return 1.0
+
+    def dt_factor(self, scheme: str | None = None) -> float:
+        sch = self.scheme if scheme is None else scheme
+        return self.positivity_scale() * step_coefficient(self.ops.ref.order, self.opts.cfl, sch)
+
+    def asarray(self, A):
+        return self._in(A)[0]
+
+    def to_numpy(self, A) -> np.ndarray:
+        torch = self._torch
+        if isinstance(A, torch.Tensor):
+            return A.detach().cpu().numpy()
+        return np.asarray(A)
+
+    # ------------------------------------------------------------ primitives
+    def residual(self, U):
+        """``(R, wave)`` with ``M dU/dt = -R`` (``assembly.py:153-157``)."""
+        u, npy = self._in(U)
+        R = self._torch.empty_like(u)
+        wave = self._torch.empty(self.E, dtype=u.dtype, device=self.device)
+        self._chk(self.lib.nz_residual(self._pp, self._p(u), self._p(R), self._p(wave),
+                                       self._stream()), "nz_residual")
+        return self._out(R, npy), self._out(wave, npy)
+
+    def rate(self, U):
+        """``-M^-1 R`` and the wave-speed sum, in one pass (``element_rate``)."""
+        u, npy = self._in(U)
+        Fr = self._torch.empty_like(u)
+        wave = self._torch.empty(self.E, dtype=u.dtype, device=self.device)
+        self._chk(self.lib.nz_rate(self._pp, self._p(u), self._p(Fr), self._p(wave),
+                                   self._stream()), "nz_rate")
+        return self._out(Fr, npy), self._out(wave, npy)
+
+    def inverse_mass(self, R):
+        r, npy = self._in(R)
+        out = self._torch.empty_like(r)
+        self._chk(self.lib.nz_inverse_mass(self._pp, self._p(r), self._p(out), self._stream()),
+                  "nz_inverse_mass")
+        return self._out(out, npy)
+
+    def limit(self, U):
+        """Limited copy of ``U``; activity counters as ``numba_backend.py:265-280``."""
+        u, npy = self._in(U)
+        out = u.clone()
+        if self.limiting:
+            self._counters.zero_()
+            self._chk(self.lib.nz_limit(self._pp, self._p(out), self._p(self._flags),
+                                        self._p(self._counters), self._stream()), "nz_limit")
+            c = self._counters.cpu().tolist()
+            self.n_limited += int(c[0])
+            self.n_mean_repaired += int(c[1])
+            self.n_limit_calls += 1
+        return self._out(out, npy)
+
+    def limit_flags(self, U):
+        """``(U_limited, flags, (n_scaled, n_repaired))`` for parity checks.
+
+        ``flags`` bit 0: the cell mean was repaired; bit 1: a probe violated
+        the density or pressure bound.  Counters are not accumulated here.
+        """
+        u, npy = self._in(U)
+        out = u.clone()
+        cnt = self._torch.zeros(2, dtype=self._torch.int64, device=self.device)
+        flags = self._torch.zeros(self.E, dtype=self._torch.uint8, device=self.device)
+        if self.limiting:
+            self._chk(self.lib.nz_limit(self._pp, self._p(out), self._p(flags), self._p(cnt),
+                                        self._stream()), "nz_limit")
+        c = cnt.cpu().tolist()
+        return self._out(out, npy), self._out(flags, npy), (int(c[0]), int(c[1]))
+
+    def time_step(self, wave_sum):
+        w, npy = self._in(wave_sum)
+        dt = self._torch.empty_like(w)
+        self._chk(self.lib.nz_time_step(self._pp, self._p(w), self.dt_factor(), self._p(dt),
+                                        self._stream()), "nz_time_step")
+        dt = dt.reshape(-1, 1, 1)
+        return self._out(dt, npy)
+
+    def norm(self, A) -> float:
+        a, _ = self._in(A)
+        self._chk(self.lib.nz_norm(self._pp, self._p(a), self._p(self._scratch),
+                                   self._p(self._res), self._stream()), "nz_norm")
+        return float(self._res.item())
+
+    # ------------------------------------------------------------ integrators
+    def euler_step(self, U):
+        out, res = self.run(U, 1, "euler")
+        return out, res
+
+    def ssprk3_step(self, U):
+        out, res = self.run(U, 1, "ssprk3")
+        return out, res
+
+    def make_step(self, scheme: str):
+        if scheme == "euler":
+            return self.euler_step
+        if scheme == "ssprk3":
+            return self.ssprk3_step
+        raise ValueError(f"unknown scheme {scheme!r}; choose from {SCHEMES}")
+
+    def _stage(self, uin, un, uout, first: bool, a: float, b: float, fac: float, st):
+        self._chk(self.lib.nz_stage(
+            self._pp, self._p(uin), self._p(un) if un is not None else None, self._p(uout),
+            self._p(self._dt), fac, a, b, int(first), self._p(self._scratch), None,
+            self._p(self._counters), st), "nz_stage")
+
+    def advance(self, cur, nxt, n_steps: int, scheme: str):
+        """``n_steps`` steps from device tensor ``cur`` into ``nxt`` (device buffers).
+
+        Returns the buffer that holds the result.  Launches only; no sync.
+        """
+        if scheme not in SCHEMES:
+            raise ValueError(f"unknown scheme {scheme!r}; choose from {SCHEMES}")
+        st = self._stream()
+        fac = self.dt_factor(scheme)
+        t1, t2 = self._S[2], self._S[3]
+        for _ in range(n_steps):
+            if scheme == "euler":
+                self._stage(cur, None, nxt, True, 1.0, 1.0, fac, st)
+            else:
+                self._stage(cur, None, t1, True, 1.0, 1.0, fac, st)
+                self._stage(t1, cur, t2, False, 0.75, 0.25, fac, st)
+                self._stage(t2, cur, nxt, False, 1.0 / 3.0, 2.0 / 3.0, fac, st)
+            cur, nxt = nxt, cur
+        self._chk(self.lib.nz_norm_finish(self._pp, self._p(self._scratch), self._p(self._res), st),
+                  "nz_norm_finish")
+        return cur
+
+    def run(self, U, n_steps: int, scheme: str):
+        """Advance ``n_steps`` steps; returns ``(U, residual_norm_of_last_step)``."""
+        if scheme not in SCHEMES:
+            raise ValueError(f"unknown scheme {scheme!r}; choose from {SCHEMES}")
+        u, npy = self._in(U)
+        if n_steps <= 0:
+            return self._out(u.clone(), npy), float("nan")
+        cur, nxt = self._S[0], self._S[1]
+        cur.copy_(u)
+        self._counters.zero_()
+        res_buf = self.advance(cur, nxt, n_steps, scheme)
+        out = res_buf.clone()
+        res = float(self._res.item())
+        if self.limiting:
+            c = self._counters.cpu().tolist()
+            self.n_limited += int(c[0])
+            self.n_mean_repaired += int(c[1])
+            self.n_limit_calls += n_steps * (1 if scheme == "euler" else 3)
+        return self._out(out, npy), res

@@ -1,0 +1,776 @@
+// MI355X (gfx950) kernels for the DG residual, the fused SSP-RK3/Euler stage
+// and the Zhang-Shu positivity limiter, plus the C ABI declared in
+// include/nozzle_hip.h.
+//
+// Work decomposition.  One workgroup of 256 threads (four wave64s) owns a
+// tile of TE consecutive elements (TE = 64, or 128 when N <= 4).  The tile's
+// state is staged once in LDS, transposed to [component][element] with a
+// padded row so that a wave reading one component of 64 elements hits 64
+// consecutive doubles.  Every phase then maps the wave index to a
+// wave-uniform table index (component s, quadrature point q or basis
+// function i) and the lane to the element, so reference tables are read with
+// scalar loads and LDS traffic is conflict-free:
+//
+//   faces   for each local face f: (e,s) items gather the neighbour's trace
+//           at the face points straight from HBM; (e,q) items form the own
+//           trace from LDS and evaluate the Riemann or boundary flux;
+//           (e,i) items lift the weighted flux into register accumulators.
+//           Every interior flux is evaluated by both elements of the edge
+//           with identical operands, so both see the same bits and the
+//           assembly stays a pure gather with no flux table in HBM.
+//   volume  rounds of BS/TE quadrature points: (e,q) items form the
+//           contravariant Euler fluxes, (e,i) items project them.
+//   stage   the rate, the Runge-Kutta combination, the element-local
+//           positivity limiter, the time step and the residual-norm partials
+//           are fused; the new state leaves through LDS in coalesced rows.
+//
+// Affine triangles (Q = 1) fold M^-1 = M_ref^-1 / det into reference tables,
+// so an element carries only (xi_x, xi_y, eta_x, eta_y, det).  Curved or
+// quadrilateral elements carry per-point metrics and apply their own M^-1.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+
+#include "nozzle_hip.h"
+#include "physics.hip.h"
+
+namespace nz {
+
+constexpr int BS = 256;
+
+enum Mode { kRes = 0, kRate = 1, kStageFirst = 2, kStageNext = 3 };
+
+template <int N_, int NQV_, int NQF_, int NF_, bool AFF_, bool STR_>
+struct Cfg {
+    static constexpr int N = N_, NQV = NQV_, NQF = NQF_, NF = NF_;
+    static constexpr bool AFF = AFF_, STR = STR_;
+    static constexpr int NS = N * 4;
+    static constexpr int TE = N <= 4 ? 128 : 64;
+    static constexpr int TP = TE + 1;          // padded LDS row (doubles)
+    static constexpr int SLOTS = BS / TE;      // wave-uniform item slots per pass
+    static constexpr int NIT = (N + SLOTS - 1) / SLOTS;  // (e,i) items per thread
+    static constexpr int QR = SLOTS;           // volume points per round
+    // LDS work area: face round | volume round | per-element R (non-affine rate)
+    static constexpr int W_FACE = (2 * NQF * 4 + NQF) * TP;
+    static constexpr int W_VOL = 2 * QR * 4 * TP;
+    static constexpr int W_R = AFF ? 0 : NS * TP;
+    static constexpr int WORK = W_FACE > W_VOL ? (W_FACE > W_R ? W_FACE : W_R)
+                                               : (W_VOL > W_R ? W_VOL : W_R);
+    static constexpr int PF = 2 * NF * N * NQF;  // phi_face (and lift) table size
+};
+
+// ---------------------------------------------------------------- limiter
+// Zhang-Shu positivity limiter on one element whose coefficients sit at
+// u[(i*4+s)*stride] (src/backends/_numba_kernels.py:927-1093; flags as in
+// src/spatial/limiter.py:125-129,222-224).  No FP contraction, so the probe
+// sums and the bound tests round exactly as the CPU oracle's.
+struct LimTables {
+    const double* phi_vol;   // (N, nqv)
+    const double* phi_face;  // (2, nface, N, nqf)
+};
+
+template <class C>
+__device__ double probe_min(const double* u, int stride, const double ub[4], double theta,
+                            const LimTables& tb, const int side[C::NF], double g, double* pmin) {
+#pragma clang fp contract(off)
+    double rm = 1e300, pm = 1e300;
+#pragma unroll 1
+    for (int k = 0; k < C::NQV + C::NF * C::NQF; ++k) {
+        double V[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int i = 0; i < C::N; ++i) {
+            double b;
+            if (k < C::NQV) {
+                b = tb.phi_vol[i * C::NQV + k];
+            } else {
+                const int f = (k - C::NQV) / C::NQF, q = (k - C::NQV) % C::NQF;
+                b = tb.phi_face[((side[f] * C::NF + f) * C::N + i) * C::NQF + q];
+            }
+            for (int s = 0; s < 4; ++s) V[s] += b * u[(i * 4 + s) * stride];
+        }
+        for (int s = 0; s < 4; ++s) V[s] = ub[s] + theta * (V[s] - ub[s]);
+        rm = V[0] < rm ? V[0] : rm;
+        const double r = V[0] > 1e-300 ? V[0] : 1e-300;
+        const double pp = (g - 1.0) * (V[3] - 0.5 * (V[1] * V[1] + V[2] * V[2]) / r);
+        pm = pp < pm ? pp : pm;
+    }
+    *pmin = pm;
+    return rm;
+}
+
+template <class C>
+__device__ uint8_t limit_element(double* u, int stride, const double* mw, const LimTables& tb,
+                                 const int side[C::NF], const nz_problem& P, int* scaled) {
+#pragma clang fp contract(off)
+    const double g = P.gamma, g1 = g - 1.0;
+    uint8_t fl = 0;
+    double ub[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = 0; i < C::N; ++i)
+        for (int s = 0; s < 4; ++s) ub[s] += mw[i] * u[(i * 4 + s) * stride];
+    const double rho_new = ub[0] > P.rho_floor ? ub[0] : P.rho_floor;
+    const double e_min = P.p_floor / g1 + 0.5 * (ub[1] * ub[1] + ub[2] * ub[2]) / rho_new;
+    const double rhoE_new = ub[3] > e_min ? ub[3] : e_min;
+    const double d0 = rho_new - ub[0], d3 = rhoE_new - ub[3];
+    if (d0 != 0.0 || d3 != 0.0) {
+        for (int i = 0; i < C::N; ++i) {
+            u[(i * 4) * stride] += d0;
+            u[(i * 4 + 3) * stride] += d3;
+        }
+        ub[0] = rho_new;
+        ub[3] = rhoE_new;
+        fl |= 1;
+    }
+    const double p_bar = g1 * (ub[3] - 0.5 * (ub[1] * ub[1] + ub[2] * ub[2]) / ub[0]);
+    double eps_rho = P.fraction * ub[0];
+    if (eps_rho < 0.5 * P.rho_floor) eps_rho = 0.5 * P.rho_floor;
+    double eps_p = P.fraction * p_bar;
+    if (eps_p < 0.5 * P.p_floor) eps_p = 0.5 * P.p_floor;
+    // O(N) sufficient screen (_numba_kernels.py:1020-1045); never changes the answer
+    if (P.lebesgue > 0.0) {
+        double dr = 0, dmx = 0, dmy = 0, den = 0;
+        for (int i = 0; i < C::N; ++i) {
+            dr = fmax(dr, fabs(u[(i * 4) * stride] - ub[0]));
+            dmx = fmax(dmx, fabs(u[(i * 4 + 1) * stride] - ub[1]));
+            dmy = fmax(dmy, fabs(u[(i * 4 + 2) * stride] - ub[2]));
+            den = fmax(den, fabs(u[(i * 4 + 3) * stride] - ub[3]));
+        }
+        const double rho_lo = ub[0] - P.lebesgue * dr;
+        if (rho_lo >= eps_rho) {
+            const double m_hi =
+                sqrt(ub[1] * ub[1] + ub[2] * ub[2]) + P.lebesgue * sqrt(dmx * dmx + dmy * dmy);
+            const double p_lo = g1 * (ub[3] - P.lebesgue * den - 0.5 * m_hi * m_hi / rho_lo);
+            if (p_lo >= eps_p) return fl;
+        }
+    }
+    double pmin;
+    const double rmin = probe_min<C>(u, stride, ub, 1.0, tb, side, g, &pmin);
+    if (rmin >= eps_rho && pmin >= eps_p) return fl;
+    fl |= 2;
+    double theta = 1.0;
+    if (rmin < eps_rho) {
+        const double gap = ub[0] - rmin;
+        theta = gap <= 0.0 ? 0.0 : fmin(fmax((ub[0] - eps_rho) / gap, 0.0), 1.0);
+    }
+    double phi;
+    probe_min<C>(u, stride, ub, theta, tb, side, g, &phi);
+    if (phi < eps_p) {
+        double lo = 0.0, hi = theta;
+#pragma unroll 1
+        for (int it = 0; it < P.bisect_steps; ++it) {
+            const double mid = 0.5 * (lo + hi);
+            double pm;
+            probe_min<C>(u, stride, ub, mid, tb, side, g, &pm);
+            if (pm >= eps_p) lo = mid;
+            else hi = mid;
+        }
+        theta = lo;
+    }
+    for (int i = 0; i < C::N; ++i)
+        for (int s = 0; s < 4; ++s) {
+            double* x = &u[(i * 4 + s) * stride];
+            *x = ub[s] + theta * (*x - ub[s]);
+        }
+    *scaled = 1;
+    return fl;
+}
+
+__device__ __forceinline__ int wave_slot() {
+    return __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
+}
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int t = threadIdx.x;
+    if ((t & 63) == 0) red[t >> 6] = v;
+    __syncthreads();
+    double tot = 0.0;
+    if (t == 0)
+        for (int w = 0; w < BS / 64; ++w) tot += red[w];
+    return tot;
+}
+
+// ---------------------------------------------------------------- DG kernel
+template <class C>
+__global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __restrict__ T,
+                                           const double* __restrict__ Uin,
+                                           const double* __restrict__ Un,
+                                           double* __restrict__ Uout, double* __restrict__ dt,
+                                           double dt_factor, double ca, double cb, int mode,
+                                           double* __restrict__ partial,
+                                           uint8_t* __restrict__ flags,
+                                           unsigned long long* __restrict__ counters) {
+    constexpr int N = C::N, NQV = C::NQV, NQF = C::NQF, NF = C::NF, NS = C::NS;
+    constexpr int TE = C::TE, TP = C::TP, SLOTS = C::SLOTS;
+    __shared__ double sU[NS * TP];
+    __shared__ double sW[C::WORK];
+    __shared__ double sPF[C::PF];
+    __shared__ double sG[6 * TE];  // xi_x, xi_y, eta_x, eta_y, det | dt
+    __shared__ int sSide[NF * TE];
+    __shared__ double sRed[BS / 64];
+
+    const int t = threadIdx.x;
+    const int el = t % TE;
+    const int slot = wave_slot() / (TE / 64);  // uniform: s, q or i base
+    const int64_t e0 = static_cast<int64_t>(blockIdx.x) * TE;
+    const int ne = static_cast<int>(P.n_elem - e0 < TE ? P.n_elem - e0 : TE);
+    const bool live = el < ne;
+    const int64_t e = e0 + el;
+    const bool fold = C::AFF && mode != kRes;  // affine rate: M^-1 folded into tables
+
+    // ---- stage the tile
+    for (int it = t; it < TE * NS; it += BS) {
+        const int r = it / NS, c = it - r * NS;
+        sU[c * TP + r] = r < ne ? Uin[(e0 + r) * NS + c] : 0.0;
+    }
+    for (int it = t; it < C::PF; it += BS) sPF[it] = T[P.off.phi_face + it];
+    if (t < TE && live) {
+        if (C::AFF)
+            for (int k = 0; k < 5; ++k) sG[k * TE + t] = P.geo[e * 5 + k];
+        if (mode == kStageNext) sG[5 * TE + t] = dt[e];
+        for (int f = 0; f < NF; ++f) sSide[f * TE + t] = (P.fe_info[e * NF + f] >> 2) & 1;
+    }
+    __syncthreads();
+
+    double acc[C::NIT][4];
+#pragma unroll
+    for (int k = 0; k < C::NIT; ++k)
+        for (int s = 0; s < 4; ++s) acc[k][s] = 0.0;
+    double wave = 0.0;  // held by threads t < TE for element t
+
+    double* sTn = sW;                      // [NQF][4][TP] neighbour traces
+    double* sFw = sW + NQF * 4 * TP;       // [NQF][4][TP] signed, weighted fluxes
+    double* sSp = sW + 2 * NQF * 4 * TP;   // [NQF][TP]    signal speeds
+    const int lift_off = fold ? P.off.lift : P.off.phi_face;
+
+    // ---------------------------------------------------------------- faces
+    for (int f = 0; f < NF; ++f) {
+        // (e,s): neighbour trace at the face points, from HBM
+        for (int s = slot; s < 4; s += SLOTS) {
+            double tr[NQF];
+#pragma unroll
+            for (int q = 0; q < NQF; ++q) tr[q] = 0.0;
+            if (live) {
+                const int nb = P.fe_nbr[e * NF + f];
+                if (nb >= 0) {
+                    const int info = P.fe_info[e * NF + f];
+                    const int of = info & 3, oside = 1 - ((info >> 2) & 1);
+                    const double* ur = Uin + static_cast<int64_t>(nb) * NS + s;
+                    const double* ph = sPF + (oside * NF + of) * N * NQF;
+#pragma unroll
+                    for (int i = 0; i < N; ++i) {
+                        const double x = ur[i * 4];
+#pragma unroll
+                        for (int q = 0; q < NQF; ++q) tr[q] += ph[i * NQF + q] * x;
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < NQF; ++q) sTn[(q * 4 + s) * TP + el] = tr[q];
+        }
+        __syncthreads();
+        // (e,q): own trace, flux, weighting
+        for (int q = slot; q < NQF; q += SLOTS) {
+            if (live) {
+                const int nb = P.fe_nbr[e * NF + f];
+                const int info = P.fe_info[e * NF + f];
+                const int side = (info >> 2) & 1;
+                const int64_t k = info >> 3;
+                const double* ph = sPF + (side * NF + f) * N * NQF + q;
+                double uo[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    const double b = ph[i * NQF];
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) uo[s] += b * sU[(i * 4 + s) * TP + el];
+                }
+                const double* eg = C::STR ? P.edge_geo + k * 3 : P.edge_geo + (k * NQF + q) * 3;
+                const double nx = eg[0], ny = eg[1], jac = eg[2];
+                double fx[4], sp;
+                if (nb >= 0) {
+                    double un[4];
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) un[s] = sTn[(q * 4 + s) * TP + el];
+                    double ul[4], ur[4];
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) {
+                        ul[s] = side == 0 ? uo[s] : un[s];
+                        ur[s] = side == 0 ? un[s] : uo[s];
+                    }
+                    sp = interface_flux(P, ul, ur, nx, ny, fx);
+                } else {
+                    sp = boundary_flux(P, -nb, uo, nx, ny, fx);
+                }
+                double sc = jac * T[P.off.w_face + q];
+                double sg = side ? -1.0 : 1.0;
+                if (fold) sg = sg / sG[4 * TE + el];
+#pragma unroll
+                for (int s = 0; s < 4; ++s) sFw[(q * 4 + s) * TP + el] = sg * (fx[s] * sc);
+                sSp[q * TP + el] = sp;
+            }
+        }
+        __syncthreads();
+        // (e,i): lift
+#pragma unroll
+        for (int kk = 0; kk < C::NIT; ++kk) {
+            const int i = slot + kk * SLOTS;
+            if (i < N && live) {
+                const int side = sSide[f * TE + el];
+#pragma unroll
+                for (int q = 0; q < NQF; ++q) {
+                    const double b0 = T[lift_off + ((0 * NF + f) * N + i) * NQF + q];
+                    const double b1 = T[lift_off + ((1 * NF + f) * N + i) * NQF + q];
+                    const double b = side ? b1 : b0;
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) acc[kk][s] += b * sFw[(q * 4 + s) * TP + el];
+                }
+            }
+        }
+        if (t < TE && live) {
+            const int info = P.fe_info[e * NF + f];
+            const int64_t k = info >> 3;
+            double smax = 0.0, len = 0.0;
+            for (int q = 0; q < NQF; ++q) {
+                const double sp = sSp[q * TP + el];
+                smax = sp > smax ? sp : smax;
+                const double jac = C::STR ? P.edge_geo[k * 3 + 2] : P.edge_geo[(k * NQF + q) * 3 + 2];
+                len += jac * T[P.off.w_face + q];
+            }
+            wave += smax * len;
+        }
+        __syncthreads();
+    }
+
+    // --------------------------------------------------------------- volume
+    double* sFx = sW;                  // [QR][4][TP]
+    double* sFy = sW + C::QR * 4 * TP;
+    const int vx_off = fold ? P.off.axi : P.off.dxi;
+    const int vy_off = fold ? P.off.aeta : P.off.deta;
+    for (int q0 = 0; q0 < NQV; q0 += C::QR) {
+        {
+            const int qq = slot, q = q0 + qq;
+            if (q < NQV && live) {
+                double uq[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    const double b = T[P.off.phi_vol + i * NQV + q];
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) uq[s] += b * sU[(i * 4 + s) * TP + el];
+                }
+                const Prim w = primitives(uq, P.gamma);
+                const double F[4] = {w.r * w.u, w.r * w.u * w.u + w.p, w.r * w.u * w.v, w.r * w.u * w.H};
+                const double G[4] = {w.r * w.v, w.r * w.u * w.v, w.r * w.v * w.v + w.p, w.r * w.v * w.H};
+                double xx, xy, ex, ey, wd;
+                if (C::AFF) {
+                    xx = sG[el];
+                    xy = sG[TE + el];
+                    ex = sG[2 * TE + el];
+                    ey = sG[3 * TE + el];
+                    wd = T[P.off.w_vol + q] * sG[4 * TE + el];
+                } else {
+                    const double* gq = P.geo + (e * NQV + q) * 5;
+                    xx = gq[0];
+                    xy = gq[1];
+                    ex = gq[2];
+                    ey = gq[3];
+                    wd = gq[4];
+                }
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    double a = xx * F[s] + xy * G[s];
+                    double b = ex * F[s] + ey * G[s];
+                    if (!fold) {
+                        a *= wd;
+                        b *= wd;
+                    }
+                    sFx[(qq * 4 + s) * TP + el] = a;
+                    sFy[(qq * 4 + s) * TP + el] = b;
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < C::NIT; ++kk) {
+            const int i = slot + kk * SLOTS;
+            if (i < N && live) {
+                for (int qq = 0; qq < C::QR && q0 + qq < NQV; ++qq) {
+                    const int q = q0 + qq;
+                    const double ax = T[vx_off + i * NQV + q], ay = T[vy_off + i * NQV + q];
+#pragma unroll
+                    for (int s = 0; s < 4; ++s)
+                        acc[kk][s] -= ax * sFx[(qq * 4 + s) * TP + el] + ay * sFy[(qq * 4 + s) * TP + el];
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // ------------------------------------------------------------ epilogue
+    // acc holds R (unfolded) or -rate (affine, folded)
+    if (mode != kRes && !C::AFF) {
+        double* sR = sW;  // [NS][TP]
+#pragma unroll
+        for (int kk = 0; kk < C::NIT; ++kk) {
+            const int i = slot + kk * SLOTS;
+            if (i < N)
+                for (int s = 0; s < 4; ++s) sR[(i * 4 + s) * TP + el] = acc[kk][s];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < C::NIT; ++kk) {
+            const int i = slot + kk * SLOTS;
+            if (i < N && live) {
+                const double* mrow = P.minv + (e * N + i) * N;
+                double r[4] = {0.0, 0.0, 0.0, 0.0};
+                for (int j = 0; j < N; ++j) {
+                    const double m = mrow[j];
+                    for (int s = 0; s < 4; ++s) r[s] += m * sR[(j * 4 + s) * TP + el];
+                }
+                for (int s = 0; s < 4; ++s) acc[kk][s] = r[s];
+            }
+        }
+    }
+    const double osign = mode == kRes ? 1.0 : -1.0;  // rate = -M^-1 R
+
+    if (mode == kRes || mode == kRate) {
+#pragma unroll
+        for (int kk = 0; kk < C::NIT; ++kk) {
+            const int i = slot + kk * SLOTS;
+            if (i < N && live)
+                for (int s = 0; s < 4; ++s) Uout[(e * N + i) * 4 + s] = osign * acc[kk][s];
+        }
+        if (t < TE && live) dt[e] = wave;
+        return;
+    }
+
+    // fused stage: dt, combination, norm partials, limiter, store
+    if (mode == kStageFirst && t < TE && live) {
+        double d = wave < kFloor ? kFloor : wave;
+        const double h = dt_factor * 2.0 * P.area[e] / d;
+        sG[5 * TE + t] = h;
+        dt[e] = h;
+    }
+    __syncthreads();
+    double nacc = 0.0;
+#pragma unroll
+    for (int kk = 0; kk < C::NIT; ++kk) {
+        const int i = slot + kk * SLOTS;
+        if (i < N && live) {
+            const double h = sG[5 * TE + el];
+            for (int s = 0; s < 4; ++s) {
+                const double F = -acc[kk][s];
+                double* x = &sU[(i * 4 + s) * TP + el];
+                if (mode == kStageFirst) {
+                    *x = *x + h * F;
+                    const double v = F * P.wnorm[s];
+                    nacc += v * v;
+                } else {
+                    *x = ca * Un[(e * N + i) * 4 + s] + cb * (*x + h * F);
+                }
+            }
+        }
+    }
+    if (mode == kStageFirst) {
+        const double tot = block_sum(nacc, sRed);
+        if (t == 0) partial[blockIdx.x] = tot;
+    }
+    __syncthreads();
+    if (P.limiter == 1 && N > 1 && t < TE && live) {
+        int side[NF];
+        for (int f = 0; f < NF; ++f) side[f] = sSide[f * TE + t];
+        double mw[N];
+        for (int i = 0; i < N; ++i) mw[i] = C::AFF ? T[P.off.mean_w + i] : P.mean_w[e * N + i];
+        const LimTables tb{T + P.off.phi_vol, sPF};
+        int scaled = 0;
+        const uint8_t fl = limit_element<C>(sU + t, TP, mw, tb, side, P, &scaled);
+        if (flags) flags[e] = fl;
+        if (counters && (scaled || (fl & 1))) {
+            if (scaled) atomicAdd(&counters[0], 1ull);
+            if (fl & 1) atomicAdd(&counters[1], 1ull);
+        }
+    } else if (flags && t < TE && live) {
+        flags[e] = 0;
+    }
+    __syncthreads();
+    for (int it = t; it < TE * NS; it += BS) {
+        const int r = it / NS, c = it - r * NS;
+        if (r < ne) Uout[(e0 + r) * NS + c] = sU[c * TP + r];
+    }
+}
+
+// ------------------------------------------------------- standalone limiter
+template <class C>
+__global__ __launch_bounds__(BS) void k_limit(const nz_problem P, const double* __restrict__ T,
+                                              double* __restrict__ U, uint8_t* __restrict__ flags,
+                                              unsigned long long* __restrict__ counters) {
+    __shared__ double sPF[C::PF];
+    for (int it = threadIdx.x; it < C::PF; it += BS) sPF[it] = T[P.off.phi_face + it];
+    __syncthreads();
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * BS + threadIdx.x;
+    if (e >= P.n_elem) return;
+    uint8_t fl = 0;
+    if (P.limiter == 1 && C::N > 1) {
+        int side[C::NF];
+        for (int f = 0; f < C::NF; ++f) side[f] = (P.fe_info[e * C::NF + f] >> 2) & 1;
+        double mw[C::N];
+        for (int i = 0; i < C::N; ++i) mw[i] = C::AFF ? T[P.off.mean_w + i] : P.mean_w[e * C::N + i];
+        const LimTables tb{T + P.off.phi_vol, sPF};
+        int scaled = 0;
+        fl = limit_element<C>(U + e * C::NS, 1, mw, tb, side, P, &scaled);
+        if (counters) {
+            if (scaled) atomicAdd(&counters[0], 1ull);
+            if (fl & 1) atomicAdd(&counters[1], 1ull);
+        }
+    }
+    if (flags) flags[e] = fl;
+}
+
+// ------------------------------------------------------------ small kernels
+template <class C>
+__global__ __launch_bounds__(BS) void k_inverse_mass(const nz_problem P, const double* __restrict__ T,
+                                                     const double* __restrict__ R,
+                                                     double* __restrict__ out) {
+    const int64_t it = static_cast<int64_t>(blockIdx.x) * BS + threadIdx.x;
+    if (it >= P.n_elem * C::N) return;
+    const int64_t e = it / C::N;
+    const int i = static_cast<int>(it - e * C::N);
+    double r[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < C::N; ++j) {
+        const double m = C::AFF ? T[P.off.minv_ref + i * C::N + j] / P.geo[e * 5 + 4]
+                                : P.minv[(e * C::N + i) * C::N + j];
+        for (int s = 0; s < 4; ++s) r[s] += m * R[(e * C::N + j) * 4 + s];
+    }
+    for (int s = 0; s < 4; ++s) out[it * 4 + s] = r[s];
+}
+
+}  // namespace nz
+
+// ---------------------------------------------------------------- generic kernels
+namespace {
+
+__global__ __launch_bounds__(nz::BS) void k_norm_partial(const double* __restrict__ A, int64_t n,
+                                                         double w0, double w1, double w2, double w3,
+                                                         double* __restrict__ partial) {
+    __shared__ double red[nz::BS / 64];
+    const double w[4] = {w0, w1, w2, w3};
+    double acc = 0.0;
+    for (int64_t it = static_cast<int64_t>(blockIdx.x) * nz::BS + threadIdx.x; it < n;
+         it += static_cast<int64_t>(gridDim.x) * nz::BS) {
+        const double v = A[it] * w[it & 3];
+        acc += v * v;
+    }
+    const double tot = nz::block_sum(acc, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(nz::BS) void k_norm_finish(const double* __restrict__ partial,
+                                                        int64_t nparts, double count,
+                                                        double* __restrict__ out) {
+    __shared__ double red[nz::BS / 64];
+    double acc = 0.0;
+    for (int64_t it = threadIdx.x; it < nparts; it += nz::BS) acc += partial[it];
+    const double tot = nz::block_sum(acc, red);
+    if (threadIdx.x == 0) out[0] = sqrt(tot / count);
+}
+
+__global__ __launch_bounds__(nz::BS) void k_time_step(const double* __restrict__ wave,
+                                                      const double* __restrict__ area, int64_t n,
+                                                      double factor, double* __restrict__ dt) {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * nz::BS + threadIdx.x;
+    if (e >= n) return;
+    const double d = wave[e] < nz::kFloor ? nz::kFloor : wave[e];
+    dt[e] = factor * 2.0 * area[e] / d;
+}
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char* msg) {
+    std::snprintf(g_err, sizeof g_err, "%s", msg);
+    return code;
+}
+
+int check_launch(const char* what) {
+    const hipError_t st = hipGetLastError();
+    if (st != hipSuccess) {
+        std::snprintf(g_err, sizeof g_err, "%s: %s", what, hipGetErrorString(st));
+        return NZ_E_LAUNCH;
+    }
+    return NZ_OK;
+}
+
+// ------------------------------------------------------------- dispatch
+struct Ops {
+    void (*dg)(const nz_problem&, const double*, const double*, double*, double*, double, double,
+               double, int, double*, uint8_t*, unsigned long long*, int64_t, hipStream_t);
+    void (*limit)(const nz_problem&, double*, uint8_t*, unsigned long long*, hipStream_t);
+    void (*inverse_mass)(const nz_problem&, const double*, double*, hipStream_t);
+    int te;
+};
+
+template <class C>
+struct Launch {
+    static void dg(const nz_problem& P, const double* Uin, const double* Un, double* Uout,
+                   double* dt, double fac, double a, double b, int mode, double* partial,
+                   uint8_t* flags, unsigned long long* counters, int64_t nblk, hipStream_t st) {
+        hipLaunchKernelGGL((nz::k_dg<C>), dim3(static_cast<unsigned>(nblk)), dim3(nz::BS), 0, st, P,
+                           P.tables, Uin, Un, Uout, dt, fac, a, b, mode, partial, flags, counters);
+    }
+    static void limit(const nz_problem& P, double* U, uint8_t* flags, unsigned long long* counters,
+                      hipStream_t st) {
+        const unsigned nb = static_cast<unsigned>((P.n_elem + nz::BS - 1) / nz::BS);
+        hipLaunchKernelGGL((nz::k_limit<C>), dim3(nb), dim3(nz::BS), 0, st, P, P.tables, U, flags,
+                           counters);
+    }
+    static void inverse_mass(const nz_problem& P, const double* R, double* out, hipStream_t st) {
+        const unsigned nb = static_cast<unsigned>((P.n_elem * C::N + nz::BS - 1) / nz::BS);
+        hipLaunchKernelGGL((nz::k_inverse_mass<C>), dim3(nb), dim3(nz::BS), 0, st, P, P.tables, R, out);
+    }
+    static Ops ops() { return Ops{&dg, &limit, &inverse_mass, C::TE}; }
+};
+
+// (N, nqv, nqf, nface, affine, straight) for p = 0..3 of each element family
+#define NZ_FAMILY(X)                                                              \
+    X(1, 1, 1, 3, true, true) X(3, 4, 2, 3, true, true) X(6, 9, 3, 3, true, true) \
+    X(10, 16, 4, 3, true, true)                                                   \
+    X(1, 4, 2, 3, false, false) X(3, 9, 3, 3, false, false)                       \
+    X(6, 16, 4, 3, false, false) X(10, 25, 5, 3, false, false)                    \
+    X(1, 1, 1, 4, false, true) X(4, 4, 2, 4, false, true)                         \
+    X(9, 9, 3, 4, false, true) X(16, 16, 4, 4, false, true)                       \
+    X(1, 4, 2, 4, false, false) X(4, 9, 3, 4, false, false)                       \
+    X(9, 16, 4, 4, false, false) X(16, 25, 5, 4, false, false)
+
+bool lookup(const nz_problem* p, Ops* out) {
+    if (!p) return false;
+#define NZ_TRY(n, qv, qf, nf, aff, str)                                                  \
+    if (p->nbf == n && p->nqv == qv && p->nqf == qf && p->nface == nf &&                 \
+        (p->affine != 0) == aff && (p->straight != 0) == str) {                          \
+        *out = Launch<nz::Cfg<n, qv, qf, nf, aff, str>>::ops();                          \
+        return true;                                                                     \
+    }
+    NZ_FAMILY(NZ_TRY)
+#undef NZ_TRY
+    return false;
+}
+
+int resolve(const nz_problem* p, Ops* ops) {
+    if (!p) return fail(NZ_E_ARG, "null problem");
+    if (!lookup(p, ops)) {
+        char msg[256];
+        std::snprintf(msg, sizeof msg,
+                      "unsupported element configuration: nface=%d N=%d nqv=%d nqf=%d affine=%d "
+                      "straight=%d",
+                      p->nface, p->nbf, p->nqv, p->nqf, p->affine, p->straight);
+        return fail(NZ_E_ARG, msg);
+    }
+    if (p->n_elem <= 0) return fail(NZ_E_ARG, "empty mesh");
+    if (p->flux_id < 0 || p->flux_id > 2) return fail(NZ_E_ARG, "flux_id must be 0, 1 or 2");
+    return NZ_OK;
+}
+
+int64_t n_blocks(const nz_problem* p, const Ops& o) { return (p->n_elem + o.te - 1) / o.te; }
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI
+extern "C" {
+
+const char* nz_last_error(void) { return g_err; }
+
+int64_t nz_scratch_doubles(const nz_problem* p) {
+    Ops o;
+    if (resolve(p, &o) != NZ_OK) return -1;
+    const int64_t a = n_blocks(p, o);
+    return a > 1024 ? a : 1024;
+}
+
+int nz_residual(const nz_problem* p, const double* U, double* R, double* wave, void* stream) {
+    Ops o;
+    int rc = resolve(p, &o);
+    if (rc) return rc;
+    if (!U || !R || !wave) return fail(NZ_E_ARG, "null array");
+    o.dg(*p, U, nullptr, R, wave, 0.0, 0.0, 0.0, nz::kRes, nullptr, nullptr, nullptr, n_blocks(p, o),
+         static_cast<hipStream_t>(stream));
+    return check_launch("nz_residual");
+}
+
+int nz_rate(const nz_problem* p, const double* U, double* F, double* wave, void* stream) {
+    Ops o;
+    int rc = resolve(p, &o);
+    if (rc) return rc;
+    if (!U || !F || !wave) return fail(NZ_E_ARG, "null array");
+    o.dg(*p, U, nullptr, F, wave, 0.0, 0.0, 0.0, nz::kRate, nullptr, nullptr, nullptr,
+         n_blocks(p, o), static_cast<hipStream_t>(stream));
+    return check_launch("nz_rate");
+}
+
+int nz_inverse_mass(const nz_problem* p, const double* R, double* out, void* stream) {
+    Ops o;
+    int rc = resolve(p, &o);
+    if (rc) return rc;
+    if (!R || !out) return fail(NZ_E_ARG, "null array");
+    o.inverse_mass(*p, R, out, static_cast<hipStream_t>(stream));
+    return check_launch("nz_inverse_mass");
+}
+
+int nz_stage(const nz_problem* p, const double* Uin, const double* Un, double* Uout, double* dt,
+             double dt_factor, double a, double b, int first, double* scratch, uint8_t* flags,
+             uint64_t* counters, void* stream) {
+    Ops o;
+    int rc = resolve(p, &o);
+    if (rc) return rc;
+    if (!Uin || !Uout || !dt) return fail(NZ_E_ARG, "null array");
+    if (Uout == Uin || (Un && Uout == Un)) return fail(NZ_E_ARG, "Uout must not alias Uin or Un");
+    if (!first && !Un) return fail(NZ_E_ARG, "Un is required after the first stage");
+    if (first && !scratch) return fail(NZ_E_ARG, "scratch is required for the first stage");
+    o.dg(*p, Uin, Un, Uout, dt, dt_factor, a, b, first ? nz::kStageFirst : nz::kStageNext, scratch,
+         flags, reinterpret_cast<unsigned long long*>(counters), n_blocks(p, o),
+         static_cast<hipStream_t>(stream));
+    return check_launch("nz_stage");
+}
+
+int nz_norm_finish(const nz_problem* p, const double* scratch, double* out, void* stream) {
+    Ops o;
+    int rc = resolve(p, &o);
+    if (rc) return rc;
+    if (!scratch || !out) return fail(NZ_E_ARG, "null array");
+    const double count = static_cast<double>(p->n_elem) * p->nbf * 4.0;
+    hipLaunchKernelGGL(k_norm_finish, dim3(1), dim3(nz::BS), 0, static_cast<hipStream_t>(stream),
+                       scratch, n_blocks(p, o), count, out);
+    return check_launch("nz_norm_finish");
+}
+
+int nz_norm(const nz_problem* p, const double* A, double* scratch, double* out, void* stream) {
+    Ops o;
+    int rc = resolve(p, &o);
+    if (rc) return rc;
+    if (!A || !scratch || !out) return fail(NZ_E_ARG, "null array");
+    const int64_t n = p->n_elem * p->nbf * 4;
+    int64_t nb = (n + nz::BS - 1) / nz::BS;
+    if (nb > 1024) nb = 1024;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_norm_partial, dim3(static_cast<unsigned>(nb)), dim3(nz::BS), 0, st, A, n,
+                       p->wnorm[0], p->wnorm[1], p->wnorm[2], p->wnorm[3], scratch);
+    hipLaunchKernelGGL(k_norm_finish, dim3(1), dim3(nz::BS), 0, st, scratch, nb,
+                       static_cast<double>(n), out);
+    return check_launch("nz_norm");
+}
+
+int nz_limit(const nz_problem* p, double* U, uint8_t* flags, uint64_t* counters, void* stream) {
+    Ops o;
+    int rc = resolve(p, &o);
+    if (rc) return rc;
+    if (!U) return fail(NZ_E_ARG, "null array");
+    o.limit(*p, U, flags, reinterpret_cast<unsigned long long*>(counters),
+            static_cast<hipStream_t>(stream));
+    return check_launch("nz_limit");
+}
+
+int nz_time_step(const nz_problem* p, const double* wave, double factor, double* dt, void* stream) {
+    if (!p || !wave || !dt || !p->area) return fail(NZ_E_ARG, "null argument");
+    const unsigned nb = static_cast<unsigned>((p->n_elem + nz::BS - 1) / nz::BS);
+    hipLaunchKernelGGL(k_time_step, dim3(nb), dim3(nz::BS), 0, static_cast<hipStream_t>(stream),
+                       wave, p->area, p->n_elem, factor, dt);
+    return check_launch("nz_time_step");
+}
+
+}  // extern "C"

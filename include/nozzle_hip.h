@@ -1,0 +1,135 @@
+/*
+ * C ABI of the MI355X DG residual / SSP-RK3 engine (gfx950).
+ *
+ * Plain pointers and sizes only.  Every pointer argument below is a DEVICE
+ * pointer unless stated otherwise; `stream` is a hipStream_t passed as void*.
+ * Calls are asynchronous on `stream`; return codes are NZ_OK or a negative
+ * NZ_E* value, with a message from nz_last_error() (thread-local).  No C++
+ * exception crosses this boundary.  The library allocates nothing: every
+ * buffer, including scratch, is passed in by the caller.
+ *
+ * Arrays are FP64 and C-contiguous.  The state U is (E, N, 4) in the
+ * reference's element-batched layout (src/spatial/operators.py:138-190),
+ * N = n_basis, with conserved variables (rho, rho*vx, rho*vy, rho*E).
+ *
+ * The Python backend (2d_nozzle_amd/backend.py, class HipBackend) drives
+ * these entry points behind the reference's Backend interface
+ * (src/backends/base.py:81-201).
+ */
+#ifndef NOZZLE_HIP_H
+#define NOZZLE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NZ_OK 0
+#define NZ_E_ARG -1       /* invalid argument or unsupported (kind, p, Q) */
+#define NZ_E_LAUNCH -2    /* kernel launch failed */
+
+/* Offsets (in doubles) of the reference tables inside nz_problem.tables. */
+typedef struct {
+    int32_t phi_vol;   /* (N, nqv)                basis at volume points          */
+    int32_t dxi;       /* (N, nqv)                d phi / d xi                    */
+    int32_t deta;      /* (N, nqv)                d phi / d eta                   */
+    int32_t axi;       /* (N, nqv)  sum_j Minv_ref[i,j] w_q dphi_j/dxi (affine rate) */
+    int32_t aeta;      /* (N, nqv)                                                */
+    int32_t phi_face;  /* (2, nface, N, nqf)      side 0 walks sigma, 1 walks 1-sigma */
+    int32_t lift;      /* (2, nface, N, nqf)  Minv_ref @ phi_face (affine rate)   */
+    int32_t w_vol;     /* (nqv,)                                                  */
+    int32_t w_face;    /* (nqf,)                                                  */
+    int32_t mean_w;    /* (N,)  cell-average weights of an affine element         */
+    int32_t minv_ref;  /* (N, N) inverse reference mass matrix (affine)           */
+    int32_t pad;
+} nz_table_offsets;
+
+/* Everything one mesh/order/operating point needs.  Filled on the host by
+ * HipBackend; pointers are device pointers owned by the caller. */
+typedef struct {
+    int64_t n_elem;
+    int64_t n_edge;
+    int32_t nface, nbf, nqv, nqf;
+    int32_t affine;     /* 1: straight-sided triangles, constant metrics per element */
+    int32_t straight;   /* 1: straight edges (Q = 1), one normal/jacobian per edge   */
+    /* geometry */
+    const double *geo;      /* affine: (E, 5) = xi_x, xi_y, eta_x, eta_y, det
+                               else:   (E, nqv, 5) = xi_x, xi_y, eta_x, eta_y, w*det */
+    const double *area;     /* (E,) element area, sum_q w_q det_q                   */
+    const double *minv;     /* (E, N, N) inverse mass matrix (non-affine only)      */
+    const double *mean_w;   /* (E, N) cell-average weights (non-affine only)        */
+    const double *edge_geo; /* straight: (F, 3) = nx, ny, |dX/dsigma|;  else (F, nqf, 3) */
+    const int32_t *fe_nbr;  /* (E, nface) element across the face, or -tag (1..4)   */
+    const int32_t *fe_info; /* (E, nface) (edge << 3) | (side << 2) | other_face    */
+    const double *tables;   /* reference tables, laid out by `off`                  */
+    nz_table_offsets off;
+    /* flow (src/config.py:43-180; src/backends/numba_backend.py:69-109) */
+    double gamma, efix, low_mach;
+    int32_t flux_id;        /* 0 roe, 1 hllc, 2 ausmup */
+    int32_t limiter;        /* 0 none, 1 positivity (Zhang-Shu) */
+    double rho_t_bf, p_t_bf, band_bf;
+    double at2, at, rho_t, ca, sa, p_back;
+    /* limiter (src/spatial/limiter.py:98-243) */
+    double rho_floor, p_floor, fraction, lebesgue;
+    int32_t bisect_steps;
+    int32_t pad0;
+    double wnorm[4];        /* residual-norm component weights (src/backends/base.py:42-78) */
+} nz_problem;
+
+/* Number of scratch doubles nz_* calls need for norm partials, and the
+ * number of elements one workgroup handles. */
+int64_t nz_scratch_doubles(const nz_problem *p);
+
+/* R(U) and the wave-speed sum, unfolded.
+ * Replaces edge_pass + element_pass(apply_mass=False)
+ * (src/backends/_numba_kernels.py:484-583, 667-759) and the NumPy
+ * assembly.residual (src/spatial/assembly.py:153-157).  R: (E,N,4); wave: (E,). */
+int nz_residual(const nz_problem *p, const double *U, double *R, double *wave, void *stream);
+
+/* The rate -M^-1 R(U) and the wave-speed sum.
+ * Replaces edge_pass + element_rate (src/backends/_numba_kernels.py:586-664). */
+int nz_rate(const nz_problem *p, const double *U, double *F, double *wave, void *stream);
+
+/* out = M^-1 R per element (src/backends/_numba_kernels.py:762-777). */
+int nz_inverse_mass(const nz_problem *p, const double *R, double *out, void *stream);
+
+/* One fused Runge-Kutta stage: rate of Uin, stage combination, positivity
+ * limiter, written to Uout (which must not alias Uin or Un).
+ *   first != 0:  Uout = L(Uin + dt * F(Uin)); dt is computed from the wave
+ *                speeds with `dt_factor` and written to `dt`; the weighted
+ *                residual-norm partial sums of F are written to `scratch`.
+ *   first == 0:  Uout = L(a * Un + b * (Uin + dt * F(Uin))), dt read.
+ * flags (nullable): per-element limiter activity bits (bit 0 mean repaired,
+ * bit 1 probe bound violated).  counters (nullable, 2 x uint64, accumulated):
+ * elements scaled, cell means repaired.
+ * Replaces one stage of NumbaBackend._ssprk3_fused / _euler_fused
+ * (src/backends/numba_backend.py:305-337): _rate_into, _dt_into,
+ * rms_weighted, stage / combine_ssp and positivity_limit. */
+int nz_stage(const nz_problem *p, const double *Uin, const double *Un, double *Uout,
+             double *dt, double dt_factor, double a, double b, int first,
+             double *scratch, uint8_t *flags, uint64_t *counters, void *stream);
+
+/* Finish the residual norm from the partials nz_stage(first=1) left in
+ * scratch: out[0] = sqrt(sum / (E*N*4)) (src/backends/_numba_kernels.py:789-814). */
+int nz_norm_finish(const nz_problem *p, const double *scratch, double *out, void *stream);
+
+/* Weighted RMS norm of an arbitrary (E,N,4) array (Backend.norm,
+ * src/backends/numpy_backend.py:55-60).  Uses scratch; result in out[0]. */
+int nz_norm(const nz_problem *p, const double *A, double *scratch, double *out, void *stream);
+
+/* In-place Zhang-Shu positivity limiter (src/backends/_numba_kernels.py:927-1093,
+ * src/spatial/limiter.py:181-243).  flags/counters as in nz_stage. */
+int nz_limit(const nz_problem *p, double *U, uint8_t *flags, uint64_t *counters, void *stream);
+
+/* dt_e = factor * 2 A_e / max(wave_e, 1e-10) (src/timestepping/step.py:171-183). */
+int nz_time_step(const nz_problem *p, const double *wave, double factor, double *dt,
+                 void *stream);
+
+/* Last error message of this thread (static storage; never NULL). */
+const char *nz_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NOZZLE_HIP_H */

@@ -1,0 +1,67 @@
+"""The C ABI: the library loads, exports every declared symbol, and the
+ctypes mirror of ``nz_problem`` has the C layout.  No compute calls (CPU)."""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import ROOT, nz_module
+
+HEADER = os.path.join(ROOT, "include", "nozzle_hip.h")
+
+
+def declared_functions() -> list[str]:
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return re.findall(r"\b(nz_[a-z_]+)\s*\(", text)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = nz_module("_lib")
+    if not os.path.exists(lib.LIB_PATH):
+        pytest.fail("libnozzle_hip.so is not built; run __graft_entry__.build()")
+    dll = C.CDLL(lib.LIB_PATH)
+    names = declared_functions()
+    assert len(names) >= 9
+    for n in names:
+        assert hasattr(dll, n), n
+        assert n in lib.SIGNATURES, n
+
+
+def test_problem_struct_layout(tmp_path):
+    """Compile a probe against the header and compare offsets with ctypes."""
+    lib = nz_module("_lib")
+    fields = [f for f, _ in lib.Problem._fields_]
+    src = tmp_path / "probe.c"
+    body = "\n".join(f'printf("%s %zu\\n", "{f}", offsetof(nz_problem, {f}));' for f in fields)
+    src.write_text(
+        "#include <stdio.h>\n#include <stddef.h>\n#include \"nozzle_hip.h\"\n"
+        "int main(void){" + body + 'printf("size %zu\\n", sizeof(nz_problem)); return 0;}\n'
+    )
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = dict(line.split() for line in out if line)
+    for f in fields:
+        assert int(got[f]) == getattr(lib.Problem, f).offset, f
+    assert int(got["size"]) == C.sizeof(lib.Problem)
+
+
+def test_unsupported_configuration_is_an_error():
+    """An unknown element configuration is rejected on the host, before any launch."""
+    lib = nz_module("_lib")
+    if not os.path.exists(lib.LIB_PATH):
+        pytest.fail("libnozzle_hip.so is not built")
+    dll = lib.load()
+    P = lib.Problem()
+    P.n_elem, P.nface, P.nbf, P.nqv, P.nqf = 10, 3, 7, 5, 3  # no such element
+    rc = dll.nz_residual(C.byref(P), None, None, None, None)
+    assert rc == -1
+    assert b"unsupported element configuration" in dll.nz_last_error()
+    assert dll.nz_scratch_doubles(C.byref(P)) == -1

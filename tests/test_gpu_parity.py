@@ -1,0 +1,152 @@
+"""Device parity: the HIP path against the reference's golden vectors and the
+CPU oracle, through the C ABI (via ``HipBackend``).
+
+Bars: one residual evaluation <= 1e-12 relative in max norm (per flux);
+rates and steps <= 1e-12; limiter activation flags bit-exact with limited
+states <= 1e-13; wave-speed sums and time steps <= 1e-13.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import build_case, golden, nz_module, rel_err
+
+pytestmark = pytest.mark.gpu
+
+FLUXES = ("roe", "hllc", "ausmup")
+
+
+def backend(ops, flow, **opt):
+    cf = nz_module("config")
+    be = nz_module("backend")
+    return be.HipBackend(ops, flow, cf.SolverOptions(**opt))
+
+
+@pytest.mark.parametrize("flux", FLUXES)
+def test_residual_cfg1(flux):
+    g = golden("cfg1")
+    ops, flow = build_case("cfg1", dense=False)
+    bk = backend(ops, flow.replace(flux=flux))
+    for st in ("U0", "Ur"):
+        R, w = bk.residual(g[st])
+        print(flux, st, rel_err(R, g[f"R_{flux}_{st}"]), rel_err(w, g[f"wave_{flux}_{st}"]))
+        assert rel_err(R, g[f"R_{flux}_{st}"]) <= 1e-12
+        assert rel_err(w, g[f"wave_{flux}_{st}"]) <= 1e-13
+
+
+@pytest.mark.parametrize("name,key", [("quad_q2_p2", "U0"), ("tri_p2_q2_backflow", "U"),
+                                      ("synth_p3", "U"), ("synth_p1", "U"), ("tri_p0", "U0")])
+@pytest.mark.parametrize("flux", FLUXES)
+def test_residual_other_cases(name, key, flux):
+    g = golden(name)
+    ops, flow = build_case(name, dense=False)
+    suffix = f"_{key}" if name in ("quad_q2_p2", "tri_p0") else ""
+    if f"R_{flux}{suffix}" not in g:
+        pytest.skip("no golden for this flux")
+    R, w = backend(ops, flow.replace(flux=flux)).residual(g[key])
+    print(name, flux, rel_err(R, g[f"R_{flux}{suffix}"]))
+    assert rel_err(R, g[f"R_{flux}{suffix}"]) <= 1e-12
+    assert rel_err(w, g[f"wave_{flux}{suffix}"]) <= 1e-13
+
+
+@pytest.mark.parametrize("name", ["cfg1", "quad_q2_p2", "synth_p3"])
+def test_rate_and_inverse_mass(name):
+    from oracle.oracle import CpuOracle
+
+    g = golden(name)
+    ops, flow = build_case(name)
+    U = g["U0"] if "U0" in g else g["U"]
+    bk = backend(ops, flow)
+    F, w = bk.rate(U)
+    ref_key = {"cfg1": "rate_roe_U0", "quad_q2_p2": "rate_hllc_U0", "synth_p3": "rate_roe"}[name]
+    assert rel_err(F, g[ref_key]) <= 1e-12
+    orc = CpuOracle(ops, flow)
+    Fo, wo = orc.rate(U)
+    assert rel_err(F, Fo) <= 1e-12 and rel_err(w, wo) <= 1e-13
+    R, _ = bk.residual(U)
+    assert rel_err(-bk.inverse_mass(R), g[ref_key]) <= 1e-12
+
+
+def test_time_step_and_norm():
+    g = golden("cfg1")
+    ops, flow = build_case("cfg1", dense=False)
+    bk = backend(ops, flow)
+    F, w = bk.rate(g["U0"])
+    assert rel_err(bk.time_step(w)[:, 0, 0], g["dt_U0"]) <= 1e-13
+    assert abs(bk.norm(F) / float(g["norm_rate_roe_U0"]) - 1) <= 1e-13
+
+
+@pytest.mark.parametrize("name", ["cfg1", "quad_q2_p2"])
+def test_limiter_flags_bit_exact(name):
+    from oracle.oracle import CpuOracle
+
+    g = golden(name)
+    ops, flow = build_case(name)
+    bk = backend(ops, flow)
+    out, flags, (ns, nr) = bk.limit_flags(g["Us"])
+    assert np.array_equal(flags, g["Us_flags"])
+    assert rel_err(out, g["Us_positivity"]) <= 1e-13
+    o2, f2, c2 = CpuOracle(ops, flow).limit(g["Us"])
+    assert np.array_equal(flags, f2) and (ns, nr) == c2
+    assert rel_err(out, o2) <= 1e-15
+    # inactive on an admissible state: bit-identical output
+    out, flags, cnt = bk.limit_flags(g["U0"])
+    assert np.array_equal(out, g["U0"]) and not flags.any() and cnt == (0, 0)
+
+
+@pytest.mark.parametrize("scheme,steps,key", [("ssprk3", 1, "ssprk3_1step"),
+                                              ("ssprk3", 10, "ssprk3_10step"),
+                                              ("euler", 1, "euler_1step")])
+def test_steps_cfg1(scheme, steps, key):
+    g = golden("cfg1")
+    ops, flow = build_case("cfg1", dense=False)
+    bk = backend(ops, flow, scheme=scheme)
+    U, res = bk.run(g["U0"], steps, scheme)
+    print(scheme, steps, rel_err(U, g[f"{key}_U"]), res, float(g[f"{key}_res"]))
+    assert rel_err(U, g[f"{key}_U"]) <= 1e-12
+    assert abs(res / float(g[f"{key}_res"]) - 1) <= 1e-12
+
+
+def test_steps_quad_p2_and_p0():
+    g = golden("quad_q2_p2")
+    ops, flow = build_case("quad_q2_p2", dense=False)
+    U, res = backend(ops, flow).run(g["U0"], 1, "ssprk3")
+    assert rel_err(U, g["ssprk3_1step_U"]) <= 1e-12
+    assert abs(res / float(g["ssprk3_1step_res"]) - 1) <= 1e-12
+    g = golden("tri_p0")
+    ops, flow = build_case("tri_p0", dense=False)
+    bk = backend(ops, flow)
+    assert bk.scheme == "euler"
+    U, res = bk.run(g["U0"], 5, bk.scheme)
+    assert rel_err(U, g["5step_U"]) <= 1e-12
+    assert abs(res / float(g["5step_res"]) - 1) <= 1e-12
+
+
+def test_run_chunking_and_no_alias():
+    """Chunking invariance (<=1e-13) and the result never aliases backend buffers."""
+    g = golden("cfg1")
+    ops, flow = build_case("cfg1", dense=False)
+    bk = backend(ops, flow)
+    a, ra = bk.run(g["U0"], 6, "ssprk3")
+    b, _ = bk.run(g["U0"], 2, "ssprk3")
+    b, rb = bk.run(b, 4, "ssprk3")
+    assert rel_err(a, b) <= 1e-13 and abs(ra - rb) <= 1e-13 * abs(ra)
+    import torch
+
+    t = bk.asarray(g["U0"])
+    out, _ = bk.run(t, 2, "ssprk3")
+    for buf in bk._S:
+        assert out.data_ptr() != buf.data_ptr()
+
+
+def test_unknown_inputs_fail_loudly():
+    cf = nz_module("config")
+    be = nz_module("backend")
+    ops, flow = build_case("cfg1", dense=False)
+    bk = be.HipBackend(ops, flow, cf.SolverOptions())
+    with pytest.raises(ValueError, match="unknown scheme"):
+        bk.run(golden("cfg1")["U0"], 1, "rk4")
+    with pytest.raises(ValueError, match="unknown scheme"):
+        bk.make_step("rk4")
