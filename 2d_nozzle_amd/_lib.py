@@ -32,6 +32,7 @@ class Problem(C.Structure):
         ("affine", C.c_int32), ("straight", C.c_int32),
         ("geo", C.c_void_p), ("area", C.c_void_p), ("minv", C.c_void_p), ("mean_w", C.c_void_p),
         ("edge_geo", C.c_void_p), ("fe_nbr", C.c_void_p), ("fe_info", C.c_void_p),
+        ("n_int", C.c_int64), ("bedge", C.c_void_p), ("bflux", C.c_void_p),
         ("tables", C.c_void_p),
         ("off", TableOffsets),
         ("gamma", C.c_double), ("efix", C.c_double), ("low_mach", C.c_double),

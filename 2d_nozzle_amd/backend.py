@@ -142,6 +142,8 @@ class HipBackend:
         iface = np.asarray(ed.iedge_face)
         oface[inner] = np.where(fs[inner] == 0, iface[fe[inner], 1], iface[fe[inner], 0])
         info = (fe << 3) | (fs << 2) | oface
+        bedge = np.stack([np.asarray(ed.bedge_elem),
+                          (np.asarray(ed.bedge_face) << 3) | np.asarray(ed.bedge_tag)], axis=-1)
 
         tables, off = _tables(rd)
 
@@ -159,6 +161,8 @@ class HipBackend:
             "edge_geo": dev(edge_geo, torch.float64),
             "fe_nbr": dev(nbr.astype(np.int32)),
             "fe_info": dev(info.astype(np.int32)),
+            "bedge": dev(bedge.astype(np.int32)),
+            "bflux": torch.empty(max(ops.n_boundary, 1) * nqf * 5, **f64),
             "tables": dev(tables),
         }
 
@@ -173,8 +177,10 @@ class HipBackend:
         P.n_elem, P.n_edge = E, F
         P.nface, P.nbf, P.nqv, P.nqf = nface, N, nqv, nqf
         P.affine, P.straight = int(self.affine), int(self.straight)
-        for k in ("geo", "area", "minv", "mean_w", "edge_geo", "fe_nbr", "fe_info", "tables"):
+        for k in ("geo", "area", "minv", "mean_w", "edge_geo", "fe_nbr", "fe_info", "bedge",
+                  "bflux", "tables"):
             setattr(P, k, ptr(k))
+        P.n_int = ni
         for k, v in off.items():
             setattr(P.off, k, v)
         P.gamma, P.efix = flow.gamma, flow.entropy_fix

@@ -78,6 +78,9 @@ __device__ double probe_min(const double* u, int stride, const double ub[4], dou
     double rm = 1e300, pm = 1e300;
 #pragma unroll 1
     for (int k = 0; k < C::NQV + C::NF * C::NQF; ++k) {
+        // re-read the coefficients every probe instead of pinning N*4 of
+        // them in registers for the whole kernel
+        asm volatile("" ::: "memory");
         double V[4] = {0.0, 0.0, 0.0, 0.0};
         for (int i = 0; i < C::N; ++i) {
             double b;
@@ -191,7 +194,7 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 }
 
 // ---------------------------------------------------------------- DG kernel
-template <class C>
+template <class C, int FLUX>
 __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __restrict__ T,
                                            const double* __restrict__ Uin,
                                            const double* __restrict__ Un,
@@ -297,15 +300,21 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
                         ul[s] = side == 0 ? uo[s] : un[s];
                         ur[s] = side == 0 ? un[s] : uo[s];
                     }
-                    sp = interface_flux(P, ul, ur, nx, ny, fx);
+                    sp = interface_flux<FLUX>(P, ul, ur, nx, ny, fx);
+                    const double sc = jac * T[P.off.w_face + q];
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) fx[s] = fx[s] * sc;
                 } else {
-                    sp = boundary_flux(P, -nb, uo, nx, ny, fx);
+                    // boundary faces: weighted flux precomputed by k_bflux
+                    const double* bf = P.bflux + ((k - P.n_int) * NQF + q) * 5;
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) fx[s] = bf[s];
+                    sp = bf[4];
                 }
-                double sc = jac * T[P.off.w_face + q];
                 double sg = side ? -1.0 : 1.0;
                 if (fold) sg = sg / sG[4 * TE + el];
 #pragma unroll
-                for (int s = 0; s < 4; ++s) sFw[(q * 4 + s) * TP + el] = sg * (fx[s] * sc);
+                for (int s = 0; s < 4; ++s) sFw[(q * 4 + s) * TP + el] = sg * fx[s];
                 sSp[q * TP + el] = sp;
             }
         }
@@ -498,6 +507,37 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
     }
 }
 
+// ------------------------------------------------- boundary-face pre-pass
+// One item per (boundary edge, face point): owner trace, boundary flux
+// (inflow / outflow / wall / axis), weighted by |dX/dsigma| w_q, plus the
+// signal speed.  Runs before k_dg on the same stream; it keeps the boundary
+// physics out of the main kernel's register budget.
+template <class C>
+__global__ __launch_bounds__(BS) void k_bflux(const nz_problem P, const double* __restrict__ T,
+                                              const double* __restrict__ U) {
+    const int64_t it = static_cast<int64_t>(blockIdx.x) * BS + threadIdx.x;
+    const int64_t nb = P.n_edge - P.n_int;
+    if (it >= nb * C::NQF) return;
+    const int64_t b = it / C::NQF;
+    const int q = static_cast<int>(it - b * C::NQF);
+    const int64_t e = P.bedge[2 * b];
+    const int ft = P.bedge[2 * b + 1];
+    const int f = ft >> 3, tag = ft & 7;
+    double uo[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = 0; i < C::N; ++i) {
+        const double w = T[P.off.phi_face + (f * C::N + i) * C::NQF + q];
+        for (int s = 0; s < 4; ++s) uo[s] += w * U[(e * C::N + i) * 4 + s];
+    }
+    const int64_t k = P.n_int + b;
+    const double* eg = C::STR ? P.edge_geo + k * 3 : P.edge_geo + (k * C::NQF + q) * 3;
+    double fx[4];
+    const double sp = boundary_flux(P, tag, uo, eg[0], eg[1], fx);
+    const double sc = eg[2] * T[P.off.w_face + q];
+    double* out = P.bflux + (b * C::NQF + q) * 5;
+    for (int s = 0; s < 4; ++s) out[s] = fx[s] * sc;
+    out[4] = sp;
+}
+
 // ------------------------------------------------------- standalone limiter
 template <class C>
 __global__ __launch_bounds__(BS) void k_limit(const nz_problem P, const double* __restrict__ T,
@@ -612,8 +652,20 @@ struct Launch {
     static void dg(const nz_problem& P, const double* Uin, const double* Un, double* Uout,
                    double* dt, double fac, double a, double b, int mode, double* partial,
                    uint8_t* flags, unsigned long long* counters, int64_t nblk, hipStream_t st) {
-        hipLaunchKernelGGL((nz::k_dg<C>), dim3(static_cast<unsigned>(nblk)), dim3(nz::BS), 0, st, P,
-                           P.tables, Uin, Un, Uout, dt, fac, a, b, mode, partial, flags, counters);
+        const int64_t nbq = (P.n_edge - P.n_int) * C::NQF;
+        if (nbq > 0)
+            hipLaunchKernelGGL((nz::k_bflux<C>), dim3(static_cast<unsigned>((nbq + nz::BS - 1) / nz::BS)),
+                               dim3(nz::BS), 0, st, P, P.tables, Uin);
+        const dim3 g(static_cast<unsigned>(nblk));
+        if (P.flux_id == 1)
+            hipLaunchKernelGGL((nz::k_dg<C, 1>), g, dim3(nz::BS), 0, st, P, P.tables, Uin, Un, Uout,
+                               dt, fac, a, b, mode, partial, flags, counters);
+        else if (P.flux_id == 2)
+            hipLaunchKernelGGL((nz::k_dg<C, 2>), g, dim3(nz::BS), 0, st, P, P.tables, Uin, Un, Uout,
+                               dt, fac, a, b, mode, partial, flags, counters);
+        else
+            hipLaunchKernelGGL((nz::k_dg<C, 0>), g, dim3(nz::BS), 0, st, P, P.tables, Uin, Un, Uout,
+                               dt, fac, a, b, mode, partial, flags, counters);
     }
     static void limit(const nz_problem& P, double* U, uint8_t* flags, unsigned long long* counters,
                       hipStream_t st) {
@@ -664,6 +716,9 @@ int resolve(const nz_problem* p, Ops* ops) {
     }
     if (p->n_elem <= 0) return fail(NZ_E_ARG, "empty mesh");
     if (p->flux_id < 0 || p->flux_id > 2) return fail(NZ_E_ARG, "flux_id must be 0, 1 or 2");
+    if (p->n_int < 0 || p->n_int > p->n_edge) return fail(NZ_E_ARG, "n_int out of range");
+    if (p->n_edge > p->n_int && (!p->bedge || !p->bflux))
+        return fail(NZ_E_ARG, "boundary edges need bedge and the bflux workspace");
     return NZ_OK;
 }
 

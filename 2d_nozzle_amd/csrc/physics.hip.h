@@ -239,12 +239,14 @@ __device__ __forceinline__ double flux_outflow(const double U[4], double nx, dou
     return fabs(o.u * nx + o.v * ny) + sqrt(g * o.p / o.r);
 }
 
+// FLUX: 0 roe, 1 hllc, 2 ausmup (src/backends/numba_backend.py:59-67)
+template <int FLUX>
 __device__ __forceinline__ double interface_flux(const nz_problem& P, const double UL[4],
                                                  const double UR[4], double nx, double ny,
                                                  double out[4]) {
-    if (P.flux_id == 1) return flux_hllc(UL, UR, nx, ny, P.gamma, P.low_mach, out);
-    if (P.flux_id == 2) return flux_ausm(UL, UR, nx, ny, P.gamma, out);
-    return flux_roe(UL, UR, nx, ny, P.gamma, P.efix, out);
+    if constexpr (FLUX == 1) return flux_hllc(UL, UR, nx, ny, P.gamma, P.low_mach, out);
+    else if constexpr (FLUX == 2) return flux_ausm(UL, UR, nx, ny, P.gamma, out);
+    else return flux_roe(UL, UR, nx, ny, P.gamma, P.efix, out);
 }
 
 // tag: 1 inflow, 2 outflow, 3 wall, 4 axis (src/geometry/mesh.py:34-40)

@@ -62,6 +62,9 @@ typedef struct {
     const double *edge_geo; /* straight: (F, 3) = nx, ny, |dX/dsigma|;  else (F, nqf, 3) */
     const int32_t *fe_nbr;  /* (E, nface) element across the face, or -tag (1..4)   */
     const int32_t *fe_info; /* (E, nface) (edge << 3) | (side << 2) | other_face    */
+    int64_t n_int;          /* interior edges; boundary edge b has global id n_int + b */
+    const int32_t *bedge;   /* (Fb, 2): owner element, (local face << 3) | tag      */
+    double *bflux;          /* (Fb, nqf, 5) workspace: weighted boundary flux, speed */
     const double *tables;   /* reference tables, laid out by `off`                  */
     nz_table_offsets off;
     /* flow (src/config.py:43-180; src/backends/numba_backend.py:69-109) */
