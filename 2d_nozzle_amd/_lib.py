@@ -60,6 +60,8 @@ SIGNATURES = {
     "nz_norm": (C.c_int, [C.POINTER(Problem), P, P, P, P]),
     "nz_limit": (C.c_int, [C.POINTER(Problem), P, P, P, P]),
     "nz_time_step": (C.c_int, [C.POINTER(Problem), P, C.c_double, P, P]),
+    "nz_superbee_work_doubles": (C.c_int64, [C.POINTER(Problem)]),
+    "nz_superbee": (C.c_int, [C.POINTER(Problem), P, C.c_double, C.c_double, P, P, P]),
     "nz_last_error": (C.c_char_p, []),
 }
 

@@ -94,10 +94,8 @@ class HipBackend:
                 f"the hip kernels implement {sorted(FLUX_IDS)}, so flux={flux!r} would silently "
                 "give a different answer here; add a kernel for it"
             )
-        if opts.limiter not in ("none", "positivity"):
-            raise NotImplementedError(
-                f"limiter={opts.limiter!r} has no device kernel yet; use 'positivity' or 'none'"
-            )
+        if opts.limiter not in ("none", "positivity", "superbee"):
+            raise ValueError(f"unknown limiter {opts.limiter!r}")
         self.device = torch.device("cuda", torch.cuda.current_device() if device is None else device)
         f64 = dict(dtype=torch.float64, device=self.device)
         topo, ed = ops.topology, ops.topology.edges
@@ -187,7 +185,10 @@ class HipBackend:
         P.low_mach = getattr(flow, "hllc_low_mach", 0.0)
         P.flux_id = FLUX_IDS[flux]
         self.limiting = opts.limiter != "none" and rd.order > 0
-        P.limiter = 1 if self.limiting else 0
+        self.superbee = self.limiting and opts.limiter == "superbee"
+        P.limiter = (2 if self.superbee else 1) if self.limiting else 0
+        self.tvb = float(getattr(opts, "tvb_constant", 50.0))
+        self.area_total = float(np.sum(np.asarray(ops.elem_area)))
         P.rho_t_bf = float(flow.stagnation_density) if getattr(flow, "backflow", False) else 0.0
         P.p_t_bf, P.band_bf = flow.total_pressure, 0.05
         P.at2, P.at = at2, math.sqrt(at2)
@@ -215,7 +216,12 @@ class HipBackend:
             _lib.check(-1, "nz_scratch_doubles")
         self._scratch = torch.empty(nscr, **f64)
         self._res = torch.empty(1, **f64)
-        self._counters = torch.zeros(2, dtype=torch.int64, device=self.device)
+        # positivity scaled, cell means repaired, superbee slopes scaled
+        self._counters = torch.zeros(3, dtype=torch.int64, device=self.device)
+        self._sb_work = None
+        if self.superbee:
+            nsb = int(self.lib.nz_superbee_work_doubles(self._pp))
+            self._sb_work = torch.empty(nsb, **f64)
         self._flags = torch.zeros(E, dtype=torch.uint8, device=self.device)
 
     # ------------------------------------------------------------ helpers
@@ -298,13 +304,25 @@ class HipBackend:
         out = u.clone()
         if self.limiting:
             self._counters.zero_()
-            self._chk(self.lib.nz_limit(self._pp, self._p(out), self._p(self._flags),
-                                        self._p(self._counters), self._stream()), "nz_limit")
+            self._apply_limiter(out, self._stream())
             c = self._counters.cpu().tolist()
-            self.n_limited += int(c[0])
+            self.n_limited += max(int(c[0]), int(c[2]))
             self.n_mean_repaired += int(c[1])
             self.n_limit_calls += 1
         return self._out(out, npy)
+
+    def _apply_limiter(self, u, st, *, fused_positivity: bool = False) -> None:
+        """Superbee (when configured) then positivity, in place on device ``u``
+        (``limiter.py:408-429``).  ``fused_positivity``: the stage kernel has
+        already applied the positivity limiter."""
+        if self.superbee:
+            cnt = self._counters.data_ptr() + 2 * 8
+            self._chk(self.lib.nz_superbee(self._pp, self._p(u), self.tvb, self.area_total,
+                                           self._p(self._sb_work), C.c_void_p(cnt), st),
+                      "nz_superbee")
+        if self.superbee or not fused_positivity:
+            self._chk(self.lib.nz_limit(self._pp, self._p(u), None, self._p(self._counters), st),
+                      "nz_limit")
 
     def limit_flags(self, U):
         """``(U_limited, flags, (n_scaled, n_repaired))`` for parity checks.
@@ -357,6 +375,8 @@ class HipBackend:
             self._pp, self._p(uin), self._p(un) if un is not None else None, self._p(uout),
             self._p(self._dt), fac, a, b, int(first), self._p(self._scratch), None,
             self._p(self._counters), st), "nz_stage")
+        if self.superbee:
+            self._apply_limiter(uout, st, fused_positivity=True)
 
     def advance(self, cur, nxt, n_steps: int, scheme: str):
         """``n_steps`` steps from device tensor ``cur`` into ``nxt`` (device buffers).
@@ -394,8 +414,9 @@ class HipBackend:
         out = res_buf.clone()
         res = float(self._res.item())
         if self.limiting:
+            # per-call counts are summed on the device over the whole chunk
             c = self._counters.cpu().tolist()
-            self.n_limited += int(c[0])
+            self.n_limited += max(int(c[0]), int(c[2]))
             self.n_mean_repaired += int(c[1])
             self.n_limit_calls += n_steps * (1 if scheme == "euler" else 3)
         return self._out(out, npy), res

@@ -484,7 +484,7 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
         if (t == 0) partial[blockIdx.x] = tot;
     }
     __syncthreads();
-    if (P.limiter == 1 && N > 1 && t < TE && live) {
+    if (P.limiter == 1 && N > 1 && t < TE && live) {  // superbee runs as separate passes
         int side[NF];
         for (int f = 0; f < NF; ++f) side[f] = sSide[f * TE + t];
         double mw[N];
@@ -549,7 +549,7 @@ __global__ __launch_bounds__(BS) void k_limit(const nz_problem P, const double* 
     const int64_t e = static_cast<int64_t>(blockIdx.x) * BS + threadIdx.x;
     if (e >= P.n_elem) return;
     uint8_t fl = 0;
-    if (P.limiter == 1 && C::N > 1) {
+    if (P.limiter >= 1 && C::N > 1) {
         int side[C::NF];
         for (int f = 0; f < C::NF; ++f) side[f] = (P.fe_info[e * C::NF + f] >> 2) & 1;
         double mw[C::N];
@@ -563,6 +563,114 @@ __global__ __launch_bounds__(BS) void k_limit(const nz_problem P, const double* 
         }
     }
     if (flags) flags[e] = fl;
+}
+
+// ------------------------------------------------------------ superbee
+// Superbee TVB slope limiter (src/spatial/limiter.py:246-405) in three
+// passes: cell means and per-block maxima of |mean_s|; the global maxima;
+// then per element the trace-mean increments against the neighbours' mean
+// jumps, one theta per element, applied in place.
+__device__ __forceinline__ double minmod(double a, double b) {
+    return a * b <= 0.0 ? 0.0 : (fabs(a) < fabs(b) ? a : b);
+}
+__device__ __forceinline__ double maxmod(double a, double b) {
+    return a * b <= 0.0 ? 0.0 : (fabs(a) > fabs(b) ? a : b);
+}
+
+template <class C>
+__global__ __launch_bounds__(BS) void k_sb_means(const nz_problem P, const double* __restrict__ T,
+                                                 const double* __restrict__ U,
+                                                 double* __restrict__ ubar,
+                                                 double* __restrict__ pmax) {
+    __shared__ double red[4][BS / 64];
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * BS + threadIdx.x;
+    double m[4] = {0.0, 0.0, 0.0, 0.0};
+    if (e < P.n_elem) {
+        double ub[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int i = 0; i < C::N; ++i) {
+            const double w = C::AFF ? T[P.off.mean_w + i] : P.mean_w[e * C::N + i];
+            for (int s = 0; s < 4; ++s) ub[s] += w * U[(e * C::N + i) * 4 + s];
+        }
+        for (int s = 0; s < 4; ++s) {
+            ubar[e * 4 + s] = ub[s];
+            m[s] = fabs(ub[s]);
+        }
+    }
+    for (int s = 0; s < 4; ++s) {
+        double v = m[s];
+        for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+        if ((threadIdx.x & 63) == 0) red[s][threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double v = 0.0;
+        for (int w = 0; w < BS / 64; ++w) v = fmax(v, red[threadIdx.x][w]);
+        pmax[blockIdx.x * 4 + threadIdx.x] = v;
+    }
+}
+
+__global__ __launch_bounds__(BS) void k_sb_scale(const double* __restrict__ pmax, int64_t nblk,
+                                                 double* __restrict__ scale) {
+    __shared__ double red[4][BS / 64];
+    double m[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t b = threadIdx.x; b < nblk; b += BS)
+        for (int s = 0; s < 4; ++s) m[s] = fmax(m[s], pmax[b * 4 + s]);
+    for (int s = 0; s < 4; ++s) {
+        double v = m[s];
+        for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+        if ((threadIdx.x & 63) == 0) red[s][threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double v = 0.0;
+        for (int w = 0; w < BS / 64; ++w) v = fmax(v, red[threadIdx.x][w]);
+        scale[threadIdx.x] = v;
+    }
+}
+
+template <class C>
+__global__ __launch_bounds__(BS) void k_sb_apply(const nz_problem P, const double* __restrict__ T,
+                                                 double* __restrict__ U,
+                                                 const double* __restrict__ ubar,
+                                                 const double* __restrict__ scale, double tvb,
+                                                 double area_total,
+                                                 unsigned long long* __restrict__ counters) {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * BS + threadIdx.x;
+    if (e >= P.n_elem) return;
+    constexpr double tol = 1e-12;
+    double wsum = 0.0;
+    for (int q = 0; q < C::NQF; ++q) wsum += T[P.off.w_face + q];
+    const double share = P.area[e] / area_total;
+    double ub[4];
+    for (int s = 0; s < 4; ++s) ub[s] = ubar[e * 4 + s];
+    double theta = 1.0;
+    for (int f = 0; f < C::NF; ++f) {
+        const int nb = P.fe_nbr[e * C::NF + f];
+        const int side = (P.fe_info[e * C::NF + f] >> 2) & 1;
+        const double* ph = T + P.off.phi_face + (side * C::NF + f) * C::N * C::NQF;
+        for (int s = 0; s < 4; ++s) {
+            double tm = 0.0;
+            for (int i = 0; i < C::N; ++i) {
+                const double u = U[(e * C::N + i) * 4 + s];
+                for (int q = 0; q < C::NQF; ++q) tm += ph[i * C::NQF + q] * T[P.off.w_face + q] * u;
+            }
+            const double dev = tm / wsum - ub[s];
+            if (nb < 0 || fabs(dev) <= tol) continue;  // boundary face or no increment
+            if (fabs(dev) <= tvb * share * scale[s]) continue;  // below the TVB threshold
+            const double jump = ubar[static_cast<int64_t>(nb) * 4 + s] - ub[s];
+            const double allowed = maxmod(minmod(2.0 * jump, dev), minmod(jump, 2.0 * dev));
+            const double tf = fmin(fmax(allowed / dev, 0.0), 1.0);
+            theta = tf < theta ? tf : theta;
+        }
+    }
+    if (theta < 1.0) {
+        for (int i = 0; i < C::N; ++i)
+            for (int s = 0; s < 4; ++s) {
+                double* x = &U[(e * C::N + i) * 4 + s];
+                *x = ub[s] + theta * (*x - ub[s]);
+            }
+        if (counters) atomicAdd(&counters[0], 1ull);
+    }
 }
 
 // ------------------------------------------------------------ small kernels
@@ -644,6 +752,8 @@ struct Ops {
                double, int, double*, uint8_t*, unsigned long long*, int64_t, hipStream_t);
     void (*limit)(const nz_problem&, double*, uint8_t*, unsigned long long*, hipStream_t);
     void (*inverse_mass)(const nz_problem&, const double*, double*, hipStream_t);
+    void (*superbee)(const nz_problem&, double*, double, double, double*,
+                     unsigned long long*, hipStream_t);
     int te;
 };
 
@@ -677,7 +787,19 @@ struct Launch {
         const unsigned nb = static_cast<unsigned>((P.n_elem * C::N + nz::BS - 1) / nz::BS);
         hipLaunchKernelGGL((nz::k_inverse_mass<C>), dim3(nb), dim3(nz::BS), 0, st, P, P.tables, R, out);
     }
-    static Ops ops() { return Ops{&dg, &limit, &inverse_mass, C::TE}; }
+    static void superbee(const nz_problem& P, double* U, double tvb, double area_total,
+                         double* work, unsigned long long* counters, hipStream_t st) {
+        const int64_t nb = (P.n_elem + nz::BS - 1) / nz::BS;
+        double* ubar = work;
+        double* pmax = work + P.n_elem * 4;
+        double* scale = pmax + nb * 4;
+        hipLaunchKernelGGL((nz::k_sb_means<C>), dim3(static_cast<unsigned>(nb)), dim3(nz::BS), 0, st,
+                           P, P.tables, U, ubar, pmax);
+        hipLaunchKernelGGL(nz::k_sb_scale, dim3(1), dim3(nz::BS), 0, st, pmax, nb, scale);
+        hipLaunchKernelGGL((nz::k_sb_apply<C>), dim3(static_cast<unsigned>(nb)), dim3(nz::BS), 0, st,
+                           P, P.tables, U, ubar, scale, tvb, area_total, counters);
+    }
+    static Ops ops() { return Ops{&dg, &limit, &inverse_mass, &superbee, C::TE}; }
 };
 
 // (N, nqv, nqf, nface, affine, straight) for p = 0..3 of each element family
@@ -818,6 +940,23 @@ int nz_limit(const nz_problem* p, double* U, uint8_t* flags, uint64_t* counters,
     o.limit(*p, U, flags, reinterpret_cast<unsigned long long*>(counters),
             static_cast<hipStream_t>(stream));
     return check_launch("nz_limit");
+}
+
+int64_t nz_superbee_work_doubles(const nz_problem* p) {
+    if (!p || p->n_elem <= 0) return fail(NZ_E_ARG, "empty mesh");
+    return p->n_elem * 4 + ((p->n_elem + nz::BS - 1) / nz::BS) * 4 + 4;
+}
+
+int nz_superbee(const nz_problem* p, double* U, double tvb, double area_total, double* work,
+                uint64_t* counters, void* stream) {
+    Ops o;
+    int rc = resolve(p, &o);
+    if (rc) return rc;
+    if (!U || !work) return fail(NZ_E_ARG, "null array");
+    if (!(area_total > 0.0)) return fail(NZ_E_ARG, "area_total must be positive");
+    o.superbee(*p, U, tvb, area_total, work, reinterpret_cast<unsigned long long*>(counters),
+               static_cast<hipStream_t>(stream));
+    return check_launch("nz_superbee");
 }
 
 int nz_time_step(const nz_problem* p, const double* wave, double factor, double* dt, void* stream) {

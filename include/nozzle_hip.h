@@ -70,7 +70,7 @@ typedef struct {
     /* flow (src/config.py:43-180; src/backends/numba_backend.py:69-109) */
     double gamma, efix, low_mach;
     int32_t flux_id;        /* 0 roe, 1 hllc, 2 ausmup */
-    int32_t limiter;        /* 0 none, 1 positivity (Zhang-Shu) */
+    int32_t limiter;        /* 0 none, 1 positivity (fused), 2 superbee then positivity (separate passes) */
     double rho_t_bf, p_t_bf, band_bf;
     double at2, at, rho_t, ca, sa, p_back;
     /* limiter (src/spatial/limiter.py:98-243) */
@@ -124,6 +124,15 @@ int nz_norm(const nz_problem *p, const double *A, double *scratch, double *out, 
 /* In-place Zhang-Shu positivity limiter (src/backends/_numba_kernels.py:927-1093,
  * src/spatial/limiter.py:181-243).  flags/counters as in nz_stage. */
 int nz_limit(const nz_problem *p, double *U, uint8_t *flags, uint64_t *counters, void *stream);
+
+/* Superbee TVB slope limiter, in place, without the positivity pass that
+ * apply_limiter runs after it (src/spatial/limiter.py:286-405,408-429).
+ * area_total: sum of element areas; work: nz_superbee_work_doubles(p)
+ * doubles; counters[0] (nullable) += elements whose slope was scaled.
+ * Elements with theta = 1 are left bit-identical. */
+int64_t nz_superbee_work_doubles(const nz_problem *p);
+int nz_superbee(const nz_problem *p, double *U, double tvb, double area_total, double *work,
+                uint64_t *counters, void *stream);
 
 /* dt_e = factor * 2 A_e / max(wave_e, 1e-10) (src/timestepping/step.py:171-183). */
 int nz_time_step(const nz_problem *p, const double *wave, double factor, double *dt,

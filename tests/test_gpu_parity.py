@@ -150,3 +150,32 @@ def test_unknown_inputs_fail_loudly():
         bk.run(golden("cfg1")["U0"], 1, "rk4")
     with pytest.raises(ValueError, match="unknown scheme"):
         bk.make_step("rk4")
+
+
+@pytest.mark.parametrize("tvb,key", [(0.0, "Ub_superbee"), (50.0, "Ub_superbee_tvb50")])
+def test_superbee_alone(tvb, key):
+    import ctypes as C
+
+    import torch
+
+    g = golden("cfg1")
+    ops, flow = build_case("cfg1", dense=False)
+    bk = backend(ops, flow, limiter="superbee", tvb_constant=tvb)
+    u = bk.asarray(g["Ub"]).clone()
+    rc = bk.lib.nz_superbee(bk._pp, C.c_void_p(u.data_ptr()), tvb, bk.area_total,
+                            C.c_void_p(bk._sb_work.data_ptr()), None, bk._stream())
+    assert rc == 0
+    torch.cuda.synchronize()
+    assert rel_err(u.cpu().numpy(), g[key]) <= 1e-13
+
+
+def test_superbee_then_positivity_and_steps():
+    from oracle.oracle import CpuOracle
+
+    g = golden("cfg1")
+    ops, flow = build_case("cfg1")
+    bk = backend(ops, flow, limiter="superbee", tvb_constant=0.0)
+    assert rel_err(bk.limit(g["Ub"]), g["Ub_apply_superbee"]) <= 1e-13
+    U, res = bk.run(g["Ub"], 3, "ssprk3")
+    Uo, reso = CpuOracle(ops, flow).run(g["Ub"], 3, "ssprk3", bk.dt_factor(), "superbee", 0.0)
+    assert rel_err(U, Uo) <= 1e-12 and abs(res / reso - 1) <= 1e-12
