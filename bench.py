@@ -50,30 +50,46 @@ def _traffic(workload: str):
     return data.get(workload, {}).get("bytes_per_launch")
 
 
-def cpu_baseline(order: int, ncx: int, ncy: int, budget_s: float) -> dict:
-    """The CPU oracle on a bounded sample: the same channel at ncx x ncy cells."""
+def cpu_baseline(order: int, ncx: int, ncy: int, budget_s: float, flux: str = "roe") -> dict:
+    """The CPU oracle on a bounded sample: the same channel at ncx x ncy cells,
+    timing the same unit of work (one residual evaluation -M^-1 R)."""
     from oracle.oracle import CpuOracle
 
     wl, cf = _pkg("workloads"), _pkg("config")
     ops, U = wl.channel_case(ncx, ncy, order, dense=True)
-    flow = cf.FlowConditions()
-    orc = CpuOracle(ops, flow)
-    fac = cf.step_coefficient(order, None, "ssprk3")
-    U, _ = orc.run(U, 1, "ssprk3", fac)  # warm-up
-    steps, t0 = 0, time.perf_counter()
+    orc = CpuOracle(ops, cf.FlowConditions(flux=flux))
+    orc.rate(U)  # warm-up
+    evals, t0 = 0, time.perf_counter()
     while time.perf_counter() - t0 < budget_s:
-        U, _ = orc.run(U, 1, "ssprk3", fac)
-        steps += 1
+        orc.rate(U)
+        evals += 1
     dt = time.perf_counter() - t0
     cores = len(os.sched_getaffinity(0))
     return {
-        "value": 3 * steps * ops.n_elem * ops.ref.n_basis / dt / 1e9,
+        "value": evals * ops.n_elem * ops.ref.n_basis / dt / 1e9,
         "unit": "Gdof-stage/s",
         "cores": int(os.environ.get("OMP_NUM_THREADS", cores)),
         "kind": "port",
-        "sample": f"{ncx}x{ncy}-cell channel, {ops.n_elem} P{order} triangles, {steps} SSP-RK3 "
-                  f"steps in {dt:.1f} s (C/OpenMP restatement of the reference CPU kernels)",
+        "sample": f"{ncx}x{ncy}-cell channel, {ops.n_elem} P{order} triangles, {evals} residual "
+                  f"evaluations ({flux}) in {dt:.1f} s with the C/OpenMP restatement of the "
+                  "reference's CPU kernels (edge_pass + element_rate)",
     }
+
+
+def _timed(fn, steps: int, warmup: int, stream) -> float:
+    """Seconds for ``steps`` calls of ``fn`` (HIP events on ``stream``)."""
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(steps):
+        fn()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    return ev0.elapsed_time(ev1) / 1e3
 
 
 def main() -> int:
@@ -85,7 +101,9 @@ def main() -> int:
     ap.add_argument("--ncy", type=int, default=512)
     ap.add_argument("--order", type=int, default=3)
     ap.add_argument("--flux", default="roe", choices=("roe", "hllc", "ausmup"))
+    ap.add_argument("--limiter", default="positivity", choices=("positivity", "none"))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-march", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -93,65 +111,93 @@ def main() -> int:
                           "are not part of this build", "n_gpus": args.gpus}))
         return 2
 
+    import ctypes as C
+
     import torch
 
     wl, cf, be = _pkg("workloads"), _pkg("config"), _pkg("backend")
     t_setup = time.perf_counter()
     ops, U = wl.channel_case(args.ncx, args.ncy, args.order)
-    flow = cf.FlowConditions(flux=args.flux)
-    bk = be.HipBackend(ops, flow, cf.SolverOptions())
-    print(f"setup {time.perf_counter() - t_setup:.1f} s: {ops.n_elem} elements, "
-          f"{ops.n_edges_total} edges, N={ops.ref.n_basis}", file=sys.stderr, flush=True)
     E, N = ops.n_elem, ops.ref.n_basis
-    cur, nxt = bk._S[0], bk._S[1]
-    cur.copy_(torch.from_numpy(U).to(bk.device))
+    print(f"setup {time.perf_counter() - t_setup:.1f} s: {E} elements, {ops.n_edges_total} edges, "
+          f"N={N}", file=sys.stderr, flush=True)
     st = torch.cuda.current_stream()
-    cur = bk.advance(cur, nxt, args.warmup, "ssprk3")
-    nxt = bk._S[1] if cur is bk._S[0] else bk._S[0]
-    torch.cuda.synchronize()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ev0.record(st)
-    cur = bk.advance(cur, nxt, args.steps, "ssprk3")
-    ev1.record(st)
-    torch.cuda.synchronize()
-    ms = ev0.elapsed_time(ev1)
-    res = float(bk._res.item())
-    t = ms / 1e3
-    gdof = 3 * args.steps * E * N / t / 1e9
-
     nb = wl.residual_bytes(ops)
-    launches = 3 * args.steps
-    per_launch_bytes = nb["step"] / 3.0
-    avg_launch_s = t / launches
-    achieved = per_launch_bytes / avg_launch_s / 1e9
-    workload = f"channel-tri-p{args.order}-{args.ncx}x{args.ncy}-{args.flux}"
+
+    # ---- the residual microbenchmark: one rate evaluation per step, per flux
+    per_flux = {}
+    u0 = None
+    for flux in ("roe", "hllc", "ausmup"):
+        bk = be.HipBackend(ops, cf.FlowConditions(flux=flux), cf.SolverOptions(limiter=args.limiter))
+        if u0 is None:
+            u0 = torch.from_numpy(U).to(bk.device)
+        out = torch.empty_like(u0)
+        wave = torch.empty(E, dtype=torch.float64, device=bk.device)
+
+        def rate(bk=bk, out=out, wave=wave):
+            bk._chk(bk.lib.nz_rate(bk._pp, C.c_void_p(u0.data_ptr()), C.c_void_p(out.data_ptr()),
+                                   C.c_void_p(wave.data_ptr()), bk._stream()), "nz_rate")
+
+        t = _timed(rate, args.steps, args.warmup, st)
+        per_flux[flux] = {"s_per_eval": t / args.steps, "bk": bk}
+        print(f"  residual {flux}: {t / args.steps * 1e6:.1f} us", file=sys.stderr, flush=True)
+    t_eval = per_flux[args.flux]["s_per_eval"]
+    gdof = E * N / t_eval / 1e9
+    achieved = nb["residual"] / t_eval / 1e9
+    workload = f"channel-tri-p{args.order}-{args.ncx}x{args.ncy}"
     out = {
-        "metric": "Gdof-stage/s FP64 (fused SSP-RK3 stage: residual + stage update + limiter)",
+        "metric": "Gdof-stage/s FP64 (one DG residual evaluation -M^-1 R(U) per stage, "
+                  "residual-kernel microbenchmark)",
         "value": round(gdof, 4),
         "unit": "Gdof-stage/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms / args.steps, 4),
+        "ms_per_step": round(t_eval * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": workload, "elements": E, "n_basis": N, "edges": ops.n_edges_total,
-                   "scheme": "ssprk3", "limiter": "positivity", "flux": args.flux},
+        "config": {"workload": workload, "elements": E, "n_basis": N,
+                   "edges": ops.n_edges_total, "flux": args.flux,
+                   "wall": "y=1+1.5(x-0.3)^2", "state_seed": 0},
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": _traffic(workload),
+            "traffic": _traffic(workload + "-" + args.flux),
         },
-        "residual_norm_last": res,
-        "kernel_avg_us": round(avg_launch_s * 1e6, 2),
-        "algorithmic_bytes_per_launch": int(per_launch_bytes),
+        "algorithmic_bytes_per_eval": nb["residual"],
+        "per_flux_gdof_stage_s": {f: round(E * N / v["s_per_eval"] / 1e9, 4) for f, v in per_flux.items()},
     }
+
+    # ---- the fused SSP-RK3 march on the same mesh and state
+    if not args.no_march:
+        bk = per_flux[args.flux]["bk"]
+        cur, nxt = bk._S[0], bk._S[1]
+        cur.copy_(u0)
+        bk._counters.zero_()
+        state = {"cur": cur, "nxt": nxt}
+
+        def step():
+            c = bk.advance(state["cur"], state["nxt"], 1, "ssprk3")
+            state["nxt"] = state["cur"] if c is not state["cur"] else state["nxt"]
+            state["cur"] = c
+
+        msteps = max(2, args.steps // 2)
+        t = _timed(step, msteps, 1, st)
+        cnt = bk._counters.cpu().tolist()
+        out["march"] = {
+            "gdof_stage_s": round(3 * msteps * E * N / t / 1e9, 4),
+            "ms_per_step": round(t / msteps * 1e3, 4),
+            "stage_kernel_hbm_frac": round(nb["step"] / (t / msteps) / 1e9 / HBM_PEAK_GBS, 4),
+            "limiter": args.limiter,
+            "elements_scaled_per_stage": round(cnt[0] / (3 * (msteps + 1)), 1),
+            "residual_norm_last": float(bk._res.item()),
+        }
     if not args.no_cpu:
         sample_cells = (64, 32) if args.order >= 2 else (256, 128)
         out["cpu_baseline"] = cpu_baseline(args.order, sample_cells[0], sample_cells[1],
