@@ -61,6 +61,10 @@ SIGNATURES = {
     "nz_limit": (C.c_int, [C.POINTER(Problem), P, P, P, P]),
     "nz_time_step": (C.c_int, [C.POINTER(Problem), P, C.c_double, P, P]),
     "nz_superbee_work_doubles": (C.c_int64, [C.POINTER(Problem)]),
+    "nz_adjoint_work_doubles": (C.c_int64, [C.POINTER(Problem)]),
+    "nz_residual_vjp": (C.c_int, [C.POINTER(Problem), P, P, P, P, P, P, P, P]),
+    "nz_adjoint_rk4_stage": (C.c_int, [C.POINTER(Problem), P, P, P, P, P, P, P, C.c_int, P, P]),
+    "nz_adjoint_defect": (C.c_int, [C.POINTER(Problem), P, P, P, P, P, P, P]),
     "nz_superbee": (C.c_int, [C.POINTER(Problem), P, C.c_double, C.c_double, P, P, P]),
     "nz_last_error": (C.c_char_p, []),
 }

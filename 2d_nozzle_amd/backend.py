@@ -420,3 +420,107 @@ class HipBackend:
             self.n_mean_repaired += int(c[1])
             self.n_limit_calls += n_steps * (1 if scheme == "euler" else 3)
         return self._out(out, npy), res
+
+    # ------------------------------------------------------------ adjoint
+    def _adj_buffers(self):
+        if getattr(self, "_adj", None) is None:
+            torch = self._torch
+            f64 = dict(dtype=torch.float64, device=self.device)
+            nw = int(self.lib.nz_adjoint_work_doubles(self._pp))
+            shape = (self.E, self.N, 4)
+            self._adj = {
+                "work": torch.empty(max(nw, 1), **f64),
+                "lam": [torch.empty(shape, **f64) for _ in range(3)],
+                "acc": torch.empty(shape, **f64),
+                "dt": torch.empty(self.E, **f64),
+                "res": torch.empty(1, **f64),
+            }
+        return self._adj
+
+    def residual_vjp(self, U, lam, *, geometry: bool = False):
+        """``(dR/dU)^T lam`` of the unfolded residual at ``U``.
+
+        With ``geometry=True`` also returns the cotangents of the per-point
+        metrics ``(E, nqv, 5)``, of the edge normals and Jacobians
+        ``(F, nqf, 3)`` and of the back pressure (a float).
+        """
+        torch = self._torch
+        u, npy = self._in(U)
+        l, _ = self._in(lam)
+        a = self._adj_buffers()
+        dU = torch.empty_like(u)
+        gm = ge = gp = None
+        if geometry:
+            rd = self.ops.ref
+            gm = torch.zeros((self.E, rd.n_qvol, 5), dtype=torch.float64, device=self.device)
+            ge = torch.zeros((self.ops.n_edges_total, rd.n_qface, 3), dtype=torch.float64,
+                             device=self.device)
+            gp = torch.zeros(max(self.ops.n_boundary * rd.n_qface, 1), dtype=torch.float64,
+                             device=self.device)
+        P = lambda t: self._p(t) if t is not None else None  # noqa: E731
+        self._chk(self.lib.nz_residual_vjp(self._pp, self._p(u), self._p(l), self._p(dU), P(gm),
+                                           P(ge), P(gp), self._p(a["work"]), self._stream()),
+                  "nz_residual_vjp")
+        if not geometry:
+            return self._out(dU, npy)
+        return (self._out(dU, npy), self._out(gm, npy), self._out(ge, npy), float(gp.sum().item()))
+
+    def adjoint_solve(self, U, seed, *, tolerance: float = 1e-8, max_iterations: int = 100_000,
+                      check_interval: int = 100, verbose: bool = False):
+        """Solve ``(dR/dU)^T lam = seed`` by the reference's pseudo-time march.
+
+        Classical RK4 with the frozen local step of the converged state,
+        sized with the ``ssprk3`` calibration and without the positivity
+        factor; convergence when ``rms(vjp(lam) - seed) <= tolerance *
+        rms(seed)``, tested every ``check_interval`` steps
+        (``src/sensitivity.py:289-383``).  Returns
+        ``(lam, relative_residual, iterations, converged)``.
+        """
+        if max_iterations < 1 or check_interval < 1:
+            raise ValueError("max_iterations and check_interval must be at least 1")
+        torch = self._torch
+        u, npy = self._in(U)
+        s, _ = self._in(seed)
+        a = self._adj_buffers()
+        st = self._stream()
+        # frozen step from the converged state
+        Fr = torch.empty_like(u)
+        wave = torch.empty(self.E, dtype=torch.float64, device=self.device)
+        self._chk(self.lib.nz_rate(self._pp, self._p(u), self._p(Fr), self._p(wave), st), "nz_rate")
+        fac = step_coefficient(self.ops.ref.order, self.opts.cfl, "ssprk3")
+        self._chk(self.lib.nz_time_step(self._pp, self._p(wave), fac, self._p(a["dt"]), st),
+                  "nz_time_step")
+        lam, tmp, nxt = a["lam"]
+        lam.zero_()
+        norm0 = float(torch.sqrt(torch.mean(s * s)).item())
+        if norm0 == 0.0:
+            return self._out(lam.clone(), npy), 0.0, 0, True
+        work, acc, dt = self._p(a["work"]), self._p(a["acc"]), self._p(a["dt"])
+        pu, ps = self._p(u), self._p(s)
+        it, res, converged = 0, norm0, False
+        while it < max_iterations:
+            steps = min(check_interval, max_iterations - it)
+            for _ in range(steps):
+                self._chk(self.lib.nz_adjoint_rk4_stage(self._pp, pu, self._p(lam), self._p(lam), ps,
+                                                        dt, acc, self._p(tmp), 1, work, st), "rk4")
+                self._chk(self.lib.nz_adjoint_rk4_stage(self._pp, pu, self._p(tmp), self._p(lam), ps,
+                                                        dt, acc, self._p(nxt), 2, work, st), "rk4")
+                self._chk(self.lib.nz_adjoint_rk4_stage(self._pp, pu, self._p(nxt), self._p(lam), ps,
+                                                        dt, acc, self._p(tmp), 3, work, st), "rk4")
+                self._chk(self.lib.nz_adjoint_rk4_stage(self._pp, pu, self._p(tmp), self._p(lam), ps,
+                                                        dt, acc, self._p(nxt), 4, work, st), "rk4")
+                lam, nxt = nxt, lam
+            it += steps
+            self._chk(self.lib.nz_adjoint_defect(self._pp, pu, self._p(lam), ps, work,
+                                                 self._p(self._scratch), self._p(a["res"]), st),
+                      "nz_adjoint_defect")
+            res = float(a["res"].item())
+            if verbose:
+                print(f"    adjoint {it:>7d}   {res:.4e}  ({res / norm0:.2e} relative)")
+            if not math.isfinite(res):
+                break
+            if res <= tolerance * norm0:
+                converged = True
+                break
+        a["lam"] = [lam, tmp, nxt]
+        return self._out(lam.clone(), npy), res / norm0, it, converged

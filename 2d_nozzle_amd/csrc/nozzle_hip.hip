@@ -59,6 +59,10 @@ struct Cfg {
     static constexpr int WORK = W_FACE > W_VOL ? (W_FACE > W_R ? W_FACE : W_R)
                                                : (W_VOL > W_R ? W_VOL : W_R);
     static constexpr int PF = 2 * NF * N * NQF;  // phi_face (and lift) table size
+    // adjoint work area: face round (neighbour U and lambda traces, cotangents)
+    static constexpr int A_FACE = 3 * NQF * 4 * TP;
+    static constexpr int AWORK = A_FACE > NS * TP ? (A_FACE > W_VOL ? A_FACE : W_VOL)
+                                                 : (NS * TP > W_VOL ? NS * TP : W_VOL);
 };
 
 // ---------------------------------------------------------------- limiter
@@ -531,7 +535,8 @@ __global__ __launch_bounds__(BS) void k_bflux(const nz_problem P, const double* 
     const int64_t k = P.n_int + b;
     const double* eg = C::STR ? P.edge_geo + k * 3 : P.edge_geo + (k * C::NQF + q) * 3;
     double fx[4];
-    const double sp = boundary_flux(P, tag, uo, eg[0], eg[1], fx);
+    const double nx = eg[0], ny = eg[1], pb = P.p_back;
+    const double sp = boundary_flux(P, tag, uo, nx, ny, pb, fx);
     const double sc = eg[2] * T[P.off.w_face + q];
     double* out = P.bflux + (b * C::NQF + q) * 5;
     for (int s = 0; s < 4; ++s) out[s] = fx[s] * sc;
@@ -673,6 +678,371 @@ __global__ __launch_bounds__(BS) void k_sb_apply(const nz_problem P, const doubl
     }
 }
 
+// ---------------------------------------------------------------- adjoint
+// Vector-Jacobian product of the unfolded residual, lambda^T dR/dU, and the
+// cotangents of the geometry it depends on (per-point metrics, edge normals
+// and Jacobians, back pressure).  Flux, boundary and Euler-flux Jacobians are
+// recomputed per point with forward-mode duals and contracted at once; no
+// Jacobian is stored.  Structure as k_dg: a tile of TE elements per
+// workgroup, the element's own U and lambda staged in LDS, the neighbours'
+// traces gathered from HBM, register accumulators per (element, basis).
+//
+// With R_i = -sum_q wdet_q [D^xi_iq (xi_x F + xi_y G) + D^eta_iq (eta_x F + eta_y G)]
+//            + sum_f sign sum_q phi_f,iq Fhat_q |dX/dsigma|_q w_q
+// (src/spatial/assembly.py:52-157), the VJP is
+//   volume: g_q = -(b^x_q . dF/dU + b^y_q . dG/dU), b^x = wdet (a^xi xi_x + a^eta eta_x),
+//           a^xi_s = sum_i lambda_is D^xi_iq; dU_j += sum_q phi_jq g_q;
+//   faces:  a = |dX/dsigma| w (Lambda_owner - Lambda_neighbour) at each point,
+//           dU_own,j += phi_own,jq a . dFhat/dU_own.
+
+enum AdjMode { kAdjVjp = 0, kAdjRk = 1, kAdjDefect = 2 };
+
+// Boundary faces: a . dF_b/dU_own per point, plus edge-geometry and
+// back-pressure cotangents.  One item per (boundary edge, point).
+template <class C>
+__global__ __launch_bounds__(BS) void k_bvjp(const nz_problem P, const double* __restrict__ T,
+                                             const double* __restrict__ U,
+                                             const double* __restrict__ L,
+                                             double* __restrict__ bv,
+                                             double* __restrict__ gedge,
+                                             double* __restrict__ gpb) {
+    const int64_t it = static_cast<int64_t>(blockIdx.x) * BS + threadIdx.x;
+    const int64_t nb = P.n_edge - P.n_int;
+    if (it >= nb * C::NQF) return;
+    const int64_t b = it / C::NQF;
+    const int q = static_cast<int>(it - b * C::NQF);
+    const int64_t e = P.bedge[2 * b];
+    const int ft = P.bedge[2 * b + 1];
+    const int f = ft >> 3, tag = ft & 7;
+    double uo[4] = {0.0, 0.0, 0.0, 0.0}, lo[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = 0; i < C::N; ++i) {
+        const double w = T[P.off.phi_face + (f * C::N + i) * C::NQF + q];
+        for (int s = 0; s < 4; ++s) {
+            uo[s] += w * U[(e * C::N + i) * 4 + s];
+            lo[s] += w * L[(e * C::N + i) * 4 + s];
+        }
+    }
+    const int64_t k = P.n_int + b;
+    const double* eg = C::STR ? P.edge_geo + k * 3 : P.edge_geo + (k * C::NQF + q) * 3;
+    using D = Dual<7>;
+    D ud[4];
+    for (int s = 0; s < 4; ++s) ud[s] = D::seed(uo[s], s);
+    const D nx = D::seed(eg[0], 4), ny = D::seed(eg[1], 5), pb = D::seed(P.p_back, 6);
+    D fx[4];
+    boundary_flux(P, tag, ud, nx, ny, pb, fx);
+    const double wq = T[P.off.w_face + q];
+    const double jw = eg[2] * wq;
+    double v[7] = {0, 0, 0, 0, 0, 0, 0}, gj = 0.0;
+    for (int s = 0; s < 4; ++s) {
+        const double a = jw * lo[s];
+        for (int d = 0; d < 7; ++d) v[d] += a * fx[s].d[d];
+        gj += lo[s] * fx[s].v;
+    }
+    for (int d = 0; d < 4; ++d) bv[(b * C::NQF + q) * 4 + d] = v[d];
+    if (gedge) {
+        double* ge = gedge + (k * C::NQF + q) * 3;
+        ge[0] = v[4];
+        ge[1] = v[5];
+        ge[2] = gj * wq;
+    }
+    if (gpb) gpb[b * C::NQF + q] = v[6];
+}
+
+template <class C, int FLUX>
+__global__ __launch_bounds__(BS) void k_vjp(const nz_problem P, const double* __restrict__ T,
+                                            const double* __restrict__ U,
+                                            const double* __restrict__ Lin,
+                                            const double* __restrict__ bv,
+                                            double* __restrict__ out, double* __restrict__ gmet,
+                                            double* __restrict__ gedge, int mode,
+                                            const double* __restrict__ seed,
+                                            const double* __restrict__ Lbase,
+                                            const double* __restrict__ dt,
+                                            double* __restrict__ acc_rk, double* __restrict__ Lnext,
+                                            int rk_stage) {
+    constexpr int N = C::N, NQV = C::NQV, NQF = C::NQF, NF = C::NF, NS = C::NS;
+    constexpr int TE = C::TE, TP = C::TP, SLOTS = C::SLOTS;
+    __shared__ double sU[NS * TP];
+    __shared__ double sL[NS * TP];
+    __shared__ double sW[C::AWORK];
+    __shared__ double sPF[C::PF];
+    __shared__ double sG[5 * TE];
+    __shared__ int sSide[NF * TE];
+
+    const int t = threadIdx.x;
+    const int el = t % TE;
+    const int slot = wave_slot() / (TE / 64);
+    const int64_t e0 = static_cast<int64_t>(blockIdx.x) * TE;
+    const int ne = static_cast<int>(P.n_elem - e0 < TE ? P.n_elem - e0 : TE);
+    const bool live = el < ne;
+    const int64_t e = e0 + el;
+
+    for (int it = t; it < TE * NS; it += BS) {
+        const int r = it / NS, c = it - r * NS;
+        sU[c * TP + r] = r < ne ? U[(e0 + r) * NS + c] : 0.0;
+        sL[c * TP + r] = r < ne ? Lin[(e0 + r) * NS + c] : 0.0;
+    }
+    for (int it = t; it < C::PF; it += BS) sPF[it] = T[P.off.phi_face + it];
+    if (t < TE && live) {
+        if (C::AFF)
+            for (int k = 0; k < 5; ++k) sG[k * TE + t] = P.geo[e * 5 + k];
+        for (int f = 0; f < NF; ++f) sSide[f * TE + t] = (P.fe_info[e * NF + f] >> 2) & 1;
+    }
+    __syncthreads();
+
+    double acc[C::NIT][4];
+#pragma unroll
+    for (int k = 0; k < C::NIT; ++k)
+        for (int s = 0; s < 4; ++s) acc[k][s] = 0.0;
+
+    // ---------------------------------------------------------------- faces
+    double* sTn = sW;                     // [NQF][4][TP] neighbour U trace
+    double* sLn = sW + NQF * 4 * TP;      // [NQF][4][TP] neighbour lambda trace
+    double* sV = sW + 2 * NQF * 4 * TP;   // [NQF][4][TP] a . dFhat/dU_own
+    for (int f = 0; f < NF; ++f) {
+        for (int s = slot; s < 4; s += SLOTS) {
+            double tu[NQF], tl[NQF];
+#pragma unroll
+            for (int q = 0; q < NQF; ++q) tu[q] = tl[q] = 0.0;
+            if (live) {
+                const int nb = P.fe_nbr[e * NF + f];
+                if (nb >= 0) {
+                    const int info = P.fe_info[e * NF + f];
+                    const int of = info & 3, oside = 1 - ((info >> 2) & 1);
+                    const double* ur = U + static_cast<int64_t>(nb) * NS + s;
+                    const double* lr = Lin + static_cast<int64_t>(nb) * NS + s;
+                    const double* ph = sPF + (oside * NF + of) * N * NQF;
+                    for (int i = 0; i < N; ++i) {
+                        const double x = ur[i * 4], y = lr[i * 4];
+#pragma unroll
+                        for (int q = 0; q < NQF; ++q) {
+                            tu[q] += ph[i * NQF + q] * x;
+                            tl[q] += ph[i * NQF + q] * y;
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < NQF; ++q) {
+                sTn[(q * 4 + s) * TP + el] = tu[q];
+                sLn[(q * 4 + s) * TP + el] = tl[q];
+            }
+        }
+        __syncthreads();
+        for (int q = slot; q < NQF; q += SLOTS) {
+            if (live) {
+                const int nb = P.fe_nbr[e * NF + f];
+                const int info = P.fe_info[e * NF + f];
+                const int side = (info >> 2) & 1;
+                const int64_t k = info >> 3;
+                double v[4];
+                if (nb >= 0) {
+                    const double* ph = sPF + (side * NF + f) * N * NQF + q;
+                    double uo[4] = {0.0, 0.0, 0.0, 0.0}, lo[4] = {0.0, 0.0, 0.0, 0.0};
+                    for (int i = 0; i < N; ++i) {
+                        const double b = ph[i * NQF];
+                        for (int s = 0; s < 4; ++s) {
+                            uo[s] += b * sU[(i * 4 + s) * TP + el];
+                            lo[s] += b * sL[(i * 4 + s) * TP + el];
+                        }
+                    }
+                    const double* eg = C::STR ? P.edge_geo + k * 3 : P.edge_geo + (k * NQF + q) * 3;
+                    const double wq = T[P.off.w_face + q];
+                    const double jw = eg[2] * wq;
+                    using D = Dual<6>;
+                    D ul[4], ur[4];
+                    double dl[4];  // Lambda_owner - Lambda_neighbour
+                    for (int s = 0; s < 4; ++s) {
+                        const double un = sTn[(q * 4 + s) * TP + el];
+                        const double ln = sLn[(q * 4 + s) * TP + el];
+                        if (side == 0) {
+                            ul[s] = D::seed(uo[s], s);
+                            ur[s] = D(un);
+                            dl[s] = lo[s] - ln;
+                        } else {
+                            ul[s] = D(un);
+                            ur[s] = D::seed(uo[s], s);
+                            dl[s] = ln - lo[s];
+                        }
+                    }
+                    const D nx = D::seed(eg[0], 4), ny = D::seed(eg[1], 5);
+                    D fx[4];
+                    interface_flux<FLUX>(P, ul, ur, nx, ny, fx);
+                    double g[6] = {0, 0, 0, 0, 0, 0}, gj = 0.0;
+                    for (int s = 0; s < 4; ++s) {
+                        const double a = jw * dl[s];
+                        for (int d = 0; d < 6; ++d) g[d] += a * fx[s].d[d];
+                        gj += dl[s] * fx[s].v;
+                    }
+                    for (int d = 0; d < 4; ++d) v[d] = g[d];
+                    if (gedge && side == 0) {
+                        double* ge = gedge + (k * NQF + q) * 3;
+                        ge[0] = g[4];
+                        ge[1] = g[5];
+                        ge[2] = gj * wq;
+                    }
+                } else {
+                    const double* b = bv + ((k - P.n_int) * NQF + q) * 4;
+                    for (int d = 0; d < 4; ++d) v[d] = b[d];
+                }
+                for (int d = 0; d < 4; ++d) sV[(q * 4 + d) * TP + el] = v[d];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < C::NIT; ++kk) {
+            const int i = slot + kk * SLOTS;
+            if (i < N && live) {
+                const int side = sSide[f * TE + el];
+                for (int q = 0; q < NQF; ++q) {
+                    const double b0 = T[P.off.phi_face + ((0 * NF + f) * N + i) * NQF + q];
+                    const double b1 = T[P.off.phi_face + ((1 * NF + f) * N + i) * NQF + q];
+                    const double b = side ? b1 : b0;
+                    for (int d = 0; d < 4; ++d) acc[kk][d] += b * sV[(q * 4 + d) * TP + el];
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // --------------------------------------------------------------- volume
+    double* sGq = sW;  // [QR][4][TP] cotangent on the point state
+    for (int q0 = 0; q0 < NQV; q0 += C::QR) {
+        {
+            const int qq = slot, q = q0 + qq;
+            if (q < NQV && live) {
+                double uq[4] = {0, 0, 0, 0}, ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0};
+                for (int i = 0; i < N; ++i) {
+                    const double b = T[P.off.phi_vol + i * NQV + q];
+                    const double dx = T[P.off.dxi + i * NQV + q], dy = T[P.off.deta + i * NQV + q];
+                    for (int s = 0; s < 4; ++s) {
+                        uq[s] += b * sU[(i * 4 + s) * TP + el];
+                        const double l = sL[(i * 4 + s) * TP + el];
+                        ax[s] += l * dx;
+                        ay[s] += l * dy;
+                    }
+                }
+                double xx, xy, ex, ey, wd;
+                if (C::AFF) {
+                    xx = sG[el];
+                    xy = sG[TE + el];
+                    ex = sG[2 * TE + el];
+                    ey = sG[3 * TE + el];
+                    wd = T[P.off.w_vol + q] * sG[4 * TE + el];
+                } else {
+                    const double* gq = P.geo + (e * NQV + q) * 5;
+                    xx = gq[0];
+                    xy = gq[1];
+                    ex = gq[2];
+                    ey = gq[3];
+                    wd = gq[4];
+                }
+                using D = Dual<4>;
+                D ud[4];
+                for (int s = 0; s < 4; ++s) ud[s] = D::seed(uq[s], s);
+                const Prim<D> w = primitives(ud, P.gamma);
+                const D F[4] = {w.r * w.u, w.r * w.u * w.u + w.p, w.r * w.u * w.v, w.r * w.u * w.H};
+                const D G[4] = {w.r * w.v, w.r * w.u * w.v, w.r * w.v * w.v + w.p, w.r * w.v * w.H};
+                double g[4] = {0, 0, 0, 0};
+                double gm[5] = {0, 0, 0, 0, 0};
+                for (int s = 0; s < 4; ++s) {
+                    const double bx = wd * (ax[s] * xx + ay[s] * ex);
+                    const double by = wd * (ax[s] * xy + ay[s] * ey);
+                    for (int d = 0; d < 4; ++d) g[d] -= bx * F[s].d[d] + by * G[s].d[d];
+                    gm[0] -= wd * ax[s] * F[s].v;
+                    gm[1] -= wd * ax[s] * G[s].v;
+                    gm[2] -= wd * ay[s] * F[s].v;
+                    gm[3] -= wd * ay[s] * G[s].v;
+                    gm[4] -= ax[s] * (xx * F[s].v + xy * G[s].v) + ay[s] * (ex * F[s].v + ey * G[s].v);
+                }
+                for (int d = 0; d < 4; ++d) sGq[(qq * 4 + d) * TP + el] = g[d];
+                if (gmet)
+                    for (int d = 0; d < 5; ++d) gmet[(e * NQV + q) * 5 + d] = gm[d];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < C::NIT; ++kk) {
+            const int i = slot + kk * SLOTS;
+            if (i < N && live) {
+                for (int qq = 0; qq < C::QR && q0 + qq < NQV; ++qq) {
+                    const double b = T[P.off.phi_vol + i * NQV + q0 + qq];
+                    for (int d = 0; d < 4; ++d) acc[kk][d] += b * sGq[(qq * 4 + d) * TP + el];
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    if (mode == kAdjVjp) {
+#pragma unroll
+        for (int kk = 0; kk < C::NIT; ++kk) {
+            const int i = slot + kk * SLOTS;
+            if (i < N && live)
+                for (int s = 0; s < 4; ++s) out[(e * N + i) * 4 + s] = acc[kk][s];
+        }
+        return;
+    }
+    if (mode == kAdjDefect) {  // partial sums of (vjp - seed)^2; `out` holds one per block
+        __shared__ double sRed[BS / 64];
+        double sq = 0.0;
+#pragma unroll
+        for (int kk = 0; kk < C::NIT; ++kk) {
+            const int i = slot + kk * SLOTS;
+            if (i < N && live)
+                for (int s = 0; s < 4; ++s) {
+                    const double d = acc[kk][s] - seed[(e * N + i) * 4 + s];
+                    sq += d * d;
+                }
+        }
+        const double tot = block_sum(sq, sRed);
+        if (t == 0) out[blockIdx.x] = tot;
+        return;
+    }
+
+    // adjoint pseudo-time rate -M^-1 (vjp - seed) and one classical RK4 stage
+    // (src/sensitivity.py:344-357)
+    double* sR = sW;
+#pragma unroll
+    for (int kk = 0; kk < C::NIT; ++kk) {
+        const int i = slot + kk * SLOTS;
+        if (i < N)
+            for (int s = 0; s < 4; ++s)
+                sR[(i * 4 + s) * TP + el] = live ? acc[kk][s] - seed[(e * N + i) * 4 + s] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < C::NIT; ++kk) {
+        const int i = slot + kk * SLOTS;
+        if (i < N && live) {
+            double r[4] = {0, 0, 0, 0};
+            for (int j = 0; j < N; ++j) {
+                const double m = C::AFF ? T[P.off.minv_ref + i * N + j] / sG[4 * TE + el]
+                                        : P.minv[(e * N + i) * N + j];
+                for (int s = 0; s < 4; ++s) r[s] += m * sR[(j * 4 + s) * TP + el];
+            }
+            const double h = dt[e];
+            for (int s = 0; s < 4; ++s) {
+                const int64_t x = (e * N + i) * 4 + s;
+                const double kv = -r[s];
+                const double lb = Lbase[x];
+                if (rk_stage == 1) {
+                    acc_rk[x] = kv;
+                    Lnext[x] = lb + 0.5 * h * kv;
+                } else if (rk_stage == 2) {
+                    acc_rk[x] = acc_rk[x] + 2.0 * kv;
+                    Lnext[x] = lb + 0.5 * h * kv;
+                } else if (rk_stage == 3) {
+                    acc_rk[x] = acc_rk[x] + 2.0 * kv;
+                    Lnext[x] = lb + h * kv;
+                } else {
+                    Lnext[x] = lb + h / 6.0 * (acc_rk[x] + kv);
+                }
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------------ small kernels
 template <class C>
 __global__ __launch_bounds__(BS) void k_inverse_mass(const nz_problem P, const double* __restrict__ T,
@@ -754,6 +1124,9 @@ struct Ops {
     void (*inverse_mass)(const nz_problem&, const double*, double*, hipStream_t);
     void (*superbee)(const nz_problem&, double*, double, double, double*,
                      unsigned long long*, hipStream_t);
+    void (*vjp)(const nz_problem&, const double*, const double*, double*, double*, double*,
+                double*, double*, int, const double*, const double*, const double*, double*,
+                double*, int, int64_t, hipStream_t);
     int te;
 };
 
@@ -799,7 +1172,24 @@ struct Launch {
         hipLaunchKernelGGL((nz::k_sb_apply<C>), dim3(static_cast<unsigned>(nb)), dim3(nz::BS), 0, st,
                            P, P.tables, U, ubar, scale, tvb, area_total, counters);
     }
-    static Ops ops() { return Ops{&dg, &limit, &inverse_mass, &superbee, C::TE}; }
+    static void vjp(const nz_problem& P, const double* U, const double* L, double* out,
+                    double* gmet, double* gedge, double* gpb, double* bv, int mode,
+                    const double* seed, const double* Lbase, const double* dt, double* acc,
+                    double* Lnext, int stage, int64_t nblk, hipStream_t st) {
+        const int64_t nbq = (P.n_edge - P.n_int) * C::NQF;
+        if (nbq > 0)
+            hipLaunchKernelGGL((nz::k_bvjp<C>), dim3(static_cast<unsigned>((nbq + nz::BS - 1) / nz::BS)),
+                               dim3(nz::BS), 0, st, P, P.tables, U, L, bv, gedge, gpb);
+        const dim3 g(static_cast<unsigned>(nblk));
+#define NZ_VJP(FL)                                                                             \
+    hipLaunchKernelGGL((nz::k_vjp<C, FL>), g, dim3(nz::BS), 0, st, P, P.tables, U, L, bv, out, \
+                       gmet, gedge, mode, seed, Lbase, dt, acc, Lnext, stage)
+        if (P.flux_id == 1) NZ_VJP(1);
+        else if (P.flux_id == 2) NZ_VJP(2);
+        else NZ_VJP(0);
+#undef NZ_VJP
+    }
+    static Ops ops() { return Ops{&dg, &limit, &inverse_mass, &superbee, &vjp, C::TE}; }
 };
 
 // (N, nqv, nqf, nface, affine, straight) for p = 0..3 of each element family
@@ -940,6 +1330,54 @@ int nz_limit(const nz_problem* p, double* U, uint8_t* flags, uint64_t* counters,
     o.limit(*p, U, flags, reinterpret_cast<unsigned long long*>(counters),
             static_cast<hipStream_t>(stream));
     return check_launch("nz_limit");
+}
+
+int64_t nz_adjoint_work_doubles(const nz_problem* p) {
+    if (!p || p->n_elem <= 0) return fail(NZ_E_ARG, "empty mesh");
+    const int64_t nb = p->n_edge - p->n_int;
+    return (nb > 0 ? nb : 1) * p->nqf * 4;
+}
+
+int nz_residual_vjp(const nz_problem* p, const double* U, const double* lam, double* dU,
+                    double* gmet, double* gedge, double* gpb, double* work, void* stream) {
+    Ops o;
+    int rc = resolve(p, &o);
+    if (rc) return rc;
+    if (!U || !lam || !dU || !work) return fail(NZ_E_ARG, "null array");
+    o.vjp(*p, U, lam, dU, gmet, gedge, gpb, work, nz::kAdjVjp, nullptr, nullptr, nullptr, nullptr,
+          nullptr, 0, n_blocks(p, o), static_cast<hipStream_t>(stream));
+    return check_launch("nz_residual_vjp");
+}
+
+int nz_adjoint_rk4_stage(const nz_problem* p, const double* U, const double* lam_in,
+                         const double* lam_base, const double* seed, const double* dt,
+                         double* acc, double* lam_next, int stage, double* work, void* stream) {
+    Ops o;
+    int rc = resolve(p, &o);
+    if (rc) return rc;
+    if (!U || !lam_in || !lam_base || !seed || !dt || !acc || !lam_next || !work)
+        return fail(NZ_E_ARG, "null array");
+    if (stage < 1 || stage > 4) return fail(NZ_E_ARG, "stage must be 1..4");
+    if (lam_next == lam_in || lam_next == lam_base)
+        return fail(NZ_E_ARG, "lam_next must not alias lam_in or lam_base");
+    o.vjp(*p, U, lam_in, nullptr, nullptr, nullptr, nullptr, work, nz::kAdjRk, seed, lam_base, dt,
+          acc, lam_next, stage, n_blocks(p, o), static_cast<hipStream_t>(stream));
+    return check_launch("nz_adjoint_rk4_stage");
+}
+
+int nz_adjoint_defect(const nz_problem* p, const double* U, const double* lam, const double* seed,
+                      double* work, double* scratch, double* out, void* stream) {
+    Ops o;
+    int rc = resolve(p, &o);
+    if (rc) return rc;
+    if (!U || !lam || !seed || !work || !scratch || !out) return fail(NZ_E_ARG, "null array");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int64_t nb = n_blocks(p, o);
+    o.vjp(*p, U, lam, scratch, nullptr, nullptr, nullptr, work, nz::kAdjDefect, seed, nullptr,
+          nullptr, nullptr, nullptr, 0, nb, st);
+    hipLaunchKernelGGL(k_norm_finish, dim3(1), dim3(nz::BS), 0, st, scratch, nb,
+                       static_cast<double>(p->n_elem) * p->nbf * 4.0, out);
+    return check_launch("nz_adjoint_defect");
 }
 
 int64_t nz_superbee_work_doubles(const nz_problem* p) {

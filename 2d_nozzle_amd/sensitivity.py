@@ -1,0 +1,251 @@
+"""Exact discrete-adjoint design gradients on the device.
+
+Restates the reference's ``DifferentiableCase`` (``src/sensitivity.py:134-460``)
+around the hand-written residual VJP:
+
+1. forward solve on a frozen logical grid (``case_from_grids``,
+   ``src/api.py:213-244``), warm-started by the caller;
+2. the objective ``J(U, X)`` and its partials on the host (boundary integrals
+   of the owner trace, ``src/post/postprocess.py:56-75,155-171``);
+3. ``(dR/dU)^T lambda = dJ/dU`` by the RK4 pseudo-time march of
+   ``HipBackend.adjoint_solve``;
+4. ``grad = dJ/da - lambda^T dR/da``: the device returns the cotangents of
+   the metrics, edge normals/Jacobians and back pressure; they are chained to
+   node coordinates here (reverse of ``build_operators``,
+   ``src/spatial/operators.py:244-305``) and to design variables through the
+   node map ``nozzle_node_coords`` by central differences of that closed-form
+   map (the flow is not re-solved).
+"""
+
+from __future__ import annotations
+
+from collections.abc import Mapping, Sequence
+from dataclasses import dataclass, field
+from typing import Any
+
+import numpy as np
+
+from .config import FlowConditions, SolverOptions
+from .mesh import (BoundaryTag, NozzleGeometry, build_nozzle_mesh, nozzle_node_coords)
+from .operators import build_operators
+
+DESIGN_PARAMETERS = ("area_ratio", "throat_x", "throat_half_height", "inlet_half_height",
+                     "theta_initial", "theta_exit", "bezier_w1", "bezier_w2", "length")
+OBJECTIVES = ("thrust", "mass_flow")
+
+
+# ------------------------------------------------------------------ geometry chain
+def metric_vjp(ops, gmet: np.ndarray) -> np.ndarray:
+    """Node-coordinate cotangent from per-point metric cotangents.
+
+    ``gmet[e, q] = d/d(xi_x, xi_y, eta_x, eta_y, w det)``; the forward map is
+    ``J = [[x_xi, x_eta], [y_xi, y_eta]] = X^T dgeom``, ``det = x_xi y_eta -
+    x_eta y_xi``, ``xi_x = y_eta/det``, ``xi_y = -x_eta/det``,
+    ``eta_x = -y_xi/det``, ``eta_y = x_xi/det``, ``w det``.
+    """
+    rd = ops.ref
+    topo = ops.topology
+    X = np.asarray(ops.node_coords)
+    Xe = X[topo.elem_nodes]  # (E, NQ, 2)
+    x_xi = Xe[:, :, 0] @ rd.dgeom_dxi_vol
+    x_eta = Xe[:, :, 0] @ rd.dgeom_deta_vol
+    y_xi = Xe[:, :, 1] @ rd.dgeom_dxi_vol
+    y_eta = Xe[:, :, 1] @ rd.dgeom_deta_vol
+    det = x_xi * y_eta - x_eta * y_xi
+    xix, xiy, etx, ety = y_eta / det, -x_eta / det, -y_xi / det, x_xi / det
+    g = np.asarray(gmet)
+    gxix, gxiy, getx, gety, gwd = (g[..., k] for k in range(5))
+    gdet = rd.w_vol[None, :] * gwd - (gxix * xix + gxiy * xiy + getx * etx + gety * ety) / det
+    g_yeta = gxix / det + gdet * x_xi
+    g_xeta = -gxiy / det - gdet * y_xi
+    g_yxi = -getx / det - gdet * x_eta
+    g_xxi = gety / det + gdet * y_eta
+    gXe = np.empty_like(Xe)
+    gXe[:, :, 0] = g_xxi @ rd.dgeom_dxi_vol.T + g_xeta @ rd.dgeom_deta_vol.T
+    gXe[:, :, 1] = g_yxi @ rd.dgeom_dxi_vol.T + g_yeta @ rd.dgeom_deta_vol.T
+    gX = np.zeros_like(X)
+    np.add.at(gX, topo.elem_nodes.reshape(-1), gXe.reshape(-1, 2))
+    return gX
+
+
+def edge_vjp(ops, gedge: np.ndarray) -> np.ndarray:
+    """Node-coordinate cotangent from ``d/d(n_x, n_y, |dX/dsigma|)`` per edge point.
+
+    Forward: ``t = sum_n dgeom_dsigma[f, n, q] X_n`` on the owner,
+    ``jac = |t|``, ``n = (t_y, -t_x) / jac`` (``operators.py:293-302``).
+    """
+    rd = ops.ref
+    topo = ops.topology
+    ed = topo.edges
+    X = np.asarray(ops.node_coords)
+    own_e = np.concatenate([ed.iedge_elem[:, 0], ed.bedge_elem])
+    own_f = np.concatenate([ed.iedge_face[:, 0], ed.bedge_face])
+    nodes = topo.elem_nodes[own_e]  # (F, NQ)
+    dg = rd.dgeom_dsigma_face[own_f]  # (F, NQ, nqf)
+    Xo = X[nodes]
+    tx = np.einsum("enq,en->eq", dg, Xo[:, :, 0])
+    ty = np.einsum("enq,en->eq", dg, Xo[:, :, 1])
+    jac = np.sqrt(tx * tx + ty * ty)
+    j3 = jac**3
+    g = np.asarray(gedge)
+    gnx, gny, gj = g[..., 0], g[..., 1], g[..., 2]
+    gtx = gnx * (-ty * tx / j3) + gny * (-1.0 / jac + tx * tx / j3) + gj * tx / jac
+    gty = gnx * (1.0 / jac - ty * ty / j3) + gny * (tx * ty / j3) + gj * ty / jac
+    gXo = np.stack([np.einsum("enq,eq->en", dg, gtx), np.einsum("enq,eq->en", dg, gty)], axis=-1)
+    gX = np.zeros_like(X)
+    np.add.at(gX, nodes.reshape(-1), gXo.reshape(-1, 2))
+    return gX
+
+
+# ------------------------------------------------------------------ objectives
+def _trace(U, ops, tag):
+    topo = ops.topology
+    ed = topo.edges
+    sl = topo.tag_slice(tag)
+    phi = ops.ref.phi_face[0][ed.bedge_face[sl]]  # (n, N, nqf)
+    S = np.einsum("kiq,kis->kqs", phi, np.asarray(U)[ed.bedge_elem[sl]])
+    g = ops.n_interior + np.arange(sl.start, sl.stop)
+    return S, phi, ed.bedge_elem[sl], g
+
+
+def objective_and_partials(U, ops, flow, name: str = "thrust"):
+    """``J``, ``dJ/dU`` ``(E, N, 4)`` and ``dJ/d(edge geometry)`` ``(F, nqf, 3)``.
+
+    ``thrust = 2 sum_{out, in} int (rho v_x v_n + p n_x) ds``;
+    ``mass_flow = -2 int_in rho v_n ds`` (``src/sensitivity.py:96-130``).
+    """
+    if name not in OBJECTIVES:
+        raise ValueError(f"unknown objective {name!r}; choose from {OBJECTIVES}")
+    g = flow.gamma
+    w = ops.ref.w_face
+    J = 0.0
+    dU = np.zeros(np.asarray(U).shape)
+    dG = np.zeros((ops.n_edges_total, ops.ref.n_qface, 3))
+    tags = (BoundaryTag.OUTFLOW, BoundaryTag.INFLOW) if name == "thrust" else (BoundaryTag.INFLOW,)
+    fac = 2.0 if name == "thrust" else -2.0
+    for tag in tags:
+        S, phi, belem, gidx = _trace(U, ops, tag)
+        if S.size == 0:
+            continue
+        nx = ops.edge_normal[gidx][..., 0]
+        ny = ops.edge_normal[gidx][..., 1]
+        jac = ops.edge_jac[gidx]
+        ds = jac * w[None, :]
+        rho_raw = S[..., 0]
+        rho = np.maximum(rho_raw, 1e-10)
+        drho = (rho_raw > 1e-10).astype(float)
+        m1, m2, E = S[..., 1], S[..., 2], S[..., 3]
+        ke = 0.5 * (m1 * m1 + m2 * m2) / rho
+        p_raw = (g - 1.0) * (E - ke)
+        p = np.maximum(p_raw, 1e-10)
+        dp_on = (p_raw > 1e-10).astype(float)
+        vn_num = m1 * nx + m2 * ny  # rho v_n
+        # d p / d S, with the density floor
+        dp = np.stack([(g - 1.0) * ke / rho * drho, -(g - 1.0) * m1 / rho, -(g - 1.0) * m2 / rho,
+                       np.full_like(rho, g - 1.0)], axis=-1) * dp_on[..., None]
+        if name == "thrust":
+            I = m1 * vn_num / rho + p * nx
+            dI = np.stack([-m1 * vn_num / rho**2 * drho, (vn_num + m1 * nx) / rho, m1 * ny / rho,
+                           np.zeros_like(rho)], axis=-1) + dp * nx[..., None]
+            dG[gidx, :, 0] += fac * (m1 * m1 / rho + p) * ds
+            dG[gidx, :, 1] += fac * (m1 * m2 / rho) * ds
+        else:
+            I = vn_num
+            dI = np.stack([np.zeros_like(rho), nx, ny, np.zeros_like(rho)], axis=-1)
+            dG[gidx, :, 0] += fac * m1 * ds
+            dG[gidx, :, 1] += fac * m2 * ds
+        J += fac * float((I * ds).sum())
+        dG[gidx, :, 2] += fac * I * w[None, :]
+        gS = fac * dI * ds[..., None]  # (n, nqf, 4)
+        np.add.at(dU, belem, np.einsum("kiq,kqs->kis", phi, gS))
+    return J, dU, dG
+
+
+# ------------------------------------------------------------------ the case
+@dataclass
+class DifferentiableCase:
+    """A nozzle whose objective can be differentiated w.r.t. design variables.
+
+    The logical grid (axial and transverse node fractions) is frozen at the
+    base design, as in the reference, so that a perturbed design moves the
+    nodes through ``nozzle_node_coords`` only.
+    """
+
+    geometry: NozzleGeometry = field(default_factory=lambda: NozzleGeometry(contour="smooth"))
+    flow: FlowConditions = field(default_factory=FlowConditions)
+    order: int = 1
+    refine: int = 0
+    kind: str = "tri"
+    geometry_order: int = 1
+    options: SolverOptions = field(default_factory=lambda: SolverOptions(tolerance=1e-11,
+                                                                         print_interval=0))
+
+    def __post_init__(self) -> None:
+        topo, coords, grids = build_nozzle_mesh(self.geometry, kind=self.kind, refine=self.refine,
+                                                geometry_order=self.geometry_order)
+        self.topology = topo
+        self.x_frac, self.r_frac = grids["x_frac"], grids["r_frac"]
+
+    def default_params(self) -> dict[str, float]:
+        d = dict(self.geometry.params())
+        d["back_pressure"] = float(self.flow.back_pressure)
+        return d
+
+    def _merged(self, params: Mapping[str, Any] | None) -> dict[str, float]:
+        d = self.default_params()
+        for k, v in (params or {}).items():
+            if k not in d:
+                raise ValueError(f"{k!r} is not a design parameter")
+            d[k] = float(v)
+        return d
+
+    def coords(self, params: Mapping[str, Any] | None = None) -> np.ndarray:
+        return nozzle_node_coords(self.x_frac, self.r_frac, self._merged(params),
+                                  self.geometry.contour)
+
+    def flow_at(self, params=None) -> FlowConditions:
+        pb = self._merged(params)["back_pressure"]
+        return self.flow.replace(back_pressure_ratio=pb / self.flow.total_pressure)
+
+    def operators(self, params=None):
+        return build_operators(self.topology, self.coords(params), self.order, dense=False)
+
+    def solve_forward(self, params=None, *, U0, backend=None):
+        from .march import solve_steady
+
+        ops = self.operators(params)
+        return solve_steady(ops, self.flow_at(params), self.options, U0=U0, backend=backend or "hip")
+
+    def value_and_gradient(self, objective: str = "thrust", params=None, *,
+                           names: Sequence[str] | None = None, U0, adjoint_tolerance: float = 1e-8,
+                           fd_step: float = 1e-6, verbose: bool = False):
+        """``(J, {name: dJ/dname}, forward_result, adjoint_info)``."""
+        from .backend import HipBackend
+
+        base = self._merged(params)
+        names = tuple(names) if names else ("area_ratio",)
+        fwd = self.solve_forward(params, U0=U0)
+        if not fwd.converged:
+            raise RuntimeError(
+                f"the forward solve did not converge, so no gradient can be taken: {fwd.message}")
+        ops = fwd.operators
+        flow = self.flow_at(params)
+        J, J_U, J_edge = objective_and_partials(fwd.U, ops, flow, objective)
+        bk = HipBackend(ops, flow, self.options)
+        lam, rel, its, ok = bk.adjoint_solve(fwd.U, J_U, tolerance=adjoint_tolerance,
+                                             verbose=verbose)
+        if not ok:
+            raise RuntimeError(f"the adjoint march stalled at relative residual {rel:.2e} after "
+                               f"{its} iterations")
+        _, gmet, gedge, gpb = bk.residual_vjp(fwd.U, lam, geometry=True)
+        gX = edge_vjp(ops, J_edge) - metric_vjp(ops, gmet) - edge_vjp(ops, gedge)
+        grad = {}
+        for k in names:
+            if k == "back_pressure":
+                grad[k] = -gpb
+                continue
+            h = fd_step * max(1.0, abs(base[k]))
+            dX = (self.coords({**base, k: base[k] + h}) - self.coords({**base, k: base[k] - h})) / (2 * h)
+            grad[k] = float((gX * dX).sum())
+        info = {"adjoint_iterations": its, "adjoint_relative_residual": rel}
+        return J, grad, fwd, info

@@ -134,6 +134,34 @@ int64_t nz_superbee_work_doubles(const nz_problem *p);
 int nz_superbee(const nz_problem *p, double *U, double tvb, double area_total, double *work,
                 uint64_t *counters, void *stream);
 
+/* ---- adjoint (src/sensitivity.py:289-460) ---------------------------------
+ * Workspace of the boundary-face pre-pass: nz_adjoint_work_doubles(p). */
+int64_t nz_adjoint_work_doubles(const nz_problem *p);
+
+/* dU = (dR/dU)^T lam for the unfolded residual R at state U (the JAX vjp of
+ * assembly.residual, src/sensitivity.py:313-317), with flux, boundary and
+ * Euler-flux Jacobians recomputed per point.  Optional cotangents (nullable):
+ *   gmet  (E, nqv, 5): d(lam.R)/d(xi_x, xi_y, eta_x, eta_y, w*det) per volume point;
+ *   gedge (F, nqf, 3): d(lam.R)/d(n_x, n_y, |dX/dsigma|) per edge point;
+ *   gpb   (Fb, nqf):   d(lam.R)/d(back pressure) per boundary point.
+ * They chain to node coordinates and design variables on the host
+ * (src/sensitivity.py:453-457). */
+int nz_residual_vjp(const nz_problem *p, const double *U, const double *lam, double *dU,
+                    double *gmet, double *gedge, double *gpb, double *work, void *stream);
+
+/* One stage (1..4) of the classical RK4 pseudo-time march of the adjoint,
+ * rate(lam) = -M^-1 (vjp(lam) - seed) with the frozen step dt
+ * (src/sensitivity.py:330-357):
+ *   stage 1: acc = k; lam_next = base + dt/2 k     stage 3: acc += 2k; lam_next = base + dt k
+ *   stage 2: acc += 2k; lam_next = base + dt/2 k   stage 4: lam_next = base + dt/6 (acc + k) */
+int nz_adjoint_rk4_stage(const nz_problem *p, const double *U, const double *lam_in,
+                         const double *lam_base, const double *seed, const double *dt,
+                         double *acc, double *lam_next, int stage, double *work, void *stream);
+
+/* out[0] = sqrt(mean((vjp(lam) - seed)^2)) (src/sensitivity.py:378). */
+int nz_adjoint_defect(const nz_problem *p, const double *U, const double *lam, const double *seed,
+                      double *work, double *scratch, double *out, void *stream);
+
 /* dt_e = factor * 2 A_e / max(wave_e, 1e-10) (src/timestepping/step.py:171-183). */
 int nz_time_step(const nz_problem *p, const double *wave, double factor, double *dt,
                  void *stream);

@@ -424,6 +424,34 @@ def meshes(src, out: dict) -> None:
                             out[f"{key}_{k}"] = v
 
 
+def case_adjoint(src, out: dict) -> None:
+    """The reference's captured sensitivity case (examples/03_sensitivity.py):
+    smooth contour (default area ratio), p=1, refine 0, back pressure 0.15,
+    forward tolerance 1e-11, Roe.  Its adjoint gradients are only known from
+    docs/validation/gradients.md (JAX is not available here)."""
+    from src import NozzleGeometry, build_case, solve_nozzle
+    from src.config import FlowConditions, SolverOptions
+    from src.post.postprocess import performance
+    from src.timestepping import initialize as ini
+
+    geom = NozzleGeometry(contour="smooth")
+    flow = FlowConditions(back_pressure_ratio=0.15)
+    case = build_case(geom, flow, order=1, refine=0)
+    ops = case.operators
+    out.update(_mesh_arrays(case.topology, case.node_coords))
+    out["x_frac"] = case.grids["x_frac"]
+    out["r_frac"] = case.grids["r_frac"]
+    U0 = np.ascontiguousarray(ini.initial_state(ops, flow, geom, "quasi1d"))
+    out["U0"] = U0
+    res = solve_nozzle(case=case, flow=flow, options=SolverOptions(tolerance=1e-11),
+                       backend="numpy", verbose=False)
+    out["U"] = res.U
+    out["iterations"] = np.array(res.iterations)
+    out["converged"] = np.array(bool(res.converged))
+    out["thrust"] = np.array(performance(res).thrust)
+    out["area_ratio"] = np.array(geom.area_ratio)
+
+
 def main() -> None:
     root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     src = _import_reference(root)
@@ -438,6 +466,7 @@ def main() -> None:
         "tri_p0": case_p0,
         "tri_p2_q2_backflow": case_p2_backflow,
         "cfg1_converged": case_cfg1_converged,
+        "adjoint_case": case_adjoint,
     }
     meta = {"reference_version": getattr(src, "__version__", "?"), "numpy": np.__version__}
     for name, fn in jobs.items():

@@ -1,0 +1,135 @@
+"""The residual VJP kernel and the device adjoint (MI355X).
+
+* dot-product tests: ``<lam, dR/dU v> = <vjp(lam), v>`` with ``dR/dU v`` by
+  central differences of the device residual, for every flux and boundary
+  kind (FD truncation bounds the tolerance at 1e-6 relative);
+* geometry and back-pressure cotangents against central differences of the
+  residual under node motion and back-pressure change;
+* the captured sensitivity case of the reference (smooth contour, p=1,
+  refine 0, p_b/p_t = 0.15, tolerance 1e-11, Roe): thrust 0.0458513361 and
+  the adjoint derivatives printed in docs/validation/gradients.md:12-20, to
+  the 7 digits printed there; and central finite differences of the
+  converged objective (h = 1e-6 relative) for area_ratio.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import build_case, golden, nz_module
+
+pytestmark = pytest.mark.gpu
+
+FLUXES = ("roe", "hllc", "ausmup")
+
+
+def backend(ops, flow, **opt):
+    cf = nz_module("config")
+    be = nz_module("backend")
+    return be.HipBackend(ops, flow, cf.SolverOptions(**opt))
+
+
+def _jvp_fd(bk, U, v, h=1e-6):
+    Rp, _ = bk.residual(U + h * v)
+    Rm, _ = bk.residual(U - h * v)
+    return (Rp - Rm) / (2 * h)
+
+
+@pytest.mark.parametrize("name,key", [("cfg1", "U0"), ("cfg1", "Ur"), ("quad_q2_p2", "U0"),
+                                      ("tri_p2_q2_backflow", "U"), ("synth_p1", "U")])
+@pytest.mark.parametrize("flux", FLUXES)
+def test_vjp_dot_product(name, key, flux):
+    g = golden(name)
+    ops, flow = build_case(name, dense=False)
+    bk = backend(ops, flow.replace(flux=flux))
+    U = g[key]
+    rng = np.random.default_rng(7)
+    v = rng.standard_normal(U.shape) * np.abs(U).mean(axis=(0, 1)) * 1e-2
+    lam = rng.standard_normal(U.shape)
+    lhs = float((lam * _jvp_fd(bk, U, v)).sum())
+    rhs = float((bk.residual_vjp(U, lam) * v).sum())
+    print(name, key, flux, lhs, rhs)
+    assert abs(lhs - rhs) <= 1e-6 * abs(rhs)
+
+
+@pytest.mark.parametrize("name,key", [("cfg1", "U0"), ("quad_q2_p2", "U0"),
+                                      ("tri_p2_q2_backflow", "U")])
+def test_geometry_and_back_pressure_cotangents(name, key):
+    op = nz_module("operators")
+    sens = nz_module("sensitivity")
+    g = golden(name)
+    ops, flow = build_case(name, dense=False)
+    U = g[key]
+    bk = backend(ops, flow)
+    rng = np.random.default_rng(11)
+    lam = rng.standard_normal(U.shape)
+    _, gmet, gedge, gpb = bk.residual_vjp(U, lam, geometry=True)
+    gX = sens.metric_vjp(ops, gmet) + sens.edge_vjp(ops, gedge)
+    X = np.asarray(ops.node_coords)
+    dX = rng.standard_normal(X.shape) * 1e-2
+    h = 1e-6
+
+    def lamR(Xn, fl=flow):
+        o = op.build_operators(ops.topology, Xn, ops.ref.order, dense=False)
+        R, _ = backend(o, fl).residual(U)
+        return float((lam * R).sum())
+
+    fd = (lamR(X + h * dX) - lamR(X - h * dX)) / (2 * h)
+    an = float((gX * dX).sum())
+    print(name, "geometry", fd, an)
+    assert abs(fd - an) <= 1e-6 * abs(an)
+    pb = flow.back_pressure_ratio
+    hp = 1e-6
+    fdp = (lamR(X, flow.replace(back_pressure_ratio=pb + hp))
+           - lamR(X, flow.replace(back_pressure_ratio=pb - hp))) / (2 * hp)
+    print(name, "back pressure", fdp, gpb)
+    assert abs(fdp - gpb) <= 1e-6 * max(abs(gpb), 1e-3 * np.abs(lam).max())
+
+
+def _case():
+    sens = nz_module("sensitivity")
+    mesh = nz_module("mesh")
+    cf = nz_module("config")
+    g = golden("adjoint_case")
+    dc = sens.DifferentiableCase(geometry=mesh.NozzleGeometry(contour="smooth"),
+                                 flow=cf.FlowConditions(back_pressure_ratio=0.15),
+                                 order=1, refine=0,
+                                 options=cf.SolverOptions(tolerance=1e-11, print_interval=0))
+    assert np.array_equal(dc.coords(), g["node_coords"])
+    return dc, g
+
+
+def test_captured_adjoint_gradients():
+    """docs/validation/gradients.md:12-20: thrust and five adjoint derivatives."""
+    dc, g = _case()
+    names = ("area_ratio", "throat_x", "theta_exit", "inlet_half_height", "back_pressure")
+    J, grad, fwd, info = dc.value_and_gradient("thrust", names=names, U0=g["U0"])
+    print("J", J, "grad", grad, info, "iterations", fwd.iterations)
+    assert fwd.iterations == int(g["iterations"])
+    assert abs(J - float(g["thrust"])) <= 1e-10 * abs(J)
+    assert abs(J - 0.0458513361) <= 1e-10
+    captured = {"area_ratio": 9.772082e-03, "throat_x": -1.490300e-02,
+                "theta_exit": 7.602688e-03, "inlet_half_height": -1.524949e+00}
+    for k, v in captured.items():
+        assert abs(grad[k] - v) <= 6e-7 * abs(v) + 1e-12, (k, grad[k], v)
+    assert abs(grad["back_pressure"]) <= 1e-12
+
+
+def test_gradient_against_central_differences():
+    dc, g = _case()
+    J, grad, fwd, _ = dc.value_and_gradient("thrust", names=("area_ratio", "length"), U0=g["U0"])
+    sens = nz_module("sensitivity")
+    base = dc.default_params()
+    for k in ("area_ratio", "length"):
+        h = 1e-6 * abs(base[k])
+        vals = []
+        for s in (+1, -1):
+            res = dc.solve_forward({k: base[k] + s * h}, U0=fwd.U)
+            assert res.converged
+            vals.append(sens.objective_and_partials(res.U, res.operators, dc.flow_at(), "thrust")[0])
+        fd = (vals[0] - vals[1]) / (2 * h)
+        print(k, "adjoint", grad[k], "fd", fd)
+        assert abs(fd - grad[k]) <= 1e-4 * abs(grad[k])
+    # the reference's recorded length derivative (docs/validation/gradients.md:28)
+    assert abs(grad["length"] - 1.353855e-2) <= 6e-7 * 1.353855e-2
