@@ -198,6 +198,158 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 }
 
 // ---------------------------------------------------------------- DG kernel
+// Component-split mapping: a workgroup of 256 threads owns 64 consecutive
+// elements and gives each element four lanes, one per conserved variable
+// (lane = 4 * element + s).  Everything linear in U -- traces, volume-point
+// interpolation, lifting, volume projection, the stage combination and the
+// limiter's probe sums -- is separable by component, so each lane keeps its
+// element's coefficient column U[:, s] and its accumulator R[:, s] in
+// registers and reads reference tables with scalar loads.  Only the
+// pointwise physics needs all four components; it runs on (element, point)
+// items with one point per wave, exchanging point states through LDS rows
+// [point][component][element].  The limiter gathers the other components of
+// its element with quad shuffles.
+constexpr int TE = 64;   // elements per workgroup
+constexpr int TP = TE + 1;
+
+template <class C>
+struct Tile {
+    static constexpr int N = C::N, NQV = C::NQV, NQF = C::NQF, NF = C::NF, NS = C::NS;
+    static constexpr int QR = 4;  // volume points per round: one per wave
+    static constexpr int FACE = (3 * NQF * 4 + NQF) * TP;  // own, neighbour, flux rows + speeds
+    static constexpr int VOL = 3 * QR * 4 * TP;            // point states, two contravariant fluxes
+    static constexpr int IO = NS * TP;                     // staging of U in and out
+    static constexpr int WORK = FACE > VOL ? (FACE > IO ? FACE : IO) : (VOL > IO ? VOL : IO);
+};
+
+// component k of this lane's element (lanes 4e..4e+3 hold s = 0..3)
+__device__ __forceinline__ double quad_get(double v, int k) {
+    return __shfl(v, (static_cast<int>(threadIdx.x) & ~3) | k, 64);
+}
+
+// sum_i b(i) col[i] with no contraction: the limiter's probe sums must round
+// as the CPU oracle's (src/backends/_numba_kernels.py:878-884)
+template <int N, class B>
+__device__ __forceinline__ double exact_dot(const double* col, B b) {
+#pragma clang fp contract(off)
+    double v = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) v += b(i) * col[i];
+    return v;
+}
+
+// min density and pressure over the probe set at scaling theta; every lane of
+// the element returns the same values
+template <class C>
+__device__ void probe_minmax(const double* col, int s, const double ub[4], double theta,
+                             const double* __restrict__ T, const nz_problem& P, int sidebits,
+                             double* rmin, double* pmin) {
+#pragma clang fp contract(off)
+    constexpr int N = C::N, NQV = C::NQV, NQF = C::NQF, NF = C::NF;
+    double rm = 1e300, pm = 1e300;
+    const double g = P.gamma;
+#pragma unroll 1
+    for (int k = 0; k < NQV + NF * NQF; ++k) {
+        double v;
+        if (k < NQV) {
+            v = exact_dot<N>(col, [&](int i) { return T[P.off.phi_vol + i * NQV + k]; });
+        } else {
+            const int f = (k - NQV) / NQF, q = (k - NQV) - f * NQF;
+            const int sd = (sidebits >> f) & 1;
+            v = exact_dot<N>(col, [&](int i) {
+                const double b0 = T[P.off.phi_face + ((0 * NF + f) * N + i) * NQF + q];
+                const double b1 = T[P.off.phi_face + ((1 * NF + f) * N + i) * NQF + q];
+                return sd ? b1 : b0;
+            });
+        }
+        double V[4];
+        for (int c = 0; c < 4; ++c) V[c] = quad_get(v, c);
+        for (int c = 0; c < 4; ++c) V[c] = ub[c] + theta * (V[c] - ub[c]);
+        rm = V[0] < rm ? V[0] : rm;
+        const double r = V[0] > 1e-300 ? V[0] : 1e-300;
+        const double pp = (g - 1.0) * (V[3] - 0.5 * (V[1] * V[1] + V[2] * V[2]) / r);
+        pm = pp < pm ? pp : pm;
+    }
+    (void)s;
+    *rmin = rm;
+    *pmin = pm;
+}
+
+// Zhang-Shu positivity limiter on the element whose column `col` this lane
+// holds (src/backends/_numba_kernels.py:927-1093; flags as
+// src/spatial/limiter.py:125-129,222-224).  The four lanes of an element take
+// identical branches.  Returns the flag bits; *scaled = 1 when theta < 1 was
+// applied.
+template <class C>
+__device__ uint8_t limit_column(double* col, int s, const double* mw, const double* __restrict__ T,
+                                const nz_problem& P, int sidebits, int* scaled) {
+#pragma clang fp contract(off)
+    constexpr int N = C::N;
+    const double g1 = P.gamma - 1.0;
+    uint8_t fl = 0;
+    double ub[4];
+    {
+        const double m = exact_dot<N>(col, [&](int i) { return mw[i]; });
+        for (int c = 0; c < 4; ++c) ub[c] = quad_get(m, c);
+    }
+    const double rho_new = ub[0] > P.rho_floor ? ub[0] : P.rho_floor;
+    const double e_min = P.p_floor / g1 + 0.5 * (ub[1] * ub[1] + ub[2] * ub[2]) / rho_new;
+    const double rhoE_new = ub[3] > e_min ? ub[3] : e_min;
+    const double d0 = rho_new - ub[0], d3 = rhoE_new - ub[3];
+    if (d0 != 0.0 || d3 != 0.0) {
+        const double d = s == 0 ? d0 : (s == 3 ? d3 : 0.0);
+        if (s == 0 || s == 3)
+            for (int i = 0; i < N; ++i) col[i] += d;
+        ub[0] = rho_new;
+        ub[3] = rhoE_new;
+        fl |= 1;
+    }
+    const double p_bar = g1 * (ub[3] - 0.5 * (ub[1] * ub[1] + ub[2] * ub[2]) / ub[0]);
+    double eps_rho = P.fraction * ub[0];
+    if (eps_rho < 0.5 * P.rho_floor) eps_rho = 0.5 * P.rho_floor;
+    double eps_p = P.fraction * p_bar;
+    if (eps_p < 0.5 * P.p_floor) eps_p = 0.5 * P.p_floor;
+    if (P.lebesgue > 0.0) {  // O(N) sufficient screen (_numba_kernels.py:1020-1045)
+        double dm = 0.0;
+        for (int i = 0; i < N; ++i) dm = fmax(dm, fabs(col[i] - ub[s]));
+        double dv[4];
+        for (int c = 0; c < 4; ++c) dv[c] = quad_get(dm, c);
+        const double rho_lo = ub[0] - P.lebesgue * dv[0];
+        if (rho_lo >= eps_rho) {
+            const double m_hi = sqrt(ub[1] * ub[1] + ub[2] * ub[2]) +
+                                P.lebesgue * sqrt(dv[1] * dv[1] + dv[2] * dv[2]);
+            const double p_lo = g1 * (ub[3] - P.lebesgue * dv[3] - 0.5 * m_hi * m_hi / rho_lo);
+            if (p_lo >= eps_p) return fl;
+        }
+    }
+    double rmin, pmin;
+    probe_minmax<C>(col, s, ub, 1.0, T, P, sidebits, &rmin, &pmin);
+    if (rmin >= eps_rho && pmin >= eps_p) return fl;
+    fl |= 2;
+    double theta = 1.0;
+    if (rmin < eps_rho) {
+        const double gap = ub[0] - rmin;
+        theta = gap <= 0.0 ? 0.0 : fmin(fmax((ub[0] - eps_rho) / gap, 0.0), 1.0);
+    }
+    double rr, phi;
+    probe_minmax<C>(col, s, ub, theta, T, P, sidebits, &rr, &phi);
+    if (phi < eps_p) {
+        double lo = 0.0, hi = theta;
+#pragma unroll 1
+        for (int it = 0; it < P.bisect_steps; ++it) {
+            const double mid = 0.5 * (lo + hi);
+            double pm;
+            probe_minmax<C>(col, s, ub, mid, T, P, sidebits, &rr, &pm);
+            if (pm >= eps_p) lo = mid;
+            else hi = mid;
+        }
+        theta = lo;
+    }
+    for (int i = 0; i < N; ++i) col[i] = ub[s] + theta * (col[i] - ub[s]);
+    *scaled = 1;
+    return fl;
+}
+
 template <class C, int FLUX>
 __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __restrict__ T,
                                            const double* __restrict__ Uin,
@@ -207,307 +359,340 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
                                            double* __restrict__ partial,
                                            uint8_t* __restrict__ flags,
                                            unsigned long long* __restrict__ counters) {
+    using L = Tile<C>;
     constexpr int N = C::N, NQV = C::NQV, NQF = C::NQF, NF = C::NF, NS = C::NS;
-    constexpr int TE = C::TE, TP = C::TP, SLOTS = C::SLOTS;
-    __shared__ double sU[NS * TP];
-    __shared__ double sW[C::WORK];
+    __shared__ double sW[L::WORK];
     __shared__ double sPF[C::PF];
-    __shared__ double sG[6 * TE];  // xi_x, xi_y, eta_x, eta_y, det | dt
-    __shared__ int sSide[NF * TE];
     __shared__ double sRed[BS / 64];
 
     const int t = threadIdx.x;
-    const int el = t % TE;
-    const int slot = wave_slot() / (TE / 64);  // uniform: s, q or i base
+    const int el = t >> 2, s = t & 3;       // component lanes
+    const int pe = t & (TE - 1);            // point items: element ...
+    const int slot = wave_slot();           // ... and wave-uniform point slot
     const int64_t e0 = static_cast<int64_t>(blockIdx.x) * TE;
     const int ne = static_cast<int>(P.n_elem - e0 < TE ? P.n_elem - e0 : TE);
-    const bool live = el < ne;
-    const int64_t e = e0 + el;
+    const bool live = el < ne, plive = pe < ne;
+    const int64_t e = e0 + el, ep = e0 + pe;
     const bool fold = C::AFF && mode != kRes;  // affine rate: M^-1 folded into tables
 
-    // ---- stage the tile
+    // ---- stage the tile (coalesced) and pick up this lane's column
     for (int it = t; it < TE * NS; it += BS) {
         const int r = it / NS, c = it - r * NS;
-        sU[c * TP + r] = r < ne ? Uin[(e0 + r) * NS + c] : 0.0;
+        sW[c * TP + r] = r < ne ? Uin[(e0 + r) * NS + c] : 0.0;
     }
     for (int it = t; it < C::PF; it += BS) sPF[it] = T[P.off.phi_face + it];
-    if (t < TE && live) {
-        if (C::AFF)
-            for (int k = 0; k < 5; ++k) sG[k * TE + t] = P.geo[e * 5 + k];
-        if (mode == kStageNext) sG[5 * TE + t] = dt[e];
-        for (int f = 0; f < NF; ++f) sSide[f * TE + t] = (P.fe_info[e * NF + f] >> 2) & 1;
-    }
     __syncthreads();
-
-    double acc[C::NIT][4];
+    double u[N], acc[N];
 #pragma unroll
-    for (int k = 0; k < C::NIT; ++k)
-        for (int s = 0; s < 4; ++s) acc[k][s] = 0.0;
-    double wave = 0.0;  // held by threads t < TE for element t
+    for (int i = 0; i < N; ++i) {
+        u[i] = sW[(i * 4 + s) * TP + el];
+        acc[i] = 0.0;
+    }
+    // per-element data every lane of the element needs
+    int side[NF], nbr[NF], info[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        nbr[f] = live ? P.fe_nbr[e * NF + f] : 0;
+        info[f] = live ? P.fe_info[e * NF + f] : 0;
+        side[f] = (info[f] >> 2) & 1;
+    }
+    int sidebits = 0;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) sidebits |= side[f] << f;
+    double wave = 0.0;
+    __syncthreads();  // the staging rows are reused below
 
-    double* sTn = sW;                      // [NQF][4][TP] neighbour traces
-    double* sFw = sW + NQF * 4 * TP;       // [NQF][4][TP] signed, weighted fluxes
-    double* sSp = sW + 2 * NQF * 4 * TP;   // [NQF][TP]    signal speeds
+    double* sTo = sW;                        // [NQF][4][TP] own trace
+    double* sTn = sW + NQF * 4 * TP;         // [NQF][4][TP] neighbour trace
+    double* sFw = sW + 2 * NQF * 4 * TP;     // [NQF][4][TP] signed, weighted flux
+    double* sSp = sW + 3 * NQF * 4 * TP;     // [NQF][TP]    signal speed
     const int lift_off = fold ? P.off.lift : P.off.phi_face;
 
     // ---------------------------------------------------------------- faces
+#pragma unroll 1
     for (int f = 0; f < NF; ++f) {
-        // (e,s): neighbour trace at the face points, from HBM
-        for (int s = slot; s < 4; s += SLOTS) {
-            double tr[NQF];
-#pragma unroll
-            for (int q = 0; q < NQF; ++q) tr[q] = 0.0;
-            if (live) {
-                const int nb = P.fe_nbr[e * NF + f];
-                if (nb >= 0) {
-                    const int info = P.fe_info[e * NF + f];
-                    const int of = info & 3, oside = 1 - ((info >> 2) & 1);
-                    const double* ur = Uin + static_cast<int64_t>(nb) * NS + s;
-                    const double* ph = sPF + (oside * NF + of) * N * NQF;
-#pragma unroll
-                    for (int i = 0; i < N; ++i) {
-                        const double x = ur[i * 4];
-#pragma unroll
-                        for (int q = 0; q < NQF; ++q) tr[q] += ph[i * NQF + q] * x;
-                    }
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < NQF; ++q) sTn[(q * 4 + s) * TP + el] = tr[q];
-        }
-        __syncthreads();
-        // (e,q): own trace, flux, weighting
-        for (int q = slot; q < NQF; q += SLOTS) {
-            if (live) {
-                const int nb = P.fe_nbr[e * NF + f];
-                const int info = P.fe_info[e * NF + f];
-                const int side = (info >> 2) & 1;
-                const int64_t k = info >> 3;
-                const double* ph = sPF + (side * NF + f) * N * NQF + q;
-                double uo[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-                for (int i = 0; i < N; ++i) {
-                    const double b = ph[i * NQF];
-#pragma unroll
-                    for (int s = 0; s < 4; ++s) uo[s] += b * sU[(i * 4 + s) * TP + el];
-                }
-                const double* eg = C::STR ? P.edge_geo + k * 3 : P.edge_geo + (k * NQF + q) * 3;
-                const double nx = eg[0], ny = eg[1], jac = eg[2];
-                double fx[4], sp;
-                if (nb >= 0) {
-                    double un[4];
-#pragma unroll
-                    for (int s = 0; s < 4; ++s) un[s] = sTn[(q * 4 + s) * TP + el];
-                    double ul[4], ur[4];
-#pragma unroll
-                    for (int s = 0; s < 4; ++s) {
-                        ul[s] = side == 0 ? uo[s] : un[s];
-                        ur[s] = side == 0 ? un[s] : uo[s];
-                    }
-                    sp = interface_flux<FLUX>(P, ul, ur, nx, ny, fx);
-                    const double sc = jac * T[P.off.w_face + q];
-#pragma unroll
-                    for (int s = 0; s < 4; ++s) fx[s] = fx[s] * sc;
-                } else {
-                    // boundary faces: weighted flux precomputed by k_bflux
-                    const double* bf = P.bflux + ((k - P.n_int) * NQF + q) * 5;
-#pragma unroll
-                    for (int s = 0; s < 4; ++s) fx[s] = bf[s];
-                    sp = bf[4];
-                }
-                double sg = side ? -1.0 : 1.0;
-                if (fold) sg = sg / sG[4 * TE + el];
-#pragma unroll
-                for (int s = 0; s < 4; ++s) sFw[(q * 4 + s) * TP + el] = sg * fx[s];
-                sSp[q * TP + el] = sp;
-            }
-        }
-        __syncthreads();
-        // (e,i): lift
-#pragma unroll
-        for (int kk = 0; kk < C::NIT; ++kk) {
-            const int i = slot + kk * SLOTS;
-            if (i < N && live) {
-                const int side = sSide[f * TE + el];
+        // (e,s): own and neighbour traces of component s at every face point
+        {
+            // Tables are read with a wave-uniform side when every lane of the
+            // wave agrees (scalar loads, the usual case on a structured
+            // mesh), and with per-lane loads otherwise.
+            double to[NQF], tn[NQF];
+            const int sd = side[f];
+            const int sd0 = __builtin_amdgcn_readfirstlane(sd);
+            if (__all(sd == sd0)) {
+                const double* b = T + P.off.phi_face + ((sd0 * NF + f) * N) * NQF;
 #pragma unroll
                 for (int q = 0; q < NQF; ++q) {
-                    const double b0 = T[lift_off + ((0 * NF + f) * N + i) * NQF + q];
-                    const double b1 = T[lift_off + ((1 * NF + f) * N + i) * NQF + q];
-                    const double b = side ? b1 : b0;
+                    double a = 0.0;
 #pragma unroll
-                    for (int s = 0; s < 4; ++s) acc[kk][s] += b * sFw[(q * 4 + s) * TP + el];
+                    for (int i = 0; i < N; ++i) a += b[i * NQF + q] * u[i];
+                    to[q] = a;
+                }
+            } else {
+                const double* b = sPF + ((sd * NF + f) * N) * NQF;
+#pragma unroll
+                for (int q = 0; q < NQF; ++q) {
+                    double a = 0.0;
+#pragma unroll
+                    for (int i = 0; i < N; ++i) a += b[i * NQF + q] * u[i];
+                    to[q] = a;
                 }
             }
-        }
-        if (t < TE && live) {
-            const int info = P.fe_info[e * NF + f];
-            const int64_t k = info >> 3;
-            double smax = 0.0, len = 0.0;
-            for (int q = 0; q < NQF; ++q) {
-                const double sp = sSp[q * TP + el];
-                smax = sp > smax ? sp : smax;
-                const double jac = C::STR ? P.edge_geo[k * 3 + 2] : P.edge_geo[(k * NQF + q) * 3 + 2];
-                len += jac * T[P.off.w_face + q];
-            }
-            wave += smax * len;
-        }
-        __syncthreads();
-    }
-
-    // --------------------------------------------------------------- volume
-    double* sFx = sW;                  // [QR][4][TP]
-    double* sFy = sW + C::QR * 4 * TP;
-    const int vx_off = fold ? P.off.axi : P.off.dxi;
-    const int vy_off = fold ? P.off.aeta : P.off.deta;
-    for (int q0 = 0; q0 < NQV; q0 += C::QR) {
-        {
-            const int qq = slot, q = q0 + qq;
-            if (q < NQV && live) {
-                double uq[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int q = 0; q < NQF; ++q) tn[q] = 0.0;
+            const bool has_nbr = live && nbr[f] >= 0;
+            const int key = has_nbr ? ((1 - sd) * NF + (info[f] & 3)) : -1;
+            const int key0 = __builtin_amdgcn_readfirstlane(key);
+            if (__all(key == key0) && key0 >= 0) {
+                const double* ur = Uin + static_cast<int64_t>(nbr[f]) * NS + s;
+                const double* ph = T + P.off.phi_face + key0 * N * NQF;
 #pragma unroll
                 for (int i = 0; i < N; ++i) {
-                    const double b = T[P.off.phi_vol + i * NQV + q];
+                    const double x = ur[i * 4];
 #pragma unroll
-                    for (int s = 0; s < 4; ++s) uq[s] += b * sU[(i * 4 + s) * TP + el];
+                    for (int q = 0; q < NQF; ++q) tn[q] += ph[i * NQF + q] * x;
                 }
-                const Prim w = primitives(uq, P.gamma);
+            } else if (has_nbr) {
+                const double* ur = Uin + static_cast<int64_t>(nbr[f]) * NS + s;
+                const double* ph = sPF + key * N * NQF;
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    const double x = ur[i * 4];
+#pragma unroll
+                    for (int q = 0; q < NQF; ++q) tn[q] += ph[i * NQF + q] * x;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < NQF; ++q) {
+                sTo[(q * 4 + s) * TP + el] = to[q];
+                sTn[(q * 4 + s) * TP + el] = tn[q];
+            }
+        }
+        __syncthreads();
+        // (e,q): the flux at one point per wave
+        for (int q = slot; q < NQF; q += BS / TE) {
+            if (plive) {
+                const int nb = P.fe_nbr[ep * NF + f];
+                const int inf = P.fe_info[ep * NF + f];
+                const int sd = (inf >> 2) & 1;
+                const int64_t k = inf >> 3;
+                double fx[4], sp;
+                if (nb >= 0) {
+                    double ul[4], ur[4];
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        const double a = sTo[(q * 4 + c) * TP + pe], b = sTn[(q * 4 + c) * TP + pe];
+                        ul[c] = sd == 0 ? a : b;
+                        ur[c] = sd == 0 ? b : a;
+                    }
+                    const double* eg = C::STR ? P.edge_geo + k * 3 : P.edge_geo + (k * NQF + q) * 3;
+                    const double nx = eg[0], ny = eg[1];
+                    sp = interface_flux<FLUX>(P, ul, ur, nx, ny, fx);
+                    const double sc = eg[2] * T[P.off.w_face + q];
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) fx[c] = fx[c] * sc;
+                } else {  // boundary faces: weighted flux precomputed by k_bflux
+                    const double* bf = P.bflux + ((k - P.n_int) * NQF + q) * 5;
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) fx[c] = bf[c];
+                    sp = bf[4];
+                }
+                double sg = sd ? -1.0 : 1.0;
+                if (fold) sg = sg / P.geo[ep * 5 + 4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) sFw[(q * 4 + c) * TP + pe] = sg * fx[c];
+                sSp[q * TP + pe] = sp;
+            }
+        }
+        __syncthreads();
+        // (e,s): lift, and the element's wave-speed sum
+        {
+            double fw[NQF];
+#pragma unroll
+            for (int q = 0; q < NQF; ++q) fw[q] = sFw[(q * 4 + s) * TP + el];
+            const int sd = side[f];
+            const int sd0 = __builtin_amdgcn_readfirstlane(sd);
+            if (__all(sd == sd0)) {
+                const double* lb = T + lift_off + ((sd0 * NF + f) * N) * NQF;
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+#pragma unroll
+                    for (int q = 0; q < NQF; ++q) acc[i] += lb[i * NQF + q] * fw[q];
+                }
+            } else {
+                const double* lb = T + lift_off + ((sd * NF + f) * N) * NQF;
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+#pragma unroll
+                    for (int q = 0; q < NQF; ++q) acc[i] += lb[i * NQF + q] * fw[q];
+                }
+            }
+            if (live) {
+                const int64_t k = info[f] >> 3;
+                double smax = 0.0, len = 0.0;
+                for (int q = 0; q < NQF; ++q) {
+                    const double sp = sSp[q * TP + el];
+                    smax = sp > smax ? sp : smax;
+                    const double jac = C::STR ? P.edge_geo[k * 3 + 2] : P.edge_geo[(k * NQF + q) * 3 + 2];
+                    len += jac * T[P.off.w_face + q];
+                }
+                wave += smax * len;
+            }
+        }
+        // the next face writes sTo/sTn only; sFw is rewritten after its next barrier
+    }
+    __syncthreads();
+
+    // --------------------------------------------------------------- volume
+    double* sUq = sW;                        // [QR][4][TP] point states
+    double* sFx = sW + L::QR * 4 * TP;       // [QR][4][TP]
+    double* sFy = sW + 2 * L::QR * 4 * TP;   // [QR][4][TP]
+    const int vx_off = fold ? P.off.axi : P.off.dxi;
+    const int vy_off = fold ? P.off.aeta : P.off.deta;
+#pragma unroll 1
+    for (int q0 = 0; q0 < NQV; q0 += L::QR) {
+#pragma unroll 1
+        for (int qq = 0; qq < L::QR; ++qq) {
+            if (q0 + qq < NQV) {
+                double a = 0.0;
+#pragma unroll
+                for (int i = 0; i < N; ++i) a += T[P.off.phi_vol + i * NQV + q0 + qq] * u[i];
+                sUq[(qq * 4 + s) * TP + el] = a;
+            }
+        }
+        __syncthreads();
+        {
+            const int qq = slot, q = q0 + qq;
+            if (q < NQV && plive) {
+                double uq[4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) uq[c] = sUq[(qq * 4 + c) * TP + pe];
+                const Prim<double> w = primitives(uq, P.gamma);
                 const double F[4] = {w.r * w.u, w.r * w.u * w.u + w.p, w.r * w.u * w.v, w.r * w.u * w.H};
                 const double G[4] = {w.r * w.v, w.r * w.u * w.v, w.r * w.v * w.v + w.p, w.r * w.v * w.H};
-                double xx, xy, ex, ey, wd;
+                double gx, gy, hx, hy, wd;
                 if (C::AFF) {
-                    xx = sG[el];
-                    xy = sG[TE + el];
-                    ex = sG[2 * TE + el];
-                    ey = sG[3 * TE + el];
-                    wd = T[P.off.w_vol + q] * sG[4 * TE + el];
+                    const double* ge = P.geo + ep * 5;
+                    gx = ge[0];
+                    gy = ge[1];
+                    hx = ge[2];
+                    hy = ge[3];
+                    wd = T[P.off.w_vol + q] * ge[4];
                 } else {
-                    const double* gq = P.geo + (e * NQV + q) * 5;
-                    xx = gq[0];
-                    xy = gq[1];
-                    ex = gq[2];
-                    ey = gq[3];
+                    const double* gq = P.geo + (ep * NQV + q) * 5;
+                    gx = gq[0];
+                    gy = gq[1];
+                    hx = gq[2];
+                    hy = gq[3];
                     wd = gq[4];
                 }
 #pragma unroll
-                for (int s = 0; s < 4; ++s) {
-                    double a = xx * F[s] + xy * G[s];
-                    double b = ex * F[s] + ey * G[s];
+                for (int c = 0; c < 4; ++c) {
+                    double a = gx * F[c] + gy * G[c];
+                    double b = hx * F[c] + hy * G[c];
                     if (!fold) {
                         a *= wd;
                         b *= wd;
                     }
-                    sFx[(qq * 4 + s) * TP + el] = a;
-                    sFy[(qq * 4 + s) * TP + el] = b;
+                    sFx[(qq * 4 + c) * TP + pe] = a;
+                    sFy[(qq * 4 + c) * TP + pe] = b;
                 }
             }
         }
         __syncthreads();
+#pragma unroll 1
+        for (int qq = 0; qq < L::QR; ++qq) {
+            const int q = q0 + qq;
+            if (q < NQV) {
+                const double a = sFx[(qq * 4 + s) * TP + el], b = sFy[(qq * 4 + s) * TP + el];
 #pragma unroll
-        for (int kk = 0; kk < C::NIT; ++kk) {
-            const int i = slot + kk * SLOTS;
-            if (i < N && live) {
-                for (int qq = 0; qq < C::QR && q0 + qq < NQV; ++qq) {
-                    const int q = q0 + qq;
-                    const double ax = T[vx_off + i * NQV + q], ay = T[vy_off + i * NQV + q];
-#pragma unroll
-                    for (int s = 0; s < 4; ++s)
-                        acc[kk][s] -= ax * sFx[(qq * 4 + s) * TP + el] + ay * sFy[(qq * 4 + s) * TP + el];
-                }
+                for (int i = 0; i < N; ++i)
+                    acc[i] -= T[vx_off + i * NQV + q] * a + T[vy_off + i * NQV + q] * b;
             }
         }
-        __syncthreads();
+        // the next round writes sUq only; sFx/sFy are rewritten after its next barrier
     }
 
     // ------------------------------------------------------------ epilogue
-    // acc holds R (unfolded) or -rate (affine, folded)
-    if (mode != kRes && !C::AFF) {
-        double* sR = sW;  // [NS][TP]
+    // acc holds R (unfolded) or -rate (affine, folded) for component s
+    if (mode != kRes && !C::AFF && live) {
+        double r[N];
 #pragma unroll
-        for (int kk = 0; kk < C::NIT; ++kk) {
-            const int i = slot + kk * SLOTS;
-            if (i < N)
-                for (int s = 0; s < 4; ++s) sR[(i * 4 + s) * TP + el] = acc[kk][s];
-        }
-        __syncthreads();
+        for (int i = 0; i < N; ++i) {
+            const double* mrow = P.minv + (e * N + i) * N;
+            double a = 0.0;
 #pragma unroll
-        for (int kk = 0; kk < C::NIT; ++kk) {
-            const int i = slot + kk * SLOTS;
-            if (i < N && live) {
-                const double* mrow = P.minv + (e * N + i) * N;
-                double r[4] = {0.0, 0.0, 0.0, 0.0};
-                for (int j = 0; j < N; ++j) {
-                    const double m = mrow[j];
-                    for (int s = 0; s < 4; ++s) r[s] += m * sR[(j * 4 + s) * TP + el];
-                }
-                for (int s = 0; s < 4; ++s) acc[kk][s] = r[s];
-            }
+            for (int j = 0; j < N; ++j) a += mrow[j] * acc[j];
+            r[i] = a;
         }
+#pragma unroll
+        for (int i = 0; i < N; ++i) acc[i] = r[i];
     }
-    const double osign = mode == kRes ? 1.0 : -1.0;  // rate = -M^-1 R
-
+    __syncthreads();  // last reads of the volume rows are done; reuse sW for staging
+    double* sIO = sW;
     if (mode == kRes || mode == kRate) {
+        const double osign = mode == kRes ? 1.0 : -1.0;  // rate = -M^-1 R
 #pragma unroll
-        for (int kk = 0; kk < C::NIT; ++kk) {
-            const int i = slot + kk * SLOTS;
-            if (i < N && live)
-                for (int s = 0; s < 4; ++s) Uout[(e * N + i) * 4 + s] = osign * acc[kk][s];
+        for (int i = 0; i < N; ++i) sIO[(i * 4 + s) * TP + el] = osign * acc[i];
+        if (s == 0 && live) dt[e] = wave;
+        __syncthreads();
+        for (int it = t; it < TE * NS; it += BS) {
+            const int r = it / NS, c = it - r * NS;
+            if (r < ne) Uout[(e0 + r) * NS + c] = sIO[c * TP + r];
         }
-        if (t < TE && live) dt[e] = wave;
         return;
     }
 
-    // fused stage: dt, combination, norm partials, limiter, store
-    if (mode == kStageFirst && t < TE && live) {
-        double d = wave < kFloor ? kFloor : wave;
-        const double h = dt_factor * 2.0 * P.area[e] / d;
-        sG[5 * TE + t] = h;
-        dt[e] = h;
+    // fused stage: time step, combination, norm partials, limiter, store
+    double h = 0.0;
+    if (live) {
+        if (mode == kStageFirst) {
+            const double d = wave < kFloor ? kFloor : wave;
+            h = dt_factor * 2.0 * P.area[e] / d;
+            if (s == 0) dt[e] = h;
+        } else {
+            h = dt[e];
+        }
     }
-    __syncthreads();
     double nacc = 0.0;
 #pragma unroll
-    for (int kk = 0; kk < C::NIT; ++kk) {
-        const int i = slot + kk * SLOTS;
-        if (i < N && live) {
-            const double h = sG[5 * TE + el];
-            for (int s = 0; s < 4; ++s) {
-                const double F = -acc[kk][s];
-                double* x = &sU[(i * 4 + s) * TP + el];
-                if (mode == kStageFirst) {
-                    *x = *x + h * F;
-                    const double v = F * P.wnorm[s];
-                    nacc += v * v;
-                } else {
-                    *x = ca * Un[(e * N + i) * 4 + s] + cb * (*x + h * F);
-                }
-            }
+    for (int i = 0; i < N; ++i) {
+        const double F = -acc[i];
+        if (mode == kStageFirst) {
+            u[i] = u[i] + h * F;
+            const double v = F * P.wnorm[s];
+            nacc += live ? v * v : 0.0;
+        } else {
+            u[i] = live ? ca * Un[(e * N + i) * 4 + s] + cb * (u[i] + h * F) : 0.0;
         }
     }
     if (mode == kStageFirst) {
         const double tot = block_sum(nacc, sRed);
         if (t == 0) partial[blockIdx.x] = tot;
     }
-    __syncthreads();
-    if (P.limiter == 1 && N > 1 && t < TE && live) {  // superbee runs as separate passes
-        int side[NF];
-        for (int f = 0; f < NF; ++f) side[f] = sSide[f * TE + t];
-        double mw[N];
-        for (int i = 0; i < N; ++i) mw[i] = C::AFF ? T[P.off.mean_w + i] : P.mean_w[e * N + i];
-        const LimTables tb{T + P.off.phi_vol, sPF};
-        int scaled = 0;
-        const uint8_t fl = limit_element<C>(sU + t, TP, mw, tb, side, P, &scaled);
-        if (flags) flags[e] = fl;
-        if (counters && (scaled || (fl & 1))) {
-            if (scaled) atomicAdd(&counters[0], 1ull);
-            if (fl & 1) atomicAdd(&counters[1], 1ull);
+    if (P.limiter == 1 && N > 1) {  // superbee runs as separate passes
+        uint8_t fl = 0;
+        if (live) {
+            double mw[N];
+#pragma unroll
+            for (int i = 0; i < N; ++i) mw[i] = C::AFF ? T[P.off.mean_w + i] : P.mean_w[e * N + i];
+            int scaled = 0;
+            fl = limit_column<C>(u, s, mw, T, P, sidebits, &scaled);
+            if (s == 0 && counters) {
+                if (scaled) atomicAdd(&counters[0], 1ull);
+                if (fl & 1) atomicAdd(&counters[1], 1ull);
+            }
         }
-    } else if (flags && t < TE && live) {
+        if (flags && live && s == 0) flags[e] = fl;
+    } else if (flags && live && s == 0) {
         flags[e] = 0;
     }
+#pragma unroll
+    for (int i = 0; i < N; ++i) sIO[(i * 4 + s) * TP + el] = u[i];
     __syncthreads();
     for (int it = t; it < TE * NS; it += BS) {
         const int r = it / NS, c = it - r * NS;
-        if (r < ne) Uout[(e0 + r) * NS + c] = sU[c * TP + r];
+        if (r < ne) Uout[(e0 + r) * NS + c] = sIO[c * TP + r];
     }
 }
 
@@ -1135,11 +1320,13 @@ struct Launch {
     static void dg(const nz_problem& P, const double* Uin, const double* Un, double* Uout,
                    double* dt, double fac, double a, double b, int mode, double* partial,
                    uint8_t* flags, unsigned long long* counters, int64_t nblk, hipStream_t st) {
+        (void)nblk;  // k_dg tiles are nz::TE elements
+        const int64_t nblk_dg = (P.n_elem + nz::TE - 1) / nz::TE;
         const int64_t nbq = (P.n_edge - P.n_int) * C::NQF;
         if (nbq > 0)
             hipLaunchKernelGGL((nz::k_bflux<C>), dim3(static_cast<unsigned>((nbq + nz::BS - 1) / nz::BS)),
                                dim3(nz::BS), 0, st, P, P.tables, Uin);
-        const dim3 g(static_cast<unsigned>(nblk));
+        const dim3 g(static_cast<unsigned>(nblk_dg));
         if (P.flux_id == 1)
             hipLaunchKernelGGL((nz::k_dg<C, 1>), g, dim3(nz::BS), 0, st, P, P.tables, Uin, Un, Uout,
                                dt, fac, a, b, mode, partial, flags, counters);
@@ -1235,6 +1422,7 @@ int resolve(const nz_problem* p, Ops* ops) {
 }
 
 int64_t n_blocks(const nz_problem* p, const Ops& o) { return (p->n_elem + o.te - 1) / o.te; }
+int64_t n_blocks_dg(const nz_problem* p) { return (p->n_elem + nz::TE - 1) / nz::TE; }
 
 }  // namespace
 
@@ -1246,7 +1434,7 @@ const char* nz_last_error(void) { return g_err; }
 int64_t nz_scratch_doubles(const nz_problem* p) {
     Ops o;
     if (resolve(p, &o) != NZ_OK) return -1;
-    const int64_t a = n_blocks(p, o);
+    const int64_t a = n_blocks(p, o) > n_blocks_dg(p) ? n_blocks(p, o) : n_blocks_dg(p);
     return a > 1024 ? a : 1024;
 }
 
@@ -1302,7 +1490,7 @@ int nz_norm_finish(const nz_problem* p, const double* scratch, double* out, void
     if (!scratch || !out) return fail(NZ_E_ARG, "null array");
     const double count = static_cast<double>(p->n_elem) * p->nbf * 4.0;
     hipLaunchKernelGGL(k_norm_finish, dim3(1), dim3(nz::BS), 0, static_cast<hipStream_t>(stream),
-                       scratch, n_blocks(p, o), count, out);
+                       scratch, n_blocks_dg(p), count, out);
     return check_launch("nz_norm_finish");
 }
 

@@ -27,12 +27,15 @@ struct Prim {
 
 template <class T>
 __device__ __forceinline__ Prim<T> primitives(const T U[4], double g) {
+    // one correctly rounded reciprocal instead of three divisions; the
+    // results differ from the quotients by at most an ulp
     Prim<T> w;
     w.r = fmax(U[0], kFloor);
-    w.u = U[1] / w.r;
-    w.v = U[2] / w.r;
+    const T ir = 1.0 / w.r;
+    w.u = U[1] * ir;
+    w.v = U[2] * ir;
     w.p = fmax((g - 1.0) * (U[3] - 0.5 * w.r * (w.u * w.u + w.v * w.v)), kFloor);
-    w.H = (U[3] + w.p) / w.r;
+    w.H = (U[3] + w.p) * ir;
     return w;
 }
 
@@ -53,12 +56,13 @@ __device__ __forceinline__ T flux_roe(const T UL[4], const T UR[4], const T& nx,
     T FL[4], FR[4];
     normal_flux(L, nx, ny, FL);
     normal_flux(R, nx, ny, FR);
-    const T sL = sqrt(L.r), sR = sqrt(R.r), den = sL + sR;
-    const T vx = (sL * L.u + sR * R.u) / den;
-    const T vy = (sL * L.v + sR * R.v) / den;
-    const T H = (sL * L.H + sR * R.H) / den;
+    const T sL = sqrt(L.r), sR = sqrt(R.r), iden = 1.0 / (sL + sR);
+    const T vx = (sL * L.u + sR * R.u) * iden;
+    const T vy = (sL * L.v + sR * R.v) * iden;
+    const T H = (sL * L.H + sR * R.H) * iden;
     const T q2 = vx * vx + vy * vy;
     const T c = sqrt(fmax((g - 1.0) * (H - 0.5 * q2), kFloor));
+    const T ic = 1.0 / c;
     const T vn = vx * nx + vy * ny;
     T l1 = fabs(vn + c), l2 = fabs(vn - c), l3 = fabs(vn);
     const T smax = fmax(l1, l2);
@@ -70,8 +74,9 @@ __device__ __forceinline__ T flux_roe(const T UL[4], const T UR[4], const T& nx,
     const T G1 = (g - 1.0) * (0.5 * q2 * d0 - vx * d1 - vy * d2 + d3);
     const T G2 = -vn * d0 + d1 * nx + d2 * ny;
     const T s1 = 0.5 * (l1 + l2), s2 = 0.5 * (l1 - l2);
-    const T C1 = (G1 / (c * c)) * (s1 - l3) + (G2 / c) * s2;
-    const T C2 = (G1 / c) * s2 + (s1 - l3) * G2;
+    const T G1c = G1 * ic;
+    const T C1 = (G1c * ic) * (s1 - l3) + (G2 * ic) * s2;
+    const T C2 = G1c * s2 + (s1 - l3) * G2;
     out[0] = 0.5 * (FL[0] + FR[0]) - 0.5 * (l3 * d0 + C1);
     out[1] = 0.5 * (FL[1] + FR[1]) - 0.5 * (l3 * d1 + C1 * vx + C2 * nx);
     out[2] = 0.5 * (FL[2] + FR[2]) - 0.5 * (l3 * d2 + C1 * vy + C2 * ny);
@@ -91,10 +96,10 @@ __device__ __forceinline__ T flux_hllc(const T UL[4], const T UR[4], const T& nx
     const Prim<T> L = primitives(UL, g), R = primitives(UR, g);
     const T vnL = L.u * nx + L.v * ny, vnR = R.u * nx + R.v * ny;
     const T aL = sqrt(fmax(g * L.p / L.r, kFloor)), aR = sqrt(fmax(g * R.p / R.r, kFloor));
-    const T sL = sqrt(L.r), sR = sqrt(R.r), den = sL + sR;
-    const T vx = (sL * L.u + sR * R.u) / den;
-    const T vy = (sL * L.v + sR * R.v) / den;
-    const T H = (sL * L.H + sR * R.H) / den;
+    const T sL = sqrt(L.r), sR = sqrt(R.r), iden = 1.0 / (sL + sR);
+    const T vx = (sL * L.u + sR * R.u) * iden;
+    const T vy = (sL * L.v + sR * R.v) * iden;
+    const T H = (sL * L.H + sR * R.H) * iden;
     const T vnt = vx * nx + vy * ny;
     const T at = sqrt(fmax((g - 1.0) * (H - 0.5 * (vx * vx + vy * vy)), kFloor));
     T SL = fmin(vnL - aL, vnt - at);
