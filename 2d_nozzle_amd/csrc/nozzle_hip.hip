@@ -209,16 +209,33 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 // items with one point per wave, exchanging point states through LDS rows
 // [point][component][element].  The limiter gathers the other components of
 // its element with quad shuffles.
-constexpr int TE = 64;   // elements per workgroup
-constexpr int TP = TE + 1;
+//
+// Face traces are formed in the face's canonical point order (the side-0
+// table: sigma increasing along the local face), so all lanes use one table.
+// Gauss-Legendre points are symmetric, so at edge point q the side-0 element
+// reads its canonical index q and the side-1 element nqf-1-q; the flux item
+// writes the weighted flux back at the element's canonical index and the
+// lift is side-independent as well.
+constexpr int TE = 64;  // elements per workgroup
+
+// Column of element `el` in an LDS row of component c.  The XOR keeps both
+// access patterns free of bank conflicts: four lanes per element
+// (lane = 4 el + c) and one lane per element (lane = el, c fixed).
+__device__ __forceinline__ int swz(int el, int c) { return el ^ (((c & 1) << 4) + (c >> 1) * 40); }
 
 template <class C>
 struct Tile {
     static constexpr int N = C::N, NQV = C::NQV, NQF = C::NQF, NF = C::NF, NS = C::NS;
-    static constexpr int QR = 4;  // volume points per round: one per wave
-    static constexpr int FACE = (3 * NQF * 4 + NQF) * TP;  // own, neighbour, flux rows + speeds
-    static constexpr int VOL = 3 * QR * 4 * TP;            // point states, two contravariant fluxes
-    static constexpr int IO = NS * TP;                     // staging of U in and out
+    static constexpr int QR = BS / 64;  // volume points per round: one per wave
+    static constexpr int PT = 4 * TE;   // one point: four component rows
+    // own trace / weighted flux (two buffers, alternating faces), neighbour
+    // trace, signal speeds
+    static constexpr int FACE = 3 * NQF * PT + NQF * TE;
+    // point states / xi-flux (two buffers, alternating rounds), eta-flux
+    static constexpr int VOL = 3 * QR * PT;
+    // staging of U in and out: rows of one element, stride = 4 mod 64 doubles
+    static constexpr int SIO = NS <= 4 ? 4 : ((NS - 4 + 63) / 64) * 64 + 4;
+    static constexpr int IO = TE * SIO;
     static constexpr int WORK = FACE > VOL ? (FACE > IO ? FACE : IO) : (VOL > IO ? VOL : IO);
 };
 
@@ -350,7 +367,7 @@ __device__ uint8_t limit_column(double* col, int s, const double* mw, const doub
     return fl;
 }
 
-template <class C, int FLUX>
+template <class C, int FLUX, bool STAGE>
 __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __restrict__ T,
                                            const double* __restrict__ Uin,
                                            const double* __restrict__ Un,
@@ -361,8 +378,11 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
                                            unsigned long long* __restrict__ counters) {
     using L = Tile<C>;
     constexpr int N = C::N, NQV = C::NQV, NQF = C::NQF, NF = C::NF, NS = C::NS;
+    constexpr int NPF = NF * N * NQF;  // canonical (side-0) face table
+    constexpr int PT = L::PT;
     __shared__ double sW[L::WORK];
-    __shared__ double sPF[C::PF];
+    __shared__ double sPF[NPF];
+    __shared__ int sNb[TE * NF], sIn[TE * NF];
     __shared__ double sRed[BS / 64];
 
     const int t = threadIdx.x;
@@ -373,78 +393,62 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
     const int ne = static_cast<int>(P.n_elem - e0 < TE ? P.n_elem - e0 : TE);
     const bool live = el < ne, plive = pe < ne;
     const int64_t e = e0 + el, ep = e0 + pe;
-    const bool fold = C::AFF && mode != kRes;  // affine rate: M^-1 folded into tables
+    const bool fold = C::AFF && (STAGE || mode != kRes);  // affine rate: M^-1 folded into tables
+    const int cs = swz(el, s);
 
-    // ---- stage the tile (coalesced) and pick up this lane's column
+    // ---- stage the tile (coalesced), the face maps and the face table
     for (int it = t; it < TE * NS; it += BS) {
         const int r = it / NS, c = it - r * NS;
-        sW[c * TP + r] = r < ne ? Uin[(e0 + r) * NS + c] : 0.0;
+        sW[r * L::SIO + c] = r < ne ? Uin[(e0 + r) * NS + c] : 0.0;
     }
-    for (int it = t; it < C::PF; it += BS) sPF[it] = T[P.off.phi_face + it];
+    for (int it = t; it < TE * NF; it += BS) {
+        const bool ok = it < ne * NF;
+        sNb[it] = ok ? P.fe_nbr[e0 * NF + it] : -1;
+        sIn[it] = ok ? P.fe_info[e0 * NF + it] : 0;
+    }
+    for (int it = t; it < NPF; it += BS) sPF[it] = T[P.off.phi_face + it];
     __syncthreads();
     double u[N], acc[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-        u[i] = sW[(i * 4 + s) * TP + el];
+        u[i] = sW[el * L::SIO + i * 4 + s];
         acc[i] = 0.0;
     }
-    // per-element data every lane of the element needs
-    int side[NF], nbr[NF], info[NF];
-#pragma unroll
-    for (int f = 0; f < NF; ++f) {
-        nbr[f] = live ? P.fe_nbr[e * NF + f] : 0;
-        info[f] = live ? P.fe_info[e * NF + f] : 0;
-        side[f] = (info[f] >> 2) & 1;
-    }
-    int sidebits = 0;
-#pragma unroll
-    for (int f = 0; f < NF; ++f) sidebits |= side[f] << f;
     double wave = 0.0;
     __syncthreads();  // the staging rows are reused below
 
-    double* sTo = sW;                        // [NQF][4][TP] own trace
-    double* sTn = sW + NQF * 4 * TP;         // [NQF][4][TP] neighbour trace
-    double* sFw = sW + 2 * NQF * 4 * TP;     // [NQF][4][TP] signed, weighted flux
-    double* sSp = sW + 3 * NQF * 4 * TP;     // [NQF][TP]    signal speed
+    double* sTn = sW + 2 * NQF * PT;  // [NQF][4][TE] neighbour trace
+    double* sSp = sW + 3 * NQF * PT;  // [NQF][TE]    signal speed
     const int lift_off = fold ? P.off.lift : P.off.phi_face;
 
     // ---------------------------------------------------------------- faces
 #pragma unroll 1
     for (int f = 0; f < NF; ++f) {
-        // (e,s): own and neighbour traces of component s at every face point
+        double* sTo = sW + (f & 1) * NQF * PT;  // [NQF][4][TE] own trace, then weighted flux
+        const int nb = sNb[el * NF + f], inf = sIn[el * NF + f];
+        // (e,s): own and neighbour traces of component s, canonical order
         {
-            // Tables are read with a wave-uniform side when every lane of the
-            // wave agrees (scalar loads, the usual case on a structured
-            // mesh), and with per-lane loads otherwise.
-            double to[NQF], tn[NQF];
-            const int sd = side[f];
-            const int sd0 = __builtin_amdgcn_readfirstlane(sd);
-            if (__all(sd == sd0)) {
-                const double* b = T + P.off.phi_face + ((sd0 * NF + f) * N) * NQF;
+            const double* b = T + P.off.phi_face + f * N * NQF;
 #pragma unroll
-                for (int q = 0; q < NQF; ++q) {
-                    double a = 0.0;
+            for (int q = 0; q < NQF; ++q) {
+                double a = 0.0;
 #pragma unroll
-                    for (int i = 0; i < N; ++i) a += b[i * NQF + q] * u[i];
-                    to[q] = a;
-                }
-            } else {
-                const double* b = sPF + ((sd * NF + f) * N) * NQF;
-#pragma unroll
-                for (int q = 0; q < NQF; ++q) {
-                    double a = 0.0;
-#pragma unroll
-                    for (int i = 0; i < N; ++i) a += b[i * NQF + q] * u[i];
-                    to[q] = a;
-                }
+                for (int i = 0; i < N; ++i) a += b[i * NQF + q] * u[i];
+                sTo[(q * 4 + s) * TE + cs] = a;
             }
+        }
+        {
+            // The neighbour's face table is read with scalar loads when the
+            // wave agrees on it (the usual case on a structured mesh) and
+            // from LDS otherwise.
+            double tn[NQF];
 #pragma unroll
             for (int q = 0; q < NQF; ++q) tn[q] = 0.0;
-            const bool has_nbr = live && nbr[f] >= 0;
-            const int key = has_nbr ? ((1 - sd) * NF + (info[f] & 3)) : -1;
+            const bool has = live && nb >= 0;
+            const int key = has ? (inf & 3) : -1;
             const int key0 = __builtin_amdgcn_readfirstlane(key);
+            const double* ur = Uin + static_cast<int64_t>(has ? nb : 0) * NS + s;
             if (__all(key == key0) && key0 >= 0) {
-                const double* ur = Uin + static_cast<int64_t>(nbr[f]) * NS + s;
                 const double* ph = T + P.off.phi_face + key0 * N * NQF;
 #pragma unroll
                 for (int i = 0; i < N; ++i) {
@@ -452,10 +456,9 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
 #pragma unroll
                     for (int q = 0; q < NQF; ++q) tn[q] += ph[i * NQF + q] * x;
                 }
-            } else if (has_nbr) {
-                const double* ur = Uin + static_cast<int64_t>(nbr[f]) * NS + s;
+            } else if (has) {
                 const double* ph = sPF + key * N * NQF;
-#pragma unroll
+#pragma unroll 2
                 for (int i = 0; i < N; ++i) {
                     const double x = ur[i * 4];
 #pragma unroll
@@ -463,25 +466,24 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
                 }
             }
 #pragma unroll
-            for (int q = 0; q < NQF; ++q) {
-                sTo[(q * 4 + s) * TP + el] = to[q];
-                sTn[(q * 4 + s) * TP + el] = tn[q];
-            }
+            for (int q = 0; q < NQF; ++q) sTn[(q * 4 + s) * TE + cs] = tn[q];
         }
         __syncthreads();
-        // (e,q): the flux at one point per wave
-        for (int q = slot; q < NQF; q += BS / TE) {
+        // (e,q): the flux at edge point q, one point per wave
+        for (int q = slot; q < NQF; q += BS / 64) {
             if (plive) {
-                const int nb = P.fe_nbr[ep * NF + f];
-                const int inf = P.fe_info[ep * NF + f];
-                const int sd = (inf >> 2) & 1;
-                const int64_t k = inf >> 3;
+                const int nbp = sNb[pe * NF + f], infp = sIn[pe * NF + f];
+                const int sd = (infp >> 2) & 1;
+                const int64_t k = infp >> 3;
+                const int qo = sd ? NQF - 1 - q : q;  // this element's canonical index
                 double fx[4], sp;
-                if (nb >= 0) {
+                if (nbp >= 0) {
+                    const int qn = NQF - 1 - qo;      // the neighbour's
                     double ul[4], ur[4];
 #pragma unroll
                     for (int c = 0; c < 4; ++c) {
-                        const double a = sTo[(q * 4 + c) * TP + pe], b = sTn[(q * 4 + c) * TP + pe];
+                        const double a = sTo[(qo * 4 + c) * TE + swz(pe, c)];
+                        const double b = sTn[(qn * 4 + c) * TE + swz(pe, c)];
                         ul[c] = sd == 0 ? a : b;
                         ur[c] = sd == 0 ? b : a;
                     }
@@ -500,8 +502,8 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
                 double sg = sd ? -1.0 : 1.0;
                 if (fold) sg = sg / P.geo[ep * 5 + 4];
 #pragma unroll
-                for (int c = 0; c < 4; ++c) sFw[(q * 4 + c) * TP + pe] = sg * fx[c];
-                sSp[q * TP + pe] = sp;
+                for (int c = 0; c < 4; ++c) sTo[(qo * 4 + c) * TE + swz(pe, c)] = sg * fx[c];
+                sSp[q * TE + pe] = sp;
             }
         }
         __syncthreads();
@@ -509,29 +511,18 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
         {
             double fw[NQF];
 #pragma unroll
-            for (int q = 0; q < NQF; ++q) fw[q] = sFw[(q * 4 + s) * TP + el];
-            const int sd = side[f];
-            const int sd0 = __builtin_amdgcn_readfirstlane(sd);
-            if (__all(sd == sd0)) {
-                const double* lb = T + lift_off + ((sd0 * NF + f) * N) * NQF;
+            for (int q = 0; q < NQF; ++q) fw[q] = sTo[(q * 4 + s) * TE + cs];
+            const double* lb = T + lift_off + f * N * NQF;
 #pragma unroll
-                for (int i = 0; i < N; ++i) {
+            for (int i = 0; i < N; ++i) {
 #pragma unroll
-                    for (int q = 0; q < NQF; ++q) acc[i] += lb[i * NQF + q] * fw[q];
-                }
-            } else {
-                const double* lb = T + lift_off + ((sd * NF + f) * N) * NQF;
-#pragma unroll
-                for (int i = 0; i < N; ++i) {
-#pragma unroll
-                    for (int q = 0; q < NQF; ++q) acc[i] += lb[i * NQF + q] * fw[q];
-                }
+                for (int q = 0; q < NQF; ++q) acc[i] += lb[i * NQF + q] * fw[q];
             }
             if (live) {
-                const int64_t k = info[f] >> 3;
+                const int64_t k = inf >> 3;
                 double smax = 0.0, len = 0.0;
                 for (int q = 0; q < NQF; ++q) {
-                    const double sp = sSp[q * TP + el];
+                    const double sp = sSp[q * TE + el];
                     smax = sp > smax ? sp : smax;
                     const double jac = C::STR ? P.edge_geo[k * 3 + 2] : P.edge_geo[(k * NQF + q) * 3 + 2];
                     len += jac * T[P.off.w_face + q];
@@ -539,25 +530,25 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
                 wave += smax * len;
             }
         }
-        // the next face writes sTo/sTn only; sFw is rewritten after its next barrier
+        // the next face writes the other own-trace buffer and the neighbour
+        // rows; the speeds are rewritten only after its first barrier
     }
     __syncthreads();
 
     // --------------------------------------------------------------- volume
-    double* sUq = sW;                        // [QR][4][TP] point states
-    double* sFx = sW + L::QR * 4 * TP;       // [QR][4][TP]
-    double* sFy = sW + 2 * L::QR * 4 * TP;   // [QR][4][TP]
+    double* sFy = sW + 2 * L::QR * PT;  // [QR][4][TE] eta-flux
     const int vx_off = fold ? P.off.axi : P.off.dxi;
     const int vy_off = fold ? P.off.aeta : P.off.deta;
 #pragma unroll 1
     for (int q0 = 0; q0 < NQV; q0 += L::QR) {
+        double* sUq = sW + ((q0 / L::QR) & 1) * L::QR * PT;  // point states, then xi-flux
 #pragma unroll 1
         for (int qq = 0; qq < L::QR; ++qq) {
             if (q0 + qq < NQV) {
                 double a = 0.0;
 #pragma unroll
                 for (int i = 0; i < N; ++i) a += T[P.off.phi_vol + i * NQV + q0 + qq] * u[i];
-                sUq[(qq * 4 + s) * TP + el] = a;
+                sUq[(qq * 4 + s) * TE + cs] = a;
             }
         }
         __syncthreads();
@@ -566,7 +557,7 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
             if (q < NQV && plive) {
                 double uq[4];
 #pragma unroll
-                for (int c = 0; c < 4; ++c) uq[c] = sUq[(qq * 4 + c) * TP + pe];
+                for (int c = 0; c < 4; ++c) uq[c] = sUq[(qq * 4 + c) * TE + swz(pe, c)];
                 const Prim<double> w = primitives(uq, P.gamma);
                 const double F[4] = {w.r * w.u, w.r * w.u * w.u + w.p, w.r * w.u * w.v, w.r * w.u * w.H};
                 const double G[4] = {w.r * w.v, w.r * w.u * w.v, w.r * w.v * w.v + w.p, w.r * w.v * w.H};
@@ -594,8 +585,8 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
                         a *= wd;
                         b *= wd;
                     }
-                    sFx[(qq * 4 + c) * TP + pe] = a;
-                    sFy[(qq * 4 + c) * TP + pe] = b;
+                    sUq[(qq * 4 + c) * TE + swz(pe, c)] = a;
+                    sFy[(qq * 4 + c) * TE + swz(pe, c)] = b;
                 }
             }
         }
@@ -604,18 +595,19 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
         for (int qq = 0; qq < L::QR; ++qq) {
             const int q = q0 + qq;
             if (q < NQV) {
-                const double a = sFx[(qq * 4 + s) * TP + el], b = sFy[(qq * 4 + s) * TP + el];
+                const double a = sUq[(qq * 4 + s) * TE + cs], b = sFy[(qq * 4 + s) * TE + cs];
 #pragma unroll
                 for (int i = 0; i < N; ++i)
                     acc[i] -= T[vx_off + i * NQV + q] * a + T[vy_off + i * NQV + q] * b;
             }
         }
-        // the next round writes sUq only; sFx/sFy are rewritten after its next barrier
+        // the next round writes the other state buffer; the eta-flux rows are
+        // rewritten only after its first barrier
     }
 
     // ------------------------------------------------------------ epilogue
     // acc holds R (unfolded) or -rate (affine, folded) for component s
-    if (mode != kRes && !C::AFF && live) {
+    if ((STAGE || mode != kRes) && !C::AFF && live) {
         double r[N];
 #pragma unroll
         for (int i = 0; i < N; ++i) {
@@ -629,16 +621,15 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
         for (int i = 0; i < N; ++i) acc[i] = r[i];
     }
     __syncthreads();  // last reads of the volume rows are done; reuse sW for staging
-    double* sIO = sW;
-    if (mode == kRes || mode == kRate) {
+    if (!STAGE) {
         const double osign = mode == kRes ? 1.0 : -1.0;  // rate = -M^-1 R
 #pragma unroll
-        for (int i = 0; i < N; ++i) sIO[(i * 4 + s) * TP + el] = osign * acc[i];
+        for (int i = 0; i < N; ++i) sW[el * L::SIO + i * 4 + s] = osign * acc[i];
         if (s == 0 && live) dt[e] = wave;
         __syncthreads();
         for (int it = t; it < TE * NS; it += BS) {
             const int r = it / NS, c = it - r * NS;
-            if (r < ne) Uout[(e0 + r) * NS + c] = sIO[c * TP + r];
+            if (r < ne) Uout[(e0 + r) * NS + c] = sW[r * L::SIO + c];
         }
         return;
     }
@@ -673,6 +664,9 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
     if (P.limiter == 1 && N > 1) {  // superbee runs as separate passes
         uint8_t fl = 0;
         if (live) {
+            int sidebits = 0;
+#pragma unroll
+            for (int f = 0; f < NF; ++f) sidebits |= ((sIn[el * NF + f] >> 2) & 1) << f;
             double mw[N];
 #pragma unroll
             for (int i = 0; i < N; ++i) mw[i] = C::AFF ? T[P.off.mean_w + i] : P.mean_w[e * N + i];
@@ -688,11 +682,11 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
         flags[e] = 0;
     }
 #pragma unroll
-    for (int i = 0; i < N; ++i) sIO[(i * 4 + s) * TP + el] = u[i];
+    for (int i = 0; i < N; ++i) sW[el * L::SIO + i * 4 + s] = u[i];
     __syncthreads();
     for (int it = t; it < TE * NS; it += BS) {
         const int r = it / NS, c = it - r * NS;
-        if (r < ne) Uout[(e0 + r) * NS + c] = sIO[c * TP + r];
+        if (r < ne) Uout[(e0 + r) * NS + c] = sW[r * L::SIO + c];
     }
 }
 
@@ -1327,15 +1321,21 @@ struct Launch {
             hipLaunchKernelGGL((nz::k_bflux<C>), dim3(static_cast<unsigned>((nbq + nz::BS - 1) / nz::BS)),
                                dim3(nz::BS), 0, st, P, P.tables, Uin);
         const dim3 g(static_cast<unsigned>(nblk_dg));
-        if (P.flux_id == 1)
-            hipLaunchKernelGGL((nz::k_dg<C, 1>), g, dim3(nz::BS), 0, st, P, P.tables, Uin, Un, Uout,
-                               dt, fac, a, b, mode, partial, flags, counters);
-        else if (P.flux_id == 2)
-            hipLaunchKernelGGL((nz::k_dg<C, 2>), g, dim3(nz::BS), 0, st, P, P.tables, Uin, Un, Uout,
-                               dt, fac, a, b, mode, partial, flags, counters);
-        else
-            hipLaunchKernelGGL((nz::k_dg<C, 0>), g, dim3(nz::BS), 0, st, P, P.tables, Uin, Un, Uout,
-                               dt, fac, a, b, mode, partial, flags, counters);
+        const bool stage = mode == nz::kStageFirst || mode == nz::kStageNext;
+#define NZ_DG(FL, ST)                                                                           \
+    hipLaunchKernelGGL((nz::k_dg<C, FL, ST>), g, dim3(nz::BS), 0, st, P, P.tables, Uin, Un, Uout, \
+                       dt, fac, a, b, mode, partial, flags, counters)
+        if (P.flux_id == 1) {
+            if (stage) NZ_DG(1, true);
+            else NZ_DG(1, false);
+        } else if (P.flux_id == 2) {
+            if (stage) NZ_DG(2, true);
+            else NZ_DG(2, false);
+        } else {
+            if (stage) NZ_DG(0, true);
+            else NZ_DG(0, false);
+        }
+#undef NZ_DG
     }
     static void limit(const nz_problem& P, double* U, uint8_t* flags, unsigned long long* counters,
                       hipStream_t st) {
