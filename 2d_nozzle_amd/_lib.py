@@ -20,7 +20,7 @@ NZ_OK = 0
 class TableOffsets(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
         "phi_vol", "dxi", "deta", "axi", "aeta", "phi_face", "lift", "w_vol", "w_face",
-        "mean_w", "minv_ref", "pad")]
+        "mean_w", "minv_ref", "vol_t")]
 
 
 class Problem(C.Structure):

@@ -63,6 +63,8 @@ def _tables(rd) -> tuple[np.ndarray, dict]:
         "mean_w": (phi @ w) / w.sum(),
         "minv_ref": minv_ref,
     }
+    # point-major copies of the volume tables: one contiguous block per point
+    parts["vol_t"] = np.stack([phi, dxi, deta, parts["axi"], parts["aeta"]], axis=0).transpose(2, 0, 1)
     off, chunks, pos = {}, [], 0
     for k, v in parts.items():
         v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)

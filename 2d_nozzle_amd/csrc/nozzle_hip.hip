@@ -537,8 +537,9 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
 
     // --------------------------------------------------------------- volume
     double* sFy = sW + 2 * L::QR * PT;  // [QR][4][TE] eta-flux
-    const int vx_off = fold ? P.off.axi : P.off.dxi;
-    const int vy_off = fold ? P.off.aeta : P.off.deta;
+    // point-major volume tables: phi, dxi, deta, axi, aeta (N each) per point
+    const double* vt = T + P.off.vol_t;
+    const int vproj = fold ? 3 * N : N;
 #pragma unroll 1
     for (int q0 = 0; q0 < NQV; q0 += L::QR) {
         double* sUq = sW + ((q0 / L::QR) & 1) * L::QR * PT;  // point states, then xi-flux
@@ -546,8 +547,9 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
         for (int qq = 0; qq < L::QR; ++qq) {
             if (q0 + qq < NQV) {
                 double a = 0.0;
+                const double* ph = vt + (q0 + qq) * 5 * N;
 #pragma unroll
-                for (int i = 0; i < N; ++i) a += T[P.off.phi_vol + i * NQV + q0 + qq] * u[i];
+                for (int i = 0; i < N; ++i) a += ph[i] * u[i];
                 sUq[(qq * 4 + s) * TE + cs] = a;
             }
         }
@@ -596,9 +598,12 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
             const int q = q0 + qq;
             if (q < NQV) {
                 const double a = sUq[(qq * 4 + s) * TE + cs], b = sFy[(qq * 4 + s) * TE + cs];
+                const double* pr = vt + q * 5 * N + vproj;
 #pragma unroll
-                for (int i = 0; i < N; ++i)
-                    acc[i] -= T[vx_off + i * NQV + q] * a + T[vy_off + i * NQV + q] * b;
+                for (int i = 0; i < N; ++i) {
+                    acc[i] = fma(-pr[i], a, acc[i]);
+                    acc[i] = fma(-pr[N + i], b, acc[i]);
+                }
             }
         }
         // the next round writes the other state buffer; the eta-flux rows are

@@ -42,7 +42,7 @@ typedef struct {
     int32_t w_face;    /* (nqf,)                                                  */
     int32_t mean_w;    /* (N,)  cell-average weights of an affine element         */
     int32_t minv_ref;  /* (N, N) inverse reference mass matrix (affine)           */
-    int32_t pad;
+    int32_t vol_t;     /* (nqv, 5, N) point-major copies: phi, dxi, deta, axi, aeta */
 } nz_table_offsets;
 
 /* Everything one mesh/order/operating point needs.  Filled on the host by
