@@ -198,15 +198,15 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 }
 
 // ---------------------------------------------------------------- DG kernel
-// Component-split mapping: a workgroup of 256 threads owns 64 consecutive
+// Component-split mapping: a workgroup of one wave owns 16 consecutive
 // elements and gives each element four lanes, one per conserved variable
 // (lane = 4 * element + s).  Everything linear in U -- traces, volume-point
 // interpolation, lifting, volume projection, the stage combination and the
 // limiter's probe sums -- is separable by component, so each lane keeps its
 // element's coefficient column U[:, s] and its accumulator R[:, s] in
 // registers and reads reference tables with scalar loads.  Only the
-// pointwise physics needs all four components; it runs on (element, point)
-// items with one point per wave, exchanging point states through LDS rows
+// pointwise physics needs all four components; it runs on (point, element)
+// items, one per lane, exchanging point states through LDS rows
 // [point][component][element].  The limiter gathers the other components of
 // its element with quad shuffles.
 //
@@ -216,23 +216,27 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 // reads its canonical index q and the side-1 element nqf-1-q; the flux item
 // writes the weighted flux back at the element's canonical index and the
 // lift is side-independent as well.
-constexpr int TE = 64;  // elements per workgroup
+constexpr int DG_BS = 64;  // one wave per workgroup: no cross-wave barriers
+constexpr int TE = 16;     // elements per wave tile (four lanes each)
+constexpr int QS = 80;     // LDS doubles per point: four component rows of TE, = 16 mod 64
 
-// Column of element `el` in an LDS row of component c.  The XOR keeps both
-// access patterns free of bank conflicts: four lanes per element
-// (lane = 4 el + c) and one lane per element (lane = el, c fixed).
-__device__ __forceinline__ int swz(int el, int c) { return el ^ (((c & 1) << 4) + (c >> 1) * 40); }
+// LDS slot of component c of tile element el at point q.  Rows are TE wide;
+// the XOR on the element index and the point stride keep both access
+// patterns free of bank conflicts: four lanes per element (lane = 4 el + c)
+// and one lane per (point, element) (lane = TE q + el, c fixed).
+__device__ __forceinline__ int at(int q, int c, int el) {
+    return q * QS + c * TE + (el ^ ((c >> 1) << 3));
+}
 
 template <class C>
 struct Tile {
     static constexpr int N = C::N, NQV = C::NQV, NQF = C::NQF, NF = C::NF, NS = C::NS;
-    static constexpr int QR = BS / 64;  // volume points per round: one per wave
-    static constexpr int PT = 4 * TE;   // one point: four component rows
+    static constexpr int QR = DG_BS / TE;  // volume points per round: one per lane
     // own trace / weighted flux (two buffers, alternating faces), neighbour
     // trace, signal speeds
-    static constexpr int FACE = 3 * NQF * PT + NQF * TE;
+    static constexpr int FACE = 3 * NQF * QS + NQF * TE;
     // point states / xi-flux (two buffers, alternating rounds), eta-flux
-    static constexpr int VOL = 3 * QR * PT;
+    static constexpr int VOL = 3 * QR * QS;
     // staging of U in and out: rows of one element, stride = 4 mod 64 doubles
     static constexpr int SIO = NS <= 4 ? 4 : ((NS - 4 + 63) / 64) * 64 + 4;
     static constexpr int IO = TE * SIO;
@@ -368,7 +372,7 @@ __device__ uint8_t limit_column(double* col, int s, const double* mw, const doub
 }
 
 template <class C, int FLUX, bool STAGE>
-__global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __restrict__ T,
+__global__ __launch_bounds__(DG_BS) void k_dg(const nz_problem P, const double* __restrict__ T,
                                            const double* __restrict__ Uin,
                                            const double* __restrict__ Un,
                                            double* __restrict__ Uout, double* __restrict__ dt,
@@ -379,34 +383,30 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
     using L = Tile<C>;
     constexpr int N = C::N, NQV = C::NQV, NQF = C::NQF, NF = C::NF, NS = C::NS;
     constexpr int NPF = NF * N * NQF;  // canonical (side-0) face table
-    constexpr int PT = L::PT;
     __shared__ double sW[L::WORK];
     __shared__ double sPF[NPF];
     __shared__ int sNb[TE * NF], sIn[TE * NF];
-    __shared__ double sRed[BS / 64];
 
     const int t = threadIdx.x;
-    const int el = t >> 2, s = t & 3;       // component lanes
-    const int pe = t & (TE - 1);            // point items: element ...
-    const int slot = wave_slot();           // ... and wave-uniform point slot
+    const int el = t >> 2, s = t & 3;  // component lanes
     const int64_t e0 = static_cast<int64_t>(blockIdx.x) * TE;
     const int ne = static_cast<int>(P.n_elem - e0 < TE ? P.n_elem - e0 : TE);
-    const bool live = el < ne, plive = pe < ne;
-    const int64_t e = e0 + el, ep = e0 + pe;
+    const bool live = el < ne;
+    const int64_t e = e0 + el;
     const bool fold = C::AFF && (STAGE || mode != kRes);  // affine rate: M^-1 folded into tables
-    const int cs = swz(el, s);
+    const int cs = at(0, s, el);  // this lane's slot at point 0
 
     // ---- stage the tile (coalesced), the face maps and the face table
-    for (int it = t; it < TE * NS; it += BS) {
+    for (int it = t; it < TE * NS; it += DG_BS) {
         const int r = it / NS, c = it - r * NS;
         sW[r * L::SIO + c] = r < ne ? Uin[(e0 + r) * NS + c] : 0.0;
     }
-    for (int it = t; it < TE * NF; it += BS) {
+    for (int it = t; it < TE * NF; it += DG_BS) {
         const bool ok = it < ne * NF;
         sNb[it] = ok ? P.fe_nbr[e0 * NF + it] : -1;
         sIn[it] = ok ? P.fe_info[e0 * NF + it] : 0;
     }
-    for (int it = t; it < NPF; it += BS) sPF[it] = T[P.off.phi_face + it];
+    for (int it = t; it < NPF; it += DG_BS) sPF[it] = T[P.off.phi_face + it];
     __syncthreads();
     double u[N], acc[N];
 #pragma unroll
@@ -417,14 +417,14 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
     double wave = 0.0;
     __syncthreads();  // the staging rows are reused below
 
-    double* sTn = sW + 2 * NQF * PT;  // [NQF][4][TE] neighbour trace
-    double* sSp = sW + 3 * NQF * PT;  // [NQF][TE]    signal speed
+    double* sTn = sW + 2 * NQF * QS;  // neighbour trace
+    double* sSp = sW + 3 * NQF * QS;  // [NQF][TE] signal speed
     const int lift_off = fold ? P.off.lift : P.off.phi_face;
 
     // ---------------------------------------------------------------- faces
 #pragma unroll 1
     for (int f = 0; f < NF; ++f) {
-        double* sTo = sW + (f & 1) * NQF * PT;  // [NQF][4][TE] own trace, then weighted flux
+        double* sTo = sW + (f & 1) * NQF * QS;  // own trace, then weighted flux
         const int nb = sNb[el * NF + f], inf = sIn[el * NF + f];
         // (e,s): own and neighbour traces of component s, canonical order
         {
@@ -434,7 +434,7 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
                 double a = 0.0;
 #pragma unroll
                 for (int i = 0; i < N; ++i) a += b[i * NQF + q] * u[i];
-                sTo[(q * 4 + s) * TE + cs] = a;
+                sTo[q * QS + cs] = a;
             }
         }
         {
@@ -466,12 +466,14 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
                 }
             }
 #pragma unroll
-            for (int q = 0; q < NQF; ++q) sTn[(q * 4 + s) * TE + cs] = tn[q];
+            for (int q = 0; q < NQF; ++q) sTn[q * QS + cs] = tn[q];
         }
         __syncthreads();
-        // (e,q): the flux at edge point q, one point per wave
-        for (int q = slot; q < NQF; q += BS / 64) {
-            if (plive) {
+        // (q,e): the flux at edge point q, one item per lane
+        for (int it = t; it < NQF * TE; it += DG_BS) {
+            const int q = it / TE, pe = it % TE;
+            const int64_t ep = e0 + pe;
+            if (pe < ne) {
                 const int nbp = sNb[pe * NF + f], infp = sIn[pe * NF + f];
                 const int sd = (infp >> 2) & 1;
                 const int64_t k = infp >> 3;
@@ -482,8 +484,8 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
                     double ul[4], ur[4];
 #pragma unroll
                     for (int c = 0; c < 4; ++c) {
-                        const double a = sTo[(qo * 4 + c) * TE + swz(pe, c)];
-                        const double b = sTn[(qn * 4 + c) * TE + swz(pe, c)];
+                        const double a = sTo[at(qo, c, pe)];
+                        const double b = sTn[at(qn, c, pe)];
                         ul[c] = sd == 0 ? a : b;
                         ur[c] = sd == 0 ? b : a;
                     }
@@ -502,7 +504,7 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
                 double sg = sd ? -1.0 : 1.0;
                 if (fold) sg = sg / P.geo[ep * 5 + 4];
 #pragma unroll
-                for (int c = 0; c < 4; ++c) sTo[(qo * 4 + c) * TE + swz(pe, c)] = sg * fx[c];
+                for (int c = 0; c < 4; ++c) sTo[at(qo, c, pe)] = sg * fx[c];
                 sSp[q * TE + pe] = sp;
             }
         }
@@ -511,7 +513,7 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
         {
             double fw[NQF];
 #pragma unroll
-            for (int q = 0; q < NQF; ++q) fw[q] = sTo[(q * 4 + s) * TE + cs];
+            for (int q = 0; q < NQF; ++q) fw[q] = sTo[q * QS + cs];
             const double* lb = T + lift_off + f * N * NQF;
 #pragma unroll
             for (int i = 0; i < N; ++i) {
@@ -536,13 +538,13 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
     __syncthreads();
 
     // --------------------------------------------------------------- volume
-    double* sFy = sW + 2 * L::QR * PT;  // [QR][4][TE] eta-flux
+    double* sFy = sW + 2 * L::QR * QS;  // eta-flux
     // point-major volume tables: phi, dxi, deta, axi, aeta (N each) per point
     const double* vt = T + P.off.vol_t;
     const int vproj = fold ? 3 * N : N;
 #pragma unroll 1
     for (int q0 = 0; q0 < NQV; q0 += L::QR) {
-        double* sUq = sW + ((q0 / L::QR) & 1) * L::QR * PT;  // point states, then xi-flux
+        double* sUq = sW + ((q0 / L::QR) & 1) * L::QR * QS;  // point states, then xi-flux
 #pragma unroll 1
         for (int qq = 0; qq < L::QR; ++qq) {
             if (q0 + qq < NQV) {
@@ -550,16 +552,17 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
                 const double* ph = vt + (q0 + qq) * 5 * N;
 #pragma unroll
                 for (int i = 0; i < N; ++i) a += ph[i] * u[i];
-                sUq[(qq * 4 + s) * TE + cs] = a;
+                sUq[qq * QS + cs] = a;
             }
         }
         __syncthreads();
         {
-            const int qq = slot, q = q0 + qq;
-            if (q < NQV && plive) {
+            const int qq = t / TE, pe = t % TE, q = q0 + qq;
+            const int64_t ep = e0 + pe;
+            if (q < NQV && pe < ne) {
                 double uq[4];
 #pragma unroll
-                for (int c = 0; c < 4; ++c) uq[c] = sUq[(qq * 4 + c) * TE + swz(pe, c)];
+                for (int c = 0; c < 4; ++c) uq[c] = sUq[at(qq, c, pe)];
                 const Prim<double> w = primitives(uq, P.gamma);
                 const double F[4] = {w.r * w.u, w.r * w.u * w.u + w.p, w.r * w.u * w.v, w.r * w.u * w.H};
                 const double G[4] = {w.r * w.v, w.r * w.u * w.v, w.r * w.v * w.v + w.p, w.r * w.v * w.H};
@@ -587,8 +590,8 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
                         a *= wd;
                         b *= wd;
                     }
-                    sUq[(qq * 4 + c) * TE + swz(pe, c)] = a;
-                    sFy[(qq * 4 + c) * TE + swz(pe, c)] = b;
+                    sUq[at(qq, c, pe)] = a;
+                    sFy[at(qq, c, pe)] = b;
                 }
             }
         }
@@ -597,7 +600,7 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
         for (int qq = 0; qq < L::QR; ++qq) {
             const int q = q0 + qq;
             if (q < NQV) {
-                const double a = sUq[(qq * 4 + s) * TE + cs], b = sFy[(qq * 4 + s) * TE + cs];
+                const double a = sUq[qq * QS + cs], b = sFy[qq * QS + cs];
                 const double* pr = vt + q * 5 * N + vproj;
 #pragma unroll
                 for (int i = 0; i < N; ++i) {
@@ -632,7 +635,7 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
         for (int i = 0; i < N; ++i) sW[el * L::SIO + i * 4 + s] = osign * acc[i];
         if (s == 0 && live) dt[e] = wave;
         __syncthreads();
-        for (int it = t; it < TE * NS; it += BS) {
+        for (int it = t; it < TE * NS; it += DG_BS) {
             const int r = it / NS, c = it - r * NS;
             if (r < ne) Uout[(e0 + r) * NS + c] = sW[r * L::SIO + c];
         }
@@ -663,7 +666,8 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
         }
     }
     if (mode == kStageFirst) {
-        const double tot = block_sum(nacc, sRed);
+        double tot = nacc;
+        for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o, 64);
         if (t == 0) partial[blockIdx.x] = tot;
     }
     if (P.limiter == 1 && N > 1) {  // superbee runs as separate passes
@@ -689,7 +693,7 @@ __global__ __launch_bounds__(BS) void k_dg(const nz_problem P, const double* __r
 #pragma unroll
     for (int i = 0; i < N; ++i) sW[el * L::SIO + i * 4 + s] = u[i];
     __syncthreads();
-    for (int it = t; it < TE * NS; it += BS) {
+    for (int it = t; it < TE * NS; it += DG_BS) {
         const int r = it / NS, c = it - r * NS;
         if (r < ne) Uout[(e0 + r) * NS + c] = sW[r * L::SIO + c];
     }
@@ -1265,14 +1269,33 @@ __global__ __launch_bounds__(nz::BS) void k_norm_partial(const double* __restric
     if (threadIdx.x == 0) partial[blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(nz::BS) void k_norm_finish(const double* __restrict__ partial,
+// One workgroup sums the per-tile partials in a fixed order (four independent
+// accumulators per thread keep loads in flight), so the norm is reproducible.
+constexpr int FIN_BS = 1024;
+
+__global__ __launch_bounds__(FIN_BS) void k_norm_finish(const double* __restrict__ partial,
                                                         int64_t nparts, double count,
                                                         double* __restrict__ out) {
-    __shared__ double red[nz::BS / 64];
-    double acc = 0.0;
-    for (int64_t it = threadIdx.x; it < nparts; it += nz::BS) acc += partial[it];
-    const double tot = nz::block_sum(acc, red);
-    if (threadIdx.x == 0) out[0] = sqrt(tot / count);
+    __shared__ double red[FIN_BS / 64];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int64_t it = threadIdx.x;
+    for (; it + 3 * FIN_BS < nparts; it += 4 * FIN_BS) {
+        a0 += partial[it];
+        a1 += partial[it + FIN_BS];
+        a2 += partial[it + 2 * FIN_BS];
+        a3 += partial[it + 3 * FIN_BS];
+    }
+    for (; it < nparts; it += FIN_BS) a0 += partial[it];
+    double v = (a0 + a1) + (a2 + a3);
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int t = threadIdx.x;
+    if ((t & 63) == 0) red[t >> 6] = v;
+    __syncthreads();
+    if (t == 0) {
+        double tot = 0.0;
+        for (int w = 0; w < FIN_BS / 64; ++w) tot += red[w];
+        out[0] = sqrt(tot / count);
+    }
 }
 
 __global__ __launch_bounds__(nz::BS) void k_time_step(const double* __restrict__ wave,
@@ -1320,15 +1343,15 @@ struct Launch {
                    double* dt, double fac, double a, double b, int mode, double* partial,
                    uint8_t* flags, unsigned long long* counters, int64_t nblk, hipStream_t st) {
         (void)nblk;  // k_dg tiles are nz::TE elements
-        const int64_t nblk_dg = (P.n_elem + nz::TE - 1) / nz::TE;
+        const int64_t nblk_dg = (P.n_elem + nz::TE - 1) / nz::TE;  // one wave each
         const int64_t nbq = (P.n_edge - P.n_int) * C::NQF;
         if (nbq > 0)
             hipLaunchKernelGGL((nz::k_bflux<C>), dim3(static_cast<unsigned>((nbq + nz::BS - 1) / nz::BS)),
                                dim3(nz::BS), 0, st, P, P.tables, Uin);
         const dim3 g(static_cast<unsigned>(nblk_dg));
         const bool stage = mode == nz::kStageFirst || mode == nz::kStageNext;
-#define NZ_DG(FL, ST)                                                                           \
-    hipLaunchKernelGGL((nz::k_dg<C, FL, ST>), g, dim3(nz::BS), 0, st, P, P.tables, Uin, Un, Uout, \
+#define NZ_DG(FL, ST)                                                                              \
+    hipLaunchKernelGGL((nz::k_dg<C, FL, ST>), g, dim3(nz::DG_BS), 0, st, P, P.tables, Uin, Un, Uout, \
                        dt, fac, a, b, mode, partial, flags, counters)
         if (P.flux_id == 1) {
             if (stage) NZ_DG(1, true);
@@ -1494,7 +1517,7 @@ int nz_norm_finish(const nz_problem* p, const double* scratch, double* out, void
     if (rc) return rc;
     if (!scratch || !out) return fail(NZ_E_ARG, "null array");
     const double count = static_cast<double>(p->n_elem) * p->nbf * 4.0;
-    hipLaunchKernelGGL(k_norm_finish, dim3(1), dim3(nz::BS), 0, static_cast<hipStream_t>(stream),
+    hipLaunchKernelGGL(k_norm_finish, dim3(1), dim3(FIN_BS), 0, static_cast<hipStream_t>(stream),
                        scratch, n_blocks_dg(p), count, out);
     return check_launch("nz_norm_finish");
 }
@@ -1510,7 +1533,7 @@ int nz_norm(const nz_problem* p, const double* A, double* scratch, double* out, 
     hipStream_t st = static_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(k_norm_partial, dim3(static_cast<unsigned>(nb)), dim3(nz::BS), 0, st, A, n,
                        p->wnorm[0], p->wnorm[1], p->wnorm[2], p->wnorm[3], scratch);
-    hipLaunchKernelGGL(k_norm_finish, dim3(1), dim3(nz::BS), 0, st, scratch, nb,
+    hipLaunchKernelGGL(k_norm_finish, dim3(1), dim3(FIN_BS), 0, st, scratch, nb,
                        static_cast<double>(n), out);
     return check_launch("nz_norm");
 }
@@ -1568,7 +1591,7 @@ int nz_adjoint_defect(const nz_problem* p, const double* U, const double* lam, c
     const int64_t nb = n_blocks(p, o);
     o.vjp(*p, U, lam, scratch, nullptr, nullptr, nullptr, work, nz::kAdjDefect, seed, nullptr,
           nullptr, nullptr, nullptr, 0, nb, st);
-    hipLaunchKernelGGL(k_norm_finish, dim3(1), dim3(nz::BS), 0, st, scratch, nb,
+    hipLaunchKernelGGL(k_norm_finish, dim3(1), dim3(FIN_BS), 0, st, scratch, nb,
                        static_cast<double>(p->n_elem) * p->nbf * 4.0, out);
     return check_launch("nz_adjoint_defect");
 }
