@@ -237,10 +237,17 @@ struct Tile {
     static constexpr int FACE = 3 * NQF * QS + NQF * TE;
     // point states / xi-flux (two buffers, alternating rounds), eta-flux
     static constexpr int VOL = 3 * QR * QS;
-    // staging of U in and out: rows of one element, stride = 4 mod 64 doubles
-    static constexpr int SIO = NS <= 4 ? 4 : ((NS - 4 + 63) / 64) * 64 + 4;
+    // staging of U in and out: rows of one element; a stride of 4 x odd
+    // doubles keeps the per-component reads conflict-free
+    static constexpr int SIO = ((NS + 3) / 4) % 2 ? ((NS + 3) / 4) * 4 : ((NS + 3) / 4 + 1) * 4;
     static constexpr int IO = TE * SIO;
     static constexpr int WORK = FACE > VOL ? (FACE > IO ? FACE : IO) : (VOL > IO ? VOL : IO);
+    // staged once per tile: edge geometry of every element face (nx, ny,
+    // |dX/dsigma| per edge, or per edge point when curved) and the affine
+    // metrics
+    static constexpr int GF = C::STR ? 3 : 3 * NQF;
+    static constexpr int EGN = TE * NF * GF;
+    static constexpr int GEN = C::AFF ? TE * 5 : 1;
 };
 
 // component k of this lane's element (lanes 4e..4e+3 hold s = 0..3)
@@ -385,6 +392,8 @@ __global__ __launch_bounds__(DG_BS) void k_dg(const nz_problem P, const double* 
     constexpr int NPF = NF * N * NQF;  // canonical (side-0) face table
     __shared__ double sW[L::WORK];
     __shared__ double sPF[NPF];
+    __shared__ double sEG[L::EGN];
+    __shared__ double sGe[L::GEN];
     __shared__ int sNb[TE * NF], sIn[TE * NF];
 
     const int t = threadIdx.x;
@@ -396,18 +405,74 @@ __global__ __launch_bounds__(DG_BS) void k_dg(const nz_problem P, const double* 
     const bool fold = C::AFF && (STAGE || mode != kRes);  // affine rate: M^-1 folded into tables
     const int cs = at(0, s, el);  // this lane's slot at point 0
 
-    // ---- stage the tile (coalesced), the face maps and the face table
-    for (int it = t; it < TE * NS; it += DG_BS) {
-        const int r = it / NS, c = it - r * NS;
-        sW[r * L::SIO + c] = r < ne ? Uin[(e0 + r) * NS + c] : 0.0;
+    // ---- stage the tile (coalesced), the face maps, the face table and the
+    // affine metrics; every load is issued before the first LDS write
+    {
+        constexpr int NU = TE * NS / DG_BS;  // = N
+        constexpr int NM = (TE * NF + DG_BS - 1) / DG_BS;
+        constexpr int NT = (NPF + DG_BS - 1) / DG_BS;
+        constexpr int NG = (L::GEN + DG_BS - 1) / DG_BS;
+        double ru[NU], rt[NT], rg[NG];
+        int rn[NM], ri[NM];
+#pragma unroll
+        for (int j = 0; j < NU; ++j) {
+            const int it = t + j * DG_BS, r = it / NS;
+            ru[j] = r < ne ? Uin[e0 * NS + it] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < NM; ++j) {
+            const int it = t + j * DG_BS;
+            const bool ok = it < ne * NF;
+            rn[j] = ok ? P.fe_nbr[e0 * NF + it] : -1;
+            ri[j] = ok ? P.fe_info[e0 * NF + it] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+            const int it = t + j * DG_BS;
+            rt[j] = it < NPF ? T[P.off.phi_face + it] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < NG; ++j) {
+            const int it = t + j * DG_BS;
+            rg[j] = C::AFF && it < ne * 5 ? P.geo[e0 * 5 + it] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < NU; ++j) {
+            const int it = t + j * DG_BS, r = it / NS, c = it - r * NS;
+            sW[r * L::SIO + c] = ru[j];
+        }
+#pragma unroll
+        for (int j = 0; j < NM; ++j) {
+            const int it = t + j * DG_BS;
+            if (it < TE * NF) {
+                sNb[it] = rn[j];
+                sIn[it] = ri[j];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NT; ++j)
+            if (t + j * DG_BS < NPF) sPF[t + j * DG_BS] = rt[j];
+#pragma unroll
+        for (int j = 0; j < NG; ++j)
+            if (t + j * DG_BS < L::GEN) sGe[t + j * DG_BS] = rg[j];
     }
-    for (int it = t; it < TE * NF; it += DG_BS) {
-        const bool ok = it < ne * NF;
-        sNb[it] = ok ? P.fe_nbr[e0 * NF + it] : -1;
-        sIn[it] = ok ? P.fe_info[e0 * NF + it] : 0;
-    }
-    for (int it = t; it < NPF; it += DG_BS) sPF[it] = T[P.off.phi_face + it];
     __syncthreads();
+    // edge geometry of the tile's faces, gathered by edge id
+    {
+        constexpr int NE_ = (L::EGN + DG_BS - 1) / DG_BS;
+        double rq[NE_];
+#pragma unroll
+        for (int j = 0; j < NE_; ++j) {
+            const int it = t + j * DG_BS;
+            const int r = it / L::GF, jj = it - r * L::GF;  // r = element-face
+            const bool ok = it < L::EGN && r < ne * NF;
+            const int64_t k = ok ? (sIn[r] >> 3) : 0;
+            rq[j] = ok ? P.edge_geo[k * L::GF + jj] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < NE_; ++j)
+            if (t + j * DG_BS < L::EGN) sEG[t + j * DG_BS] = rq[j];
+    }
     double u[N], acc[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) {
@@ -426,6 +491,16 @@ __global__ __launch_bounds__(DG_BS) void k_dg(const nz_problem P, const double* 
     for (int f = 0; f < NF; ++f) {
         double* sTo = sW + (f & 1) * NQF * QS;  // own trace, then weighted flux
         const int nb = sNb[el * NF + f], inf = sIn[el * NF + f];
+        const bool has = live && nb >= 0;
+        // the neighbour's column of component s, requested before the own
+        // trace is formed (lanes without a neighbour read the tile's first
+        // element and discard it)
+        double xn[N];
+        {
+            const double* ur = Uin + (has ? static_cast<int64_t>(nb) : e0) * NS + s;
+#pragma unroll
+            for (int i = 0; i < N; ++i) xn[i] = ur[i * 4];
+        }
         // (e,s): own and neighbour traces of component s, canonical order
         {
             const double* b = T + P.off.phi_face + f * N * NQF;
@@ -444,25 +519,21 @@ __global__ __launch_bounds__(DG_BS) void k_dg(const nz_problem P, const double* 
             double tn[NQF];
 #pragma unroll
             for (int q = 0; q < NQF; ++q) tn[q] = 0.0;
-            const bool has = live && nb >= 0;
             const int key = has ? (inf & 3) : -1;
             const int key0 = __builtin_amdgcn_readfirstlane(key);
-            const double* ur = Uin + static_cast<int64_t>(has ? nb : 0) * NS + s;
             if (__all(key == key0) && key0 >= 0) {
                 const double* ph = T + P.off.phi_face + key0 * N * NQF;
 #pragma unroll
                 for (int i = 0; i < N; ++i) {
-                    const double x = ur[i * 4];
 #pragma unroll
-                    for (int q = 0; q < NQF; ++q) tn[q] += ph[i * NQF + q] * x;
+                    for (int q = 0; q < NQF; ++q) tn[q] += ph[i * NQF + q] * xn[i];
                 }
             } else if (has) {
                 const double* ph = sPF + key * N * NQF;
-#pragma unroll 2
-                for (int i = 0; i < N; ++i) {
-                    const double x = ur[i * 4];
 #pragma unroll
-                    for (int q = 0; q < NQF; ++q) tn[q] += ph[i * NQF + q] * x;
+                for (int i = 0; i < N; ++i) {
+#pragma unroll
+                    for (int q = 0; q < NQF; ++q) tn[q] += ph[i * NQF + q] * xn[i];
                 }
             }
 #pragma unroll
@@ -472,7 +543,6 @@ __global__ __launch_bounds__(DG_BS) void k_dg(const nz_problem P, const double* 
         // (q,e): the flux at edge point q, one item per lane
         for (int it = t; it < NQF * TE; it += DG_BS) {
             const int q = it / TE, pe = it % TE;
-            const int64_t ep = e0 + pe;
             if (pe < ne) {
                 const int nbp = sNb[pe * NF + f], infp = sIn[pe * NF + f];
                 const int sd = (infp >> 2) & 1;
@@ -489,7 +559,7 @@ __global__ __launch_bounds__(DG_BS) void k_dg(const nz_problem P, const double* 
                         ul[c] = sd == 0 ? a : b;
                         ur[c] = sd == 0 ? b : a;
                     }
-                    const double* eg = C::STR ? P.edge_geo + k * 3 : P.edge_geo + (k * NQF + q) * 3;
+                    const double* eg = sEG + (pe * NF + f) * L::GF + (C::STR ? 0 : q * 3);
                     const double nx = eg[0], ny = eg[1];
                     sp = interface_flux<FLUX>(P, ul, ur, nx, ny, fx);
                     const double sc = eg[2] * T[P.off.w_face + q];
@@ -502,7 +572,7 @@ __global__ __launch_bounds__(DG_BS) void k_dg(const nz_problem P, const double* 
                     sp = bf[4];
                 }
                 double sg = sd ? -1.0 : 1.0;
-                if (fold) sg = sg / P.geo[ep * 5 + 4];
+                if (fold) sg = sg / sGe[pe * 5 + 4];
 #pragma unroll
                 for (int c = 0; c < 4; ++c) sTo[at(qo, c, pe)] = sg * fx[c];
                 sSp[q * TE + pe] = sp;
@@ -521,13 +591,12 @@ __global__ __launch_bounds__(DG_BS) void k_dg(const nz_problem P, const double* 
                 for (int q = 0; q < NQF; ++q) acc[i] += lb[i * NQF + q] * fw[q];
             }
             if (live) {
-                const int64_t k = inf >> 3;
+                const double* eg = sEG + (el * NF + f) * L::GF;
                 double smax = 0.0, len = 0.0;
                 for (int q = 0; q < NQF; ++q) {
                     const double sp = sSp[q * TE + el];
                     smax = sp > smax ? sp : smax;
-                    const double jac = C::STR ? P.edge_geo[k * 3 + 2] : P.edge_geo[(k * NQF + q) * 3 + 2];
-                    len += jac * T[P.off.w_face + q];
+                    len += eg[C::STR ? 2 : q * 3 + 2] * T[P.off.w_face + q];
                 }
                 wave += smax * len;
             }
@@ -568,7 +637,7 @@ __global__ __launch_bounds__(DG_BS) void k_dg(const nz_problem P, const double* 
                 const double G[4] = {w.r * w.v, w.r * w.u * w.v, w.r * w.v * w.v + w.p, w.r * w.v * w.H};
                 double gx, gy, hx, hy, wd;
                 if (C::AFF) {
-                    const double* ge = P.geo + ep * 5;
+                    const double* ge = sGe + pe * 5;
                     gx = ge[0];
                     gy = ge[1];
                     hx = ge[2];
