@@ -45,7 +45,7 @@ def _pkg(name):
 
 
 def _traffic(workload: str):
-    """Measured HBM bytes per launch of the stage kernel, if profiled."""
+    """Measured HBM bytes per residual evaluation (counter profile), if recorded."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
