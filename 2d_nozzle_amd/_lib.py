@@ -44,6 +44,7 @@ class Problem(C.Structure):
         ("lebesgue", C.c_double),
         ("bisect_steps", C.c_int32), ("pad0", C.c_int32),
         ("wnorm", C.c_double * 4),
+        ("tiles", C.c_void_p), ("n_tiles", C.c_int64),
     ]
 
 
@@ -57,6 +58,7 @@ SIGNATURES = {
     "nz_stage": (C.c_int, [C.POINTER(Problem), P, P, P, P, C.c_double, C.c_double, C.c_double,
                            C.c_int, P, P, P, P]),
     "nz_norm_finish": (C.c_int, [C.POINTER(Problem), P, P, P]),
+    "nz_norm_segments": (C.c_int, [C.POINTER(Problem), P, P, P, C.c_int64, P, P]),
     "nz_norm": (C.c_int, [C.POINTER(Problem), P, P, P, P]),
     "nz_limit": (C.c_int, [C.POINTER(Problem), P, P, P, P]),
     "nz_time_step": (C.c_int, [C.POINTER(Problem), P, C.c_double, P, P]),

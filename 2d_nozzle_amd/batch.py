@@ -1,0 +1,385 @@
+"""Batched steady solves of many nozzle designs on one GPU (a parameter sweep).
+
+The reference solves a sweep one design at a time, serially with warm starts
+or in a process pool (``src/sweep.py:233-265,300-446``).  Here every design is
+marched at once:
+
+* Designs that share (element kind, order, geometry order) form a *family*.
+  A family's meshes are concatenated into one block-diagonal problem:
+  elements and nodes design after design, then the interior edges of every
+  design, then the boundary edges of every design (the device path tells
+  boundary from interior edges by one threshold).
+* One launch per Runge-Kutta stage advances all active designs of a family
+  through a tile table whose tiles never straddle two designs, and whose
+  tiles for one design are exactly the tiles that design would have alone.
+  The per-tile norm partials are therefore identical, and the per-design
+  norms (``nz_norm_segments``) are summed in the same order as a lone solve's,
+  so a design marched in the batch takes the same iterations and reaches the
+  same state, bit for bit, as when it is marched alone.
+* The march control (``src/timestepping/march.py:125-230``): one step for the
+  initial residual, then chunks of ``check_interval`` steps with the
+  non-finite, tolerance, divergence and stall checks per design.  A design
+  that stops is copied out at that step and its tiles leave the table, so it
+  costs nothing afterwards (per-design convergence masking).
+* Each family runs on its own HIP stream, so small families fill the device
+  together; the chunk of a family is captured once as a HIP graph and
+  replayed while its set of active designs is unchanged.
+
+Host work (meshes, operators, quasi-1D initial states) runs once per design.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass, field
+from types import SimpleNamespace
+
+import numpy as np
+
+from .config import FlowConditions, SolverOptions, residual_scale
+from .initial import quasi1d_initial
+from .mesh import NozzleGeometry, build_nozzle_mesh
+from .operators import build_operators
+
+TILE = 16  # elements per tile of the element kernel
+
+
+@dataclass(frozen=True)
+class Design:
+    """One point of a sweep (``src/sweep.py:233-265``)."""
+
+    area_ratio: float
+    kind: str = "tri"
+    order: int = 1
+    refine: int = 1
+    contour: str = "smooth"
+    nx: int = 14
+    nr: int = 5
+    geometry_order: int = 1
+
+    def geometry(self) -> NozzleGeometry:
+        return NozzleGeometry(contour=self.contour, area_ratio=self.area_ratio)
+
+
+def design_sweep(n: int = 256, seed: int = 1) -> list[Design]:
+    """The mixed sweep: area ratio U[1.5, 4], refine, order in {1, 2, 3},
+    triangles or quadrilaterals, smooth contour (drawn in that order from
+    ``default_rng(seed)``)."""
+    rng = np.random.default_rng(seed)
+    ar = rng.uniform(1.5, 4.0, n)
+    refine = rng.integers(1, 4, n)
+    order = rng.integers(1, 4, n)
+    kind = rng.choice(np.array(["tri", "quad"]), n)
+    return [Design(area_ratio=float(ar[i]), kind=str(kind[i]), order=int(order[i]),
+                   refine=int(refine[i])) for i in range(n)]
+
+
+def concat_operators(ops_list):
+    """One block-diagonal operator set from several of the same family.
+
+    Returns the attributes the device backend reads, plus the per-design
+    element, interior-edge and boundary-edge offsets.
+    """
+    o0 = ops_list[0]
+    rd = o0.ref
+    kind, q, nface = o0.topology.kind, o0.topology.geometry_order, o0.topology.n_faces
+    for o in ops_list:
+        if (o.topology.kind, o.topology.geometry_order, o.ref.order) != (kind, q, rd.order):
+            raise ValueError("concat_operators needs one element kind, order and geometry order")
+    ne = np.array([o.n_elem for o in ops_list])
+    nn = np.array([np.asarray(o.node_coords).shape[0] for o in ops_list])
+    ni = np.array([o.n_interior for o in ops_list])
+    nb = np.array([o.n_boundary for o in ops_list])
+    eoff = np.concatenate([[0], np.cumsum(ne)])
+    noff = np.concatenate([[0], np.cumsum(nn)])
+    ioff = np.concatenate([[0], np.cumsum(ni)])
+    boff = np.concatenate([[0], np.cumsum(nb)])
+    NI = int(ioff[-1])
+
+    def edges_of(o, d):
+        ed = o.topology.edges
+        fe = np.asarray(ed.face_edge)
+        nid = int(ni[d])
+        fe_g = np.where(fe < nid, ioff[d] + fe, NI + boff[d] + (fe - nid))
+        return fe_g
+
+    def split(a, d):
+        a = np.asarray(a)
+        return a[: ni[d]], a[ni[d]:]
+
+    face_edge = np.concatenate([edges_of(o, d) for d, o in enumerate(ops_list)])
+    face_side = np.concatenate([np.asarray(o.topology.edges.face_side) for o in ops_list])
+    face_nbr = np.concatenate([np.asarray(o.topology.edges.face_neighbour) + eoff[d]
+                               for d, o in enumerate(ops_list)])
+    iedge_face = np.concatenate([np.asarray(o.topology.edges.iedge_face) for o in ops_list])
+    iedge_elem = np.concatenate([np.asarray(o.topology.edges.iedge_elem) + eoff[d]
+                                 for d, o in enumerate(ops_list)])
+    bedge_elem = np.concatenate([np.asarray(o.topology.edges.bedge_elem) + eoff[d]
+                                 for d, o in enumerate(ops_list)])
+    bedge_face = np.concatenate([np.asarray(o.topology.edges.bedge_face) for o in ops_list])
+    bedge_tag = np.concatenate([np.asarray(o.topology.edges.bedge_tag) for o in ops_list])
+    nrm = [split(o.edge_normal, d) for d, o in enumerate(ops_list)]
+    jac = [split(o.edge_jac, d) for d, o in enumerate(ops_list)]
+    edge_normal = np.concatenate([a for a, _ in nrm] + [b for _, b in nrm])
+    edge_jac = np.concatenate([a for a, _ in jac] + [b for _, b in jac])
+    elem_nodes = np.concatenate([np.asarray(o.topology.elem_nodes) + noff[d]
+                                 for d, o in enumerate(ops_list)])
+    edges = SimpleNamespace(face_edge=face_edge, face_side=face_side, face_neighbour=face_nbr,
+                            iedge_face=iedge_face, iedge_elem=iedge_elem, bedge_elem=bedge_elem,
+                            bedge_face=bedge_face, bedge_tag=bedge_tag)
+    topo = SimpleNamespace(kind=kind, geometry_order=q, n_faces=nface, edges=edges,
+                           elem_nodes=elem_nodes)
+    minv = [getattr(o, "inv_mass", None) for o in ops_list]
+    return SimpleNamespace(
+        ref=rd, topology=topo, n_elem=int(eoff[-1]), n_interior=NI, n_boundary=int(boff[-1]),
+        n_edges_total=NI + int(boff[-1]),
+        node_coords=np.concatenate([np.asarray(o.node_coords) for o in ops_list]),
+        elem_area=np.concatenate([np.asarray(o.elem_area) for o in ops_list]),
+        mean_weights=np.concatenate([np.asarray(o.mean_weights) for o in ops_list]),
+        inv_mass=None if any(m is None for m in minv) else np.concatenate(minv),
+        edge_normal=edge_normal, edge_jac=edge_jac,
+        elem_offset=eoff, interior_offset=ioff, boundary_offset=boff,
+    )
+
+
+@dataclass
+class DesignResult:
+    design: Design
+    U: np.ndarray
+    converged: bool
+    iterations: int
+    residual: float
+    residual_initial: float
+    message: str = ""
+    operators: object = None
+    flow: FlowConditions | None = None
+
+
+class _Family:
+    """One (kind, order, geometry order) family marched by one backend."""
+
+    def __init__(self, idx, ops_list, U0_list, flow, opts, device):
+        import torch
+
+        from .backend import HipBackend
+
+        self.torch = torch
+        self.idx = list(idx)
+        self.cat = concat_operators(ops_list)
+        self.bk = HipBackend(self.cat, flow, opts, device=device)
+        if self.bk.superbee:
+            raise NotImplementedError("the batched march supports limiter 'positivity' or 'none'")
+        self.N = self.cat.ref.n_basis
+        self.eoff = self.cat.elem_offset
+        self.stream = torch.cuda.Stream(device=self.bk.device)
+        f64 = dict(dtype=torch.float64, device=self.bk.device)
+        self.cur, self.nxt = self.bk._S[0], self.bk._S[1]
+        self.cur.copy_(torch.from_numpy(np.concatenate(U0_list)).to(self.bk.device))
+        self.res_dev = torch.empty(len(self.idx), **f64)
+        self.active = list(range(len(self.idx)))  # local design indices
+        self.graph = None
+        self.graph_n = 0
+        self._set_tiles()
+
+    # -------------------------------------------------------------- tiles
+    def _set_tiles(self):
+        torch = self.torch
+        tiles, seg = [], [0]
+        for d in self.active:
+            a, b = int(self.eoff[d]), int(self.eoff[d + 1])
+            for e0 in range(a, b, TILE):
+                tiles.append((e0, min(e0 + TILE, b)))
+            seg.append(len(tiles))
+        dev = self.bk.device
+        self.tiles = torch.tensor(np.asarray(tiles, dtype=np.int64).reshape(-1), device=dev)
+        self.seg = torch.tensor(np.asarray(seg, dtype=np.int64), device=dev)
+        self.counts = torch.tensor([4.0 * (self.eoff[d + 1] - self.eoff[d]) * self.N
+                                    for d in self.active], dtype=torch.float64, device=dev)
+        P = self.bk.P
+        P.tiles = C.c_void_p(self.tiles.data_ptr()) if tiles else C.c_void_p(0)
+        P.n_tiles = len(tiles)
+        if self.bk._scratch.numel() < max(len(tiles), 1024):
+            self.bk._scratch = torch.empty(max(len(tiles), 1024), dtype=torch.float64,
+                                           device=dev)
+        self.graph = None  # the captured chunk used the old table
+
+    # ------------------------------------------------------------ stepping
+    def _launch(self, n_steps: int):
+        """``n_steps`` steps between the two ping-pong buffers, then the norms."""
+        bk, torch = self.bk, self.torch
+        scheme = bk.scheme
+        fac = bk.dt_factor(scheme)
+        st = bk._stream()
+        cur, nxt = self.cur, self.nxt
+        t1, t2 = bk._S[2], bk._S[3]
+        for _ in range(n_steps):
+            if scheme == "euler":
+                bk._stage(cur, None, nxt, True, 1.0, 1.0, fac, st)
+            else:
+                bk._stage(cur, None, t1, True, 1.0, 1.0, fac, st)
+                bk._stage(t1, cur, t2, False, 0.75, 0.25, fac, st)
+                bk._stage(t2, cur, nxt, False, 1.0 / 3.0, 2.0 / 3.0, fac, st)
+            cur, nxt = nxt, cur
+        bk._chk(bk.lib.nz_norm_segments(bk._pp, bk._p(bk._scratch), bk._p(self.seg),
+                                        bk._p(self.counts), len(self.active),
+                                        bk._p(self.res_dev), st), "nz_norm_segments")
+        return n_steps % 2 == 1  # buffers swapped
+
+    def advance(self, n_steps: int, use_graph: bool):
+        """Launch ``n_steps`` steps on the family's stream (no sync)."""
+        torch = self.torch
+        if not self.active:
+            return
+        # An even chunk leaves the ping-pong buffers in their roles, so its
+        # graph can be replayed until the set of active designs changes.
+        graph = use_graph and n_steps % 2 == 0
+        with torch.cuda.stream(self.stream):
+            if not graph:
+                swapped = self._launch(n_steps)
+            else:
+                if self.graph is None or self.graph_n != n_steps:
+                    g = torch.cuda.CUDAGraph()
+                    self.stream.synchronize()
+                    with torch.cuda.graph(g, stream=self.stream):
+                        self._launch(n_steps)
+                    self.graph, self.graph_n = g, n_steps
+                self.graph.replay()
+                swapped = False
+        if swapped:
+            self.cur, self.nxt = self.nxt, self.cur
+            self.graph = None  # the captured pointers no longer match the roles
+
+    def norms(self) -> np.ndarray:
+        return self.res_dev[: len(self.active)].cpu().numpy()
+
+    def take(self, d: int) -> np.ndarray:
+        a, b = int(self.eoff[d]), int(self.eoff[d + 1])
+        return self.cur[a:b].cpu().numpy()
+
+
+class BatchedSteady:
+    """Steady solves of a list of designs, marched together on one GPU."""
+
+    def __init__(self, designs, flow: FlowConditions | None = None,
+                 opts: SolverOptions | None = None, *, device=None, ops_list=None,
+                 U0_list=None):
+        self.designs = list(designs)
+        self.flow = flow or FlowConditions()
+        self.opts = opts or SolverOptions()
+        t0 = time.perf_counter()
+        if ops_list is None:
+            ops_list, U0_list = [], []
+            for dsg in self.designs:
+                geom = dsg.geometry()
+                topo, coords, _ = build_nozzle_mesh(geom, kind=dsg.kind, nx=dsg.nx, nr=dsg.nr,
+                                                    refine=dsg.refine,
+                                                    geometry_order=dsg.geometry_order)
+                ops = build_operators(topo, coords, dsg.order, dense=False)
+                ops_list.append(ops)
+                U0_list.append(quasi1d_initial(ops, self.flow, geom))
+        self.ops_list, self.U0_list = ops_list, U0_list
+        self.host_setup_s = time.perf_counter() - t0
+        fams: dict = {}
+        for i, o in enumerate(ops_list):
+            key = (o.topology.kind, o.topology.geometry_order, o.ref.order)
+            fams.setdefault(key, []).append(i)
+        t0 = time.perf_counter()
+        self.families = [_Family(idx, [ops_list[i] for i in idx], [U0_list[i] for i in idx],
+                                 self.flow, self.opts, device) for idx in fams.values()]
+        self.device_setup_s = time.perf_counter() - t0
+
+    @property
+    def n_elements(self) -> int:
+        return int(sum(o.n_elem for o in self.ops_list))
+
+    def solve(self, *, use_graph: bool = True, progress=None) -> list[DesignResult]:
+        """March every design to its own stop; results in input order."""
+        import torch
+
+        opts = self.opts
+        scale = residual_scale(self.flow, 1.0)
+        tol = opts.tolerance * scale
+        chunk = min(max(1, opts.check_interval), opts.max_iterations)
+        results: list[DesignResult | None] = [None] * len(self.designs)
+        state = {}  # global index -> dict(res0, window)
+
+        def finish(fam, d, conv, done, res, msg):
+            g = fam.idx[d]
+            results[g] = DesignResult(
+                design=self.designs[g], U=fam.take(d), converged=conv, iterations=done,
+                residual=float(res), residual_initial=float(state[g]["res0"]), message=msg,
+                operators=self.ops_list[g], flow=self.flow)
+
+        def check(first: bool, done: int):
+            for fam in self.families:
+                if not fam.active:
+                    continue
+                res = fam.norms()
+                keep = []
+                for j, d in enumerate(fam.active):
+                    g, r = fam.idx[d], float(res[j])
+                    if first:
+                        state[g] = {"res0": r, "window": []}
+                        if not np.isfinite(r):
+                            finish(fam, d, False, done, r, "initial residual is not finite")
+                        elif r <= tol:
+                            finish(fam, d, True, done, r, "")
+                        else:
+                            keep.append(d)
+                        continue
+                    st = state[g]
+                    if not np.isfinite(r):
+                        finish(fam, d, False, done, r,
+                               f"residual became non-finite after {done} iterations; reduce cfl "
+                               "(try 0.5) or enable limiter='superbee'")
+                    elif r <= tol:
+                        finish(fam, d, True, done, r, "")
+                    elif r > opts.divergence_factor * st["res0"]:
+                        finish(fam, d, False, done, r,
+                               f"residual grew by {r / st['res0']:.1e}x after {done} iterations, "
+                               f"which is past divergence_factor={opts.divergence_factor:g}; "
+                               "reduce cfl or start from initial_condition='quasi1d'")
+                    else:
+                        w = st["window"]
+                        w.append(r)
+                        if len(w) > opts.stall_window:
+                            w.pop(0)
+                        if (len(w) == opts.stall_window and max(w) > 0.0
+                                and min(w) / max(w) > opts.stall_ratio):
+                            finish(fam, d, False, done, r,
+                                   f"the residual has stalled at {r / scale:.3e} (scaled; target "
+                                   f"{opts.tolerance:g})")
+                        elif done >= opts.max_iterations:
+                            finish(fam, d, False, done, r,
+                                   f"hit max_iterations={opts.max_iterations} at scaled residual "
+                                   f"{r / scale:.2e} (target {opts.tolerance:g}); raise "
+                                   "max_iterations or relax tolerance")
+                        else:
+                            keep.append(d)
+                if keep != fam.active:
+                    fam.active = keep
+                    fam._set_tiles()
+
+        t0 = time.perf_counter()
+        for fam in self.families:
+            fam.advance(1, use_graph=False)
+        torch.cuda.synchronize()
+        done = 1
+        check(True, done)
+        while any(f.active for f in self.families) and done < opts.max_iterations:
+            n = min(chunk, opts.max_iterations - done)
+            for fam in self.families:
+                fam.advance(n, use_graph)
+            torch.cuda.synchronize()
+            done += n
+            check(False, done)
+            if progress is not None:
+                progress(done, sum(len(f.active) for f in self.families))
+        self.wall_s = time.perf_counter() - t0
+        for fam in self.families:  # anything left at max_iterations
+            for d in list(fam.active):
+                g = fam.idx[d]
+                if results[g] is None:
+                    finish(fam, d, False, done, float("nan"), "stopped")
+        return results

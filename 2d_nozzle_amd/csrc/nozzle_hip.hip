@@ -398,8 +398,15 @@ __global__ __launch_bounds__(DG_BS) void k_dg(const nz_problem P, const double* 
 
     const int t = threadIdx.x;
     const int el = t >> 2, s = t & 3;  // component lanes
-    const int64_t e0 = static_cast<int64_t>(blockIdx.x) * TE;
-    const int ne = static_cast<int>(P.n_elem - e0 < TE ? P.n_elem - e0 : TE);
+    int64_t e0;
+    int ne;
+    if (P.tiles) {  // tile table: several disjoint meshes, or a subset of tiles
+        e0 = P.tiles[2 * blockIdx.x];
+        ne = static_cast<int>(P.tiles[2 * blockIdx.x + 1] - e0);
+    } else {
+        e0 = static_cast<int64_t>(blockIdx.x) * TE;
+        ne = static_cast<int>(P.n_elem - e0 < TE ? P.n_elem - e0 : TE);
+    }
     const bool live = el < ne;
     const int64_t e = e0 + el;
     const bool fold = C::AFF && (STAGE || mode != kRes);  // affine rate: M^-1 folded into tables
@@ -1342,9 +1349,8 @@ __global__ __launch_bounds__(nz::BS) void k_norm_partial(const double* __restric
 // accumulators per thread keep loads in flight), so the norm is reproducible.
 constexpr int FIN_BS = 1024;
 
-__global__ __launch_bounds__(FIN_BS) void k_norm_finish(const double* __restrict__ partial,
-                                                        int64_t nparts, double count,
-                                                        double* __restrict__ out) {
+__device__ void norm_of_partials(const double* __restrict__ partial, int64_t nparts, double count,
+                                 double* __restrict__ out) {
     __shared__ double red[FIN_BS / 64];
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     int64_t it = threadIdx.x;
@@ -1365,6 +1371,21 @@ __global__ __launch_bounds__(FIN_BS) void k_norm_finish(const double* __restrict
         for (int w = 0; w < FIN_BS / 64; ++w) tot += red[w];
         out[0] = sqrt(tot / count);
     }
+}
+
+__global__ __launch_bounds__(FIN_BS) void k_norm_finish(const double* __restrict__ partial,
+                                                        int64_t nparts, double count,
+                                                        double* __restrict__ out) {
+    norm_of_partials(partial, nparts, count, out);
+}
+
+// one workgroup per segment of the partials
+__global__ __launch_bounds__(FIN_BS) void k_norm_segments(const double* __restrict__ partial,
+                                                          const int64_t* __restrict__ seg,
+                                                          const double* __restrict__ counts,
+                                                          double* __restrict__ out) {
+    const int64_t d = blockIdx.x;
+    norm_of_partials(partial + seg[d], seg[d + 1] - seg[d], counts[d], out + d);
 }
 
 __global__ __launch_bounds__(nz::BS) void k_time_step(const double* __restrict__ wave,
@@ -1411,12 +1432,13 @@ struct Launch {
     static void dg(const nz_problem& P, const double* Uin, const double* Un, double* Uout,
                    double* dt, double fac, double a, double b, int mode, double* partial,
                    uint8_t* flags, unsigned long long* counters, int64_t nblk, hipStream_t st) {
-        (void)nblk;  // k_dg tiles are nz::TE elements
-        const int64_t nblk_dg = (P.n_elem + nz::TE - 1) / nz::TE;  // one wave each
+        (void)nblk;  // k_dg tiles are nz::TE elements, or come from the tile table
+        const int64_t nblk_dg = P.tiles ? P.n_tiles : (P.n_elem + nz::TE - 1) / nz::TE;
         const int64_t nbq = (P.n_edge - P.n_int) * C::NQF;
         if (nbq > 0)
             hipLaunchKernelGGL((nz::k_bflux<C>), dim3(static_cast<unsigned>((nbq + nz::BS - 1) / nz::BS)),
                                dim3(nz::BS), 0, st, P, P.tables, Uin);
+        if (nblk_dg <= 0) return;
         const dim3 g(static_cast<unsigned>(nblk_dg));
         const bool stage = mode == nz::kStageFirst || mode == nz::kStageNext;
 #define NZ_DG(FL, ST)                                                                              \
@@ -1515,11 +1537,15 @@ int resolve(const nz_problem* p, Ops* ops) {
     if (p->n_int < 0 || p->n_int > p->n_edge) return fail(NZ_E_ARG, "n_int out of range");
     if (p->n_edge > p->n_int && (!p->bedge || !p->bflux))
         return fail(NZ_E_ARG, "boundary edges need bedge and the bflux workspace");
+    if (p->tiles && (p->n_tiles < 0 || p->n_tiles > (int64_t{1} << 31)))
+        return fail(NZ_E_ARG, "n_tiles out of range");
     return NZ_OK;
 }
 
 int64_t n_blocks(const nz_problem* p, const Ops& o) { return (p->n_elem + o.te - 1) / o.te; }
-int64_t n_blocks_dg(const nz_problem* p) { return (p->n_elem + nz::TE - 1) / nz::TE; }
+int64_t n_blocks_dg(const nz_problem* p) {
+    return p->tiles ? p->n_tiles : (p->n_elem + nz::TE - 1) / nz::TE;
+}
 
 }  // namespace
 
@@ -1589,6 +1615,19 @@ int nz_norm_finish(const nz_problem* p, const double* scratch, double* out, void
     hipLaunchKernelGGL(k_norm_finish, dim3(1), dim3(FIN_BS), 0, static_cast<hipStream_t>(stream),
                        scratch, n_blocks_dg(p), count, out);
     return check_launch("nz_norm_finish");
+}
+
+int nz_norm_segments(const nz_problem* p, const double* scratch, const int64_t* seg,
+                     const double* counts, int64_t n_seg, double* out, void* stream) {
+    Ops o;
+    int rc = resolve(p, &o);
+    if (rc) return rc;
+    if (n_seg < 0) return fail(NZ_E_ARG, "n_seg must be non-negative");
+    if (n_seg == 0) return NZ_OK;
+    if (!scratch || !seg || !counts || !out) return fail(NZ_E_ARG, "null array");
+    hipLaunchKernelGGL(k_norm_segments, dim3(static_cast<unsigned>(n_seg)), dim3(FIN_BS), 0,
+                       static_cast<hipStream_t>(stream), scratch, seg, counts, out);
+    return check_launch("nz_norm_segments");
 }
 
 int nz_norm(const nz_problem* p, const double* A, double* scratch, double* out, void* stream) {

@@ -96,6 +96,65 @@ def _timed(fn, steps: int, warmup: int, stream) -> float:
     return ev0.elapsed_time(ev1) / 1e3
 
 
+def sweep_bench(n: int, tol: float, n_lone: int) -> dict:
+    """Designs/s of the batched march against one-at-a-time solves.
+
+    The mixed sweep (area ratio U[1.5, 4], refine and order in {1, 2, 3},
+    triangles and quadrilaterals, smooth contour, Roe, positivity limiter) is
+    marched to ``tol`` in one batch; ``n_lone`` evenly spaced designs of it
+    are also solved one at a time through the ordinary backend and give the
+    one-at-a-time rate.  Host setup (meshes, operators, initial states) is
+    reported separately and excluded from both rates.
+    """
+    import torch
+
+    batch, cf, mr = _pkg("batch"), _pkg("config"), _pkg("march")
+    designs = batch.design_sweep(n, 1)
+    flow = cf.FlowConditions()
+    opts = cf.SolverOptions(tolerance=tol)
+    run = batch.BatchedSteady(designs, flow, opts)
+    print(f"sweep: {n} designs, {run.n_elements} elements, {len(run.families)} families, "
+          f"host setup {run.host_setup_s:.1f} s", file=sys.stderr, flush=True)
+    last = [time.perf_counter()]
+
+    def progress(done, active):
+        if time.perf_counter() - last[0] > 20.0:
+            last[0] = time.perf_counter()
+            print(f"  sweep step {done}: {active} designs still marching", file=sys.stderr,
+                  flush=True)
+
+    res = run.solve(use_graph=True, progress=progress)
+    t_batch = run.wall_s
+    iters = np.array([r.iterations for r in res])
+    conv = int(sum(r.converged for r in res))
+    print(f"sweep: batch {t_batch:.1f} s, {conv}/{n} converged, iterations "
+          f"{iters.min()}..{iters.max()}", file=sys.stderr, flush=True)
+    pick = np.linspace(0, n - 1, min(n_lone, n)).round().astype(int)
+    t_lone, lone_match = 0.0, 0
+    for g in pick:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        lone = mr.solve_steady(run.ops_list[g], flow, opts, U0=run.U0_list[g])
+        torch.cuda.synchronize()
+        t_lone += time.perf_counter() - t0
+        lone_match += int(lone.iterations == res[g].iterations
+                          and np.array_equal(lone.U, res[g].U))
+    lone_rate = len(pick) / t_lone
+    print(f"sweep: one at a time {t_lone:.1f} s for {len(pick)} designs", file=sys.stderr,
+          flush=True)
+    return {
+        "designs": n, "elements": run.n_elements, "families": len(run.families),
+        "tolerance": tol, "converged": conv,
+        "iterations_min": int(iters.min()), "iterations_max": int(iters.max()),
+        "batch_s": round(t_batch, 3), "designs_per_s": round(n / t_batch, 3),
+        "lone_sample": int(len(pick)), "lone_s": round(t_lone, 3),
+        "lone_designs_per_s": round(lone_rate, 3),
+        "speedup": round((n / t_batch) / lone_rate, 2),
+        "lone_bit_identical": f"{lone_match}/{len(pick)}",
+        "host_setup_s": round(run.host_setup_s, 1),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +168,11 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-march", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--sweep", type=int, default=0,
+                    help="also march this many designs of the mixed sweep (seed 1) in one batch")
+    ap.add_argument("--sweep-tol", type=float, default=1e-10)
+    ap.add_argument("--sweep-lone", type=int, default=16,
+                    help="designs of the sweep also solved one at a time, for the comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on exactly one GPU; multi-GPU runs "
@@ -209,6 +273,8 @@ def main() -> int:
             "elements_scaled_per_stage": round(cnt[0] / (3 * (msteps + 1)), 1),
             "residual_norm_last": float(bk._res.item()),
         }
+    if args.sweep > 0:
+        out["sweep"] = sweep_bench(args.sweep, args.sweep_tol, args.sweep_lone)
     if not args.no_cpu:
         sample_cells = (64, 32) if args.order >= 2 else (256, 128)
         out["cpu_baseline"] = cpu_baseline(args.order, sample_cells[0], sample_cells[1],

@@ -78,6 +78,13 @@ typedef struct {
     int32_t bisect_steps;
     int32_t pad0;
     double wnorm[4];        /* residual-norm component weights (src/backends/base.py:42-78) */
+    /* optional tile table for the element kernel of nz_residual / nz_rate /
+     * nz_stage: tile k covers elements [tiles[2k], tiles[2k+1]), at most 16
+     * of them, and its norm partial is scratch[k].  NULL: consecutive tiles of
+     * 16 over the whole mesh.  Used to march several disjoint meshes
+     * (designs) in one launch and to leave converged ones out. */
+    const int64_t *tiles;
+    int64_t n_tiles;
 } nz_problem;
 
 /* Number of scratch doubles nz_* calls need for norm partials, and the
@@ -116,6 +123,14 @@ int nz_stage(const nz_problem *p, const double *Uin, const double *Un, double *U
 /* Finish the residual norm from the partials nz_stage(first=1) left in
  * scratch: out[0] = sqrt(sum / (E*N*4)) (src/backends/_numba_kernels.py:789-814). */
 int nz_norm_finish(const nz_problem *p, const double *scratch, double *out, void *stream);
+
+/* Per-segment residual norms from the partials nz_stage(first=1) left in
+ * `scratch` under a tile table: out[d] = sqrt(sum of scratch[seg[d] ..
+ * seg[d+1]) / counts[d]), summed in the same order as nz_norm_finish sums a
+ * whole mesh, so a design marched in a batch sees the norm it would see
+ * alone.  (Batched form of src/backends/_numba_kernels.py:789-814.) */
+int nz_norm_segments(const nz_problem *p, const double *scratch, const int64_t *seg,
+                     const double *counts, int64_t n_seg, double *out, void *stream);
 
 /* Weighted RMS norm of an arbitrary (E,N,4) array (Backend.norm,
  * src/backends/numpy_backend.py:55-60).  Uses scratch; result in out[0]. */
