@@ -266,30 +266,46 @@ __device__ __forceinline__ double exact_dot(const double* col, B b) {
     return v;
 }
 
-// min density and pressure over the probe set at scaling theta; every lane of
-// the element returns the same values
+// The probe set: the volume points and the element's own face points
+// (src/backends/_numba_kernels.py:858-924).  probe_value is this lane's
+// component at probe k, summed without contraction.
 template <class C>
-__device__ void probe_minmax(const double* col, int s, const double ub[4], double theta,
-                             const double* __restrict__ T, const nz_problem& P, int sidebits,
-                             double* rmin, double* pmin) {
+__device__ __forceinline__ double probe_value(const double* col, int k,
+                                              const double* __restrict__ T, const nz_problem& P,
+                                              int sidebits) {
 #pragma clang fp contract(off)
     constexpr int N = C::N, NQV = C::NQV, NQF = C::NQF, NF = C::NF;
+    if (k < NQV) return exact_dot<N>(col, [&](int i) { return T[P.off.phi_vol + i * NQV + k]; });
+    const int f = (k - NQV) / NQF, q = (k - NQV) - f * NQF;
+    const int sd = (sidebits >> f) & 1;
+    return exact_dot<N>(col, [&](int i) {
+        const double b0 = T[P.off.phi_face + ((0 * NF + f) * N + i) * NQF + q];
+        const double b1 = T[P.off.phi_face + ((1 * NF + f) * N + i) * NQF + q];
+        return sd ? b1 : b0;
+    });
+}
+
+template <class C>
+struct Probes {
+    static constexpr int NP = C::NQV + C::NF * C::NQF;
+    // the probe sums do not change while theta is searched: keep them in
+    // registers when they fit beside the element's column
+    static constexpr bool CACHE = NP <= 32 && C::N <= 10;
+};
+
+// min density and pressure of ub + theta (V - ub) over the probes; every lane
+// of the element returns the same values.  pv: cached probe sums, or null to
+// recompute them from col.
+template <class C>
+__device__ __forceinline__ void probe_minmax(const double* pv, const double* col, const double ub[4],
+                                             double theta, const double* __restrict__ T,
+                                             const nz_problem& P, int sidebits, double* rmin,
+                                             double* pmin) {
+#pragma clang fp contract(off)
+    constexpr int NP = Probes<C>::NP;
     double rm = 1e300, pm = 1e300;
     const double g = P.gamma;
-#pragma unroll 1
-    for (int k = 0; k < NQV + NF * NQF; ++k) {
-        double v;
-        if (k < NQV) {
-            v = exact_dot<N>(col, [&](int i) { return T[P.off.phi_vol + i * NQV + k]; });
-        } else {
-            const int f = (k - NQV) / NQF, q = (k - NQV) - f * NQF;
-            const int sd = (sidebits >> f) & 1;
-            v = exact_dot<N>(col, [&](int i) {
-                const double b0 = T[P.off.phi_face + ((0 * NF + f) * N + i) * NQF + q];
-                const double b1 = T[P.off.phi_face + ((1 * NF + f) * N + i) * NQF + q];
-                return sd ? b1 : b0;
-            });
-        }
+    auto one = [&](double v) {
         double V[4];
         for (int c = 0; c < 4; ++c) V[c] = quad_get(v, c);
         for (int c = 0; c < 4; ++c) V[c] = ub[c] + theta * (V[c] - ub[c]);
@@ -297,8 +313,16 @@ __device__ void probe_minmax(const double* col, int s, const double ub[4], doubl
         const double r = V[0] > 1e-300 ? V[0] : 1e-300;
         const double pp = (g - 1.0) * (V[3] - 0.5 * (V[1] * V[1] + V[2] * V[2]) / r);
         pm = pp < pm ? pp : pm;
+    };
+    if constexpr (Probes<C>::CACHE) {
+        (void)col;
+#pragma unroll
+        for (int k = 0; k < NP; ++k) one(pv[k]);
+    } else {
+        (void)pv;
+#pragma unroll 1
+        for (int k = 0; k < NP; ++k) one(probe_value<C>(col, k, T, P, sidebits));
     }
-    (void)s;
     *rmin = rm;
     *pmin = pm;
 }
@@ -350,8 +374,13 @@ __device__ uint8_t limit_column(double* col, int s, const double* mw, const doub
             if (p_lo >= eps_p) return fl;
         }
     }
+    double pv[Probes<C>::CACHE ? Probes<C>::NP : 1];
+    if constexpr (Probes<C>::CACHE) {
+#pragma unroll
+        for (int k = 0; k < Probes<C>::NP; ++k) pv[k] = probe_value<C>(col, k, T, P, sidebits);
+    }
     double rmin, pmin;
-    probe_minmax<C>(col, s, ub, 1.0, T, P, sidebits, &rmin, &pmin);
+    probe_minmax<C>(pv, col, ub, 1.0, T, P, sidebits, &rmin, &pmin);
     if (rmin >= eps_rho && pmin >= eps_p) return fl;
     fl |= 2;
     double theta = 1.0;
@@ -360,14 +389,14 @@ __device__ uint8_t limit_column(double* col, int s, const double* mw, const doub
         theta = gap <= 0.0 ? 0.0 : fmin(fmax((ub[0] - eps_rho) / gap, 0.0), 1.0);
     }
     double rr, phi;
-    probe_minmax<C>(col, s, ub, theta, T, P, sidebits, &rr, &phi);
+    probe_minmax<C>(pv, col, ub, theta, T, P, sidebits, &rr, &phi);
     if (phi < eps_p) {
         double lo = 0.0, hi = theta;
 #pragma unroll 1
         for (int it = 0; it < P.bisect_steps; ++it) {
             const double mid = 0.5 * (lo + hi);
             double pm;
-            probe_minmax<C>(col, s, ub, mid, T, P, sidebits, &rr, &pm);
+            probe_minmax<C>(pv, col, ub, mid, T, P, sidebits, &rr, &pm);
             if (pm >= eps_p) lo = mid;
             else hi = mid;
         }
