@@ -96,7 +96,17 @@ def _timed(fn, steps: int, warmup: int, stream) -> float:
     return ev0.elapsed_time(ev1) / 1e3
 
 
-def sweep_bench(n: int, tol: float, n_lone: int) -> dict:
+def _stops(res) -> dict:
+    """Converged / total per (kind, order) family of the sweep."""
+    fam: dict = {}
+    for r in res:
+        k = f"{r.design.kind}-p{r.design.order}"
+        c, t = fam.get(k, (0, 0))
+        fam[k] = (c + int(r.converged), t + 1)
+    return {k: f"{c}/{t}" for k, (c, t) in sorted(fam.items())}
+
+
+def sweep_bench(n: int, tol: float, n_lone: int, max_it: int = 50000) -> dict:
     """Designs/s of the batched march against one-at-a-time solves.
 
     The mixed sweep (area ratio U[1.5, 4], refine and order in {1, 2, 3},
@@ -111,7 +121,7 @@ def sweep_bench(n: int, tol: float, n_lone: int) -> dict:
     batch, cf, mr = _pkg("batch"), _pkg("config"), _pkg("march")
     designs = batch.design_sweep(n, 1)
     flow = cf.FlowConditions()
-    opts = cf.SolverOptions(tolerance=tol)
+    opts = cf.SolverOptions(tolerance=tol, max_iterations=max_it)
     run = batch.BatchedSteady(designs, flow, opts)
     print(f"sweep: {n} designs, {run.n_elements} elements, {len(run.families)} families, "
           f"host setup {run.host_setup_s:.1f} s", file=sys.stderr, flush=True)
@@ -138,13 +148,14 @@ def sweep_bench(n: int, tol: float, n_lone: int) -> dict:
         torch.cuda.synchronize()
         t_lone += time.perf_counter() - t0
         lone_match += int(lone.iterations == res[g].iterations
-                          and np.array_equal(lone.U, res[g].U))
+                          and np.array_equal(lone.U, res[g].U, equal_nan=True))
     lone_rate = len(pick) / t_lone
     print(f"sweep: one at a time {t_lone:.1f} s for {len(pick)} designs", file=sys.stderr,
           flush=True)
     return {
         "designs": n, "elements": run.n_elements, "families": len(run.families),
-        "tolerance": tol, "converged": conv,
+        "tolerance": tol, "max_iterations": max_it, "converged": conv,
+        "stopped_by_family": _stops(res),
         "iterations_min": int(iters.min()), "iterations_max": int(iters.max()),
         "batch_s": round(t_batch, 3), "designs_per_s": round(n / t_batch, 3),
         "lone_sample": int(len(pick)), "lone_s": round(t_lone, 3),
@@ -171,6 +182,7 @@ def main() -> int:
     ap.add_argument("--sweep", type=int, default=0,
                     help="also march this many designs of the mixed sweep (seed 1) in one batch")
     ap.add_argument("--sweep-tol", type=float, default=1e-10)
+    ap.add_argument("--sweep-maxit", type=int, default=50000)
     ap.add_argument("--sweep-lone", type=int, default=16,
                     help="designs of the sweep also solved one at a time, for the comparison")
     args = ap.parse_args()
@@ -274,7 +286,7 @@ def main() -> int:
             "residual_norm_last": float(bk._res.item()),
         }
     if args.sweep > 0:
-        out["sweep"] = sweep_bench(args.sweep, args.sweep_tol, args.sweep_lone)
+        out["sweep"] = sweep_bench(args.sweep, args.sweep_tol, args.sweep_lone, args.sweep_maxit)
     if not args.no_cpu:
         sample_cells = (64, 32) if args.order >= 2 else (256, 128)
         out["cpu_baseline"] = cpu_baseline(args.order, sample_cells[0], sample_cells[1],
