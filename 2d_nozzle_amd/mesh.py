@@ -267,7 +267,8 @@ def nozzle_x_distribution(n_nodes_x, *, nx_base, n_sub, spacing, x_throat,
 
 
 # --------------------------------------------------------------------------- contours
-CONTOURS = ("bell", "smooth", "conical", "moc", "bezier", "analytic")
+CONTOURS = ("bell", "smooth", "conical", "moc", "bezier", "bezier9", "analytic")
+BEZIER9_DEGREE = 9
 ANALYTIC_THROAT_X = 0.13880048
 ANALYTIC_THROAT_Y = 0.13989434
 ANALYTIC_INLET_Y = 0.15
@@ -278,6 +279,29 @@ def analytic_contour(x):
     t = math.pi * np.log1p((math.exp(2.0) - 1.0) * x)
     half = np.cos(0.5 * t)
     return 0.01 * (4.0 - half) * (np.cos(t) + 5.0 - half)
+
+
+def bezier9_default_heights(y_th: float, y_ex: float) -> np.ndarray:
+    """Interior control heights of the degree-9 Bernstein diverging wall that
+    reproduces the ``smooth`` cubic ``y_th + (y_ex - y_th)(3 s^2 - 2 s^3)``
+    exactly: the cubic's Bernstein coefficients ``(y_th, y_th, y_ex, y_ex)``
+    raised to degree 9."""
+    b = np.array([y_th, y_th, y_ex, y_ex])
+    n, m = 3, BEZIER9_DEGREE
+    c = np.zeros(m + 1)
+    for k in range(m + 1):
+        for j in range(max(0, k - (m - n)), min(n, k) + 1):
+            c[k] += math.comb(n, j) * math.comb(m - n, k - j) / math.comb(m, k) * b[j]
+    return c[1:m]
+
+
+def _bernstein9(s, ctrl):
+    """``sum_k B_k^9(s) ctrl[k]`` for ten control values."""
+    m = BEZIER9_DEGREE
+    out = np.zeros_like(s)
+    for k in range(m + 1):
+        out = out + math.comb(m, k) * s**k * (1.0 - s) ** (m - k) * ctrl[k]
+    return out
 
 
 def _hermite(s, y0, y1, m0, m1):
@@ -301,6 +325,9 @@ class NozzleGeometry:
     theta_exit_deg: float = 0.0
     bezier_w1: float = 0.55
     bezier_w2: float = 0.90
+    # bezier9: the eight interior control heights of the diverging wall
+    # (None: the exact degree elevation of the smooth contour)
+    bezier9_heights: tuple | None = None
 
     def __post_init__(self) -> None:
         if self.contour not in CONTOURS:
@@ -335,7 +362,17 @@ class NozzleGeometry:
             "bezier_w1": float(self.bezier_w1),
             "bezier_w2": float(self.bezier_w2),
             "length": float(self.length),
-        }
+        } | self._bezier9_params()
+
+    def _bezier9_params(self) -> dict[str, float]:
+        if self.contour != "bezier9":
+            return {}
+        y_th = float(self.throat_half_height)
+        h = (bezier9_default_heights(y_th, y_th * float(self.area_ratio))
+             if self.bezier9_heights is None else np.asarray(self.bezier9_heights, dtype=float))
+        if h.shape != (BEZIER9_DEGREE - 1,):
+            raise ValueError(f"bezier9 needs {BEZIER9_DEGREE - 1} interior control heights")
+        return {f"bz{i + 1}": float(v) for i, v in enumerate(h)}
 
     def wall(self, x):
         return wall_half_height(x, self.params(), self.contour)
@@ -373,6 +410,9 @@ def wall_half_height(x, params: Mapping[str, float], contour: str = "bell"):
     elif contour in ("bell", "smooth", "moc"):
         y_div = _hermite(s, y_th, y_ex, math.tan(params["theta_initial"]) * dx_ds,
                          math.tan(params["theta_exit"]) * dx_ds)
+    elif contour == "bezier9":
+        ctrl = [y_th] + [params[f"bz{i}"] for i in range(1, BEZIER9_DEGREE)] + [y_ex]
+        y_div = _bernstein9(s, ctrl)
     else:  # bezier
         c1 = y_th + params["bezier_w1"] * (y_ex - y_th)
         c2 = y_th + params["bezier_w2"] * (y_ex - y_th)

@@ -179,9 +179,12 @@ class DifferentiableCase:
     geometry_order: int = 1
     options: SolverOptions = field(default_factory=lambda: SolverOptions(tolerance=1e-11,
                                                                          print_interval=0))
+    nx: int = 14
+    nr: int = 5
 
     def __post_init__(self) -> None:
-        topo, coords, grids = build_nozzle_mesh(self.geometry, kind=self.kind, refine=self.refine,
+        topo, coords, grids = build_nozzle_mesh(self.geometry, kind=self.kind, nx=self.nx,
+                                                nr=self.nr, refine=self.refine,
                                                 geometry_order=self.geometry_order)
         self.topology = topo
         self.x_frac, self.r_frac = grids["x_frac"], grids["r_frac"]

@@ -73,3 +73,17 @@ def residual_bytes(ops) -> dict:
     b_res = 64 * N * E + 8 * ge * E + 8 * gf * F + 4 * nface * E + 12 * F
     st = [b_res + 8 * E, b_res + 32 * N * E + 8 * E, b_res + 32 * N * E + 8 * E]
     return {"residual": b_res, "stage": st, "step": sum(st)}
+
+
+def bezier9_geometry(seed: int = 0, spread: float = 0.05):
+    """The degree-9 Bezier wall with its eight interior control heights
+    perturbed as ``c_i (1 + spread u_i)``, ``u_i ~ U[-1, 1]`` from
+    ``default_rng(seed)``; the unperturbed heights are the exact degree
+    elevation of the smooth contour."""
+    from .mesh import NozzleGeometry, bezier9_default_heights
+
+    base = NozzleGeometry(contour="bezier9")
+    c = bezier9_default_heights(base.throat_half_height,
+                                base.throat_half_height * base.area_ratio)
+    u = np.random.default_rng(seed).uniform(-1.0, 1.0, c.size)
+    return NozzleGeometry(contour="bezier9", bezier9_heights=tuple(c * (1.0 + spread * u)))

@@ -96,6 +96,92 @@ def _timed(fn, steps: int, warmup: int, stream) -> float:
     return ev0.elapsed_time(ev1) / 1e3
 
 
+def shocked_bench(refine: int, steps: int) -> dict:
+    """The shocked full march at scale: quadrilaterals, Q2, P2, bell contour,
+    area ratio 2.5, pb 0.7 with the normal shock in the diverging section,
+    HLLC, positivity limiter on every stage, SSP-RK3, from the quasi-1D state.
+    Refine 7 of the 14 x 5 base grid is 1,146,880 elements."""
+    import torch
+
+    mesh, op, cf, ini, be, wl = (_pkg("mesh"), _pkg("operators"), _pkg("config"),
+                                 _pkg("initial"), _pkg("backend"), _pkg("workloads"))
+    t0 = time.perf_counter()
+    geom = mesh.NozzleGeometry(contour="bell", area_ratio=2.5)
+    topo, coords, _ = mesh.build_nozzle_mesh(geom, kind="quad", refine=refine, geometry_order=2)
+    ops = op.build_operators(topo, coords, 2, dense=False)
+    flow = cf.FlowConditions(back_pressure_ratio=0.7, allow_shock_in_nozzle=True, flux="hllc")
+    U0 = ini.quasi1d_initial(ops, flow, geom)
+    bk = be.HipBackend(ops, flow, cf.SolverOptions(limiter="positivity"))
+    E, N = ops.n_elem, ops.ref.n_basis
+    print(f"shocked: {E} quads Q2 P2, host setup {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    cur, nxt = bk._S[0], bk._S[1]
+    cur.copy_(torch.from_numpy(U0).to(bk.device))
+    bk._counters.zero_()
+    state = {"cur": cur, "nxt": nxt}
+
+    def step():
+        c = bk.advance(state["cur"], state["nxt"], 1, "ssprk3")
+        state["nxt"] = state["cur"] if c is not state["cur"] else state["nxt"]
+        state["cur"] = c
+
+    st = torch.cuda.current_stream()
+    t = _timed(step, steps, 1, st)
+    cnt = bk._counters.cpu().tolist()
+    nb = wl.residual_bytes(ops)
+    return {
+        "workload": f"nozzle-bell-quad-q2-p2-refine{refine}-pb0.7-hllc", "elements": E,
+        "n_basis": N, "steps": steps,
+        "gdof_stage_s": round(3 * steps * E * N / t / 1e9, 4),
+        "ms_per_step": round(t / steps * 1e3, 4),
+        "stage_hbm_frac": round(nb["step"] / (t / steps) / 1e9 / HBM_PEAK_GBS, 4),
+        "elements_scaled_per_stage": round(cnt[0] / (3 * (steps + 1)), 1),
+        "residual_norm_last": float(bk._res.item()),
+    }
+
+
+def adjoint_bench(refine: int) -> dict:
+    """Design gradients of thrust w.r.t. the eight interior control heights of
+    the degree-9 Bezier wall (perturbed 5 %, seed 0): tri P2 on the 14 x 7
+    base grid (refine 5: 200,704 elements), forward solve to 1e-12, steady
+    adjoint by the residual-VJP kernels, one central difference (warm-started,
+    h = 1e-6 relative) as a check."""
+    import torch
+
+    sens, wl, cf, ini = _pkg("sensitivity"), _pkg("workloads"), _pkg("config"), _pkg("initial")
+    geom = wl.bezier9_geometry(0)
+    case = sens.DifferentiableCase(geometry=geom, order=2, refine=refine, kind="tri", nx=14,
+                                   nr=7, options=cf.SolverOptions(tolerance=1e-12,
+                                                                  print_interval=0))
+    U0 = ini.quasi1d_initial(case.operators(), case.flow, geom)
+    names = [f"bz{i}" for i in range(1, 9)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J, grad, fwd, info = case.value_and_gradient("thrust", names=names, U0=U0,
+                                                 adjoint_tolerance=1e-10)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    print(f"adjoint: {fwd.operators.n_elem} elements, forward {fwd.iterations} it, adjoint "
+          f"{info['adjoint_iterations']} it, {t:.1f} s", file=sys.stderr, flush=True)
+    k = "bz4"
+    base = case.default_params()
+    h = 1e-6 * max(1.0, abs(base[k]))
+    vals = []
+    for sgn in (1.0, -1.0):
+        f = case.solve_forward({k: base[k] + sgn * h}, U0=fwd.U)
+        vals.append(sens.objective_and_partials(f.U, f.operators, case.flow, "thrust")[0])
+    fd = (vals[0] - vals[1]) / (2 * h)
+    return {
+        "workload": f"bezier9-tri-p2-14x7-refine{refine}", "elements": fwd.operators.n_elem,
+        "thrust": J, "forward_iterations": fwd.iterations,
+        "adjoint_iterations": info["adjoint_iterations"], "seconds": round(t, 3),
+        "gradients_per_s": round(len(names) / t, 4),
+        "gradient": {n: grad[n] for n in names},
+        "fd_check": {"name": k, "adjoint": grad[k], "central_difference": fd,
+                     "rel_diff": abs(grad[k] - fd) / max(abs(fd), 1e-300)},
+    }
+
+
 def _stops(res) -> dict:
     """Converged / total per (kind, order) family of the sweep."""
     fam: dict = {}
@@ -179,6 +265,12 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-march", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--shocked", action="store_true",
+                    help="also march the shocked quadrilateral case (quads Q2 P2, pb 0.7, HLLC)")
+    ap.add_argument("--shocked-refine", type=int, default=7)
+    ap.add_argument("--adjoint", action="store_true",
+                    help="also time the design-gradient case (degree-9 Bezier wall, tri P2)")
+    ap.add_argument("--adjoint-refine", type=int, default=5)
     ap.add_argument("--sweep", type=int, default=0,
                     help="also march this many designs of the mixed sweep (seed 1) in one batch")
     ap.add_argument("--sweep-tol", type=float, default=1e-10)
@@ -285,6 +377,10 @@ def main() -> int:
             "elements_scaled_per_stage": round(cnt[0] / (3 * (msteps + 1)), 1),
             "residual_norm_last": float(bk._res.item()),
         }
+    if args.shocked:
+        out["shocked"] = shocked_bench(args.shocked_refine, max(2, args.steps // 2))
+    if args.adjoint:
+        out["adjoint"] = adjoint_bench(args.adjoint_refine)
     if args.sweep > 0:
         out["sweep"] = sweep_bench(args.sweep, args.sweep_tol, args.sweep_lone, args.sweep_maxit)
     if not args.no_cpu:

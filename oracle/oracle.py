@@ -93,6 +93,29 @@ def or_flow(flow) -> OrFlow:
     )
 
 
+def positivity_limit(U, ops, flow, *, screen: bool = True):
+    """The Zhang-Shu limiter alone, from the limiter's own data (any operator
+    layout): ``(U_limited, flags, (n_scaled, n_repaired))``; input untouched
+    (``src/backends/_numba_kernels.py:927-1093``)."""
+    lib = load(False)
+    rd, ed = ops.ref, ops.topology.edges
+    out = _f64(U).copy()
+    E, N = out.shape[0], out.shape[1]
+    flags = np.zeros(E, dtype=np.uint8)
+    counts = np.zeros(2, dtype=np.int64)
+    if rd.order == 0:
+        return out, flags, (0, 0)
+    phi_vol, phi_face = _f64(rd.phi_vol), _f64(rd.phi_face)
+    face_side, mean_w = _i64(ed.face_side), _f64(ops.mean_weights)
+    leb = float(max(np.abs(rd.phi_vol).sum(axis=0).max(),
+                    np.abs(rd.phi_face).sum(axis=2).max())) if screen else 0.0
+    lib.or_positivity_limit(E, N, rd.n_qvol, rd.n_qface, ops.topology.n_faces, _p(out),
+                            _p(phi_vol), _p(phi_face), _p(face_side), _p(mean_w), flow.gamma,
+                            1e-8, 1e-8 * flow.stagnation_density, 1e-8 * flow.total_pressure, 12,
+                            leb, _p(flags), _p(counts))
+    return out, flags, (int(counts[0]), int(counts[1]))
+
+
 class CpuOracle:
     """Backend primitives on the CPU oracle (dense reference operator layout)."""
 

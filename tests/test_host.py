@@ -191,3 +191,35 @@ def test_operator_invariants(kind, q, p):
     # freestream: constant state has zero volume term when integrated against 1
     gx = ops.grad_x.sum(axis=1)
     assert np.abs(gx).max() < 1e-12
+
+
+def test_bezier9_default_is_the_smooth_contour():
+    """Degree elevation is exact: the unperturbed degree-9 wall is the smooth cubic."""
+    mesh = nz_module("mesh")
+    x = np.linspace(0.0, 1.0, 2001)
+    g9 = mesh.NozzleGeometry(contour="bezier9")
+    g3 = mesh.NozzleGeometry(contour="smooth")
+    assert np.abs(g9.wall(x) - g3.wall(x)).max() < 1e-15
+    # closed form of the cubic on the diverging section
+    p = g9.params()
+    s = np.clip((x - p["throat_x"]) / (1 - p["throat_x"]), 0, 1)
+    y_th, y_ex = p["throat_half_height"], p["throat_half_height"] * p["area_ratio"]
+    div = x > p["throat_x"]
+    np.testing.assert_allclose(g9.wall(x)[div], (y_th + (y_ex - y_th) * (3 * s**2 - 2 * s**3))[div],
+                               rtol=0, atol=1e-15)
+
+
+def test_bezier9_perturbation_moves_only_the_diverging_wall():
+    mesh, wl = nz_module("mesh"), nz_module("workloads")
+    g = wl.bezier9_geometry(0)
+    base = mesh.NozzleGeometry(contour="bezier9")
+    x = np.linspace(0.0, 1.0, 2001)
+    p = g.params()
+    conv = x <= p["throat_x"]
+    np.testing.assert_array_equal(g.wall(x)[conv], base.wall(x)[conv])
+    assert g.wall(np.array([1.0]))[0] == pytest.approx(base.wall(np.array([1.0]))[0], abs=1e-15)
+    h = np.array([p[f"bz{i}"] for i in range(1, 9)])
+    h0 = np.array([base.params()[f"bz{i}"] for i in range(1, 9)])
+    assert np.all(np.abs(h / h0 - 1.0) <= 0.05) and np.any(h != h0)
+    with pytest.raises(ValueError, match="eight|8 interior"):
+        mesh.NozzleGeometry(contour="bezier9", bezier9_heights=(0.2, 0.3)).params()

@@ -129,3 +129,15 @@ def test_planted_sign_error_is_caught():
     orc.face_sign = -orc.face_sign  # plant the mistake in the oracle, on the CPU
     R, _ = orc.residual(g["U0"])
     assert rel_err(R, g["R_roe_U0"]) > 1e-3
+
+
+def test_standalone_limiter_matches_oracle_limit():
+    """The operator-layout-free limiter entry point agrees with CpuOracle.limit."""
+    from oracle.oracle import CpuOracle, positivity_limit
+
+    ops, flow = build_case("quad_q2_p2", dense=True)
+    g = golden("quad_q2_p2")
+    a_U, a_f, a_c = CpuOracle(ops, flow).limit(g["Us"], "positivity")
+    b_U, b_f, b_c = positivity_limit(g["Us"], ops, flow)
+    assert np.array_equal(a_U, b_U) and np.array_equal(a_f, b_f) and a_c == b_c
+    assert np.array_equal(b_f, g["Us_flags"])
