@@ -189,7 +189,7 @@ class _Family:
         for d in self.active:
             a, b = int(self.eoff[d]), int(self.eoff[d + 1])
             for e0 in range(a, b, TILE):
-                tiles.append((e0, min(e0 + TILE, b)))
+                tiles.append((e0, min(e0 + TILE, b), len(tiles)))
             seg.append(len(tiles))
         dev = self.bk.device
         self.tiles = torch.tensor(np.asarray(tiles, dtype=np.int64).reshape(-1), device=dev)

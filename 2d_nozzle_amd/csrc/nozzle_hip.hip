@@ -427,14 +427,16 @@ __global__ __launch_bounds__(DG_BS) void k_dg(const nz_problem P, const double* 
 
     const int t = threadIdx.x;
     const int el = t >> 2, s = t & 3;  // component lanes
-    int64_t e0;
+    int64_t e0, slot;
     int ne;
-    if (P.tiles) {  // tile table: several disjoint meshes, or a subset of tiles
-        e0 = P.tiles[2 * blockIdx.x];
-        ne = static_cast<int>(P.tiles[2 * blockIdx.x + 1] - e0);
+    if (P.tiles) {  // tile table: dispatch order, disjoint meshes, or a subset of tiles
+        e0 = P.tiles[3 * blockIdx.x];
+        ne = static_cast<int>(P.tiles[3 * blockIdx.x + 1] - e0);
+        slot = P.tiles[3 * blockIdx.x + 2];
     } else {
         e0 = static_cast<int64_t>(blockIdx.x) * TE;
         ne = static_cast<int>(P.n_elem - e0 < TE ? P.n_elem - e0 : TE);
+        slot = blockIdx.x;
     }
     const bool live = el < ne;
     const int64_t e = e0 + el;
@@ -773,7 +775,7 @@ __global__ __launch_bounds__(DG_BS) void k_dg(const nz_problem P, const double* 
     if (mode == kStageFirst) {
         double tot = nacc;
         for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o, 64);
-        if (t == 0) partial[blockIdx.x] = tot;
+        if (t == 0) partial[slot] = tot;
     }
     if (P.limiter == 1 && N > 1) {  // superbee runs as separate passes
         uint8_t fl = 0;

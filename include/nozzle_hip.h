@@ -79,10 +79,12 @@ typedef struct {
     int32_t pad0;
     double wnorm[4];        /* residual-norm component weights (src/backends/base.py:42-78) */
     /* optional tile table for the element kernel of nz_residual / nz_rate /
-     * nz_stage: tile k covers elements [tiles[2k], tiles[2k+1]), at most 16
-     * of them, and its norm partial is scratch[k].  NULL: consecutive tiles of
-     * 16 over the whole mesh.  Used to march several disjoint meshes
-     * (designs) in one launch and to leave converged ones out. */
+     * nz_stage, one workgroup per entry in this order: entry k covers
+     * elements [tiles[3k], tiles[3k+1]), at most 16 of them, and writes its
+     * norm partial to scratch[tiles[3k+2]] (slots distinct, < n_tiles).
+     * NULL: consecutive tiles of 16 over the whole mesh, slot = tile index.
+     * Used to order the launch for cache locality, to march several disjoint
+     * meshes (designs) in one launch and to leave converged ones out. */
     const int64_t *tiles;
     int64_t n_tiles;
 } nz_problem;
