@@ -74,6 +74,50 @@ def _tables(rd) -> tuple[np.ndarray, dict]:
     return np.concatenate(chunks), off
 
 
+TILE = 16                      # elements per tile of the element kernel
+XCDS = 8                       # L2 domains a launch is dealt round-robin across
+LOCALITY_MIN_ELEMENTS = 1 << 14
+
+
+def _morton2(ix: np.ndarray, iy: np.ndarray) -> np.ndarray:
+    def spread(v):
+        v = v.astype(np.uint64) & np.uint64(0xFFFF)
+        for sh, m in ((8, 0x00FF00FF), (4, 0x0F0F0F0F), (2, 0x33333333), (1, 0x55555555)):
+            v = (v | (v << np.uint64(sh))) & np.uint64(m)
+        return v
+    return spread(ix) | (spread(iy) << np.uint64(1))
+
+
+def locality_tiles(coords: np.ndarray, elem_nodes: np.ndarray, E: int,
+                   n_domains: int = XCDS) -> np.ndarray:
+    """Tile table ``(T, 3)``: ``(first element, end, norm slot)`` per workgroup.
+
+    Tiles are the consecutive 16-element blocks (slot = block index).  They
+    are ordered along a Morton curve of their centroids, and the order is dealt
+    so that workgroup ``b``, which runs on L2 domain ``b % n_domains``, takes
+    the ``b // n_domains``-th tile of that domain's contiguous share.
+    """
+    T = (E + TILE - 1) // TILE
+    starts = np.arange(0, E, TILE)
+    cent = coords[elem_nodes].mean(axis=1)
+    cnt = np.diff(np.append(starts, E))
+    tc = np.stack([np.add.reduceat(cent[:, k], starts) / cnt for k in range(2)], axis=1)
+    lo, span = tc.min(axis=0), np.ptp(tc, axis=0) + 1e-300
+    q = np.minimum((tc - lo) / span * 65536.0, 65535.0).astype(np.int64)
+    order = np.argsort(_morton2(q[:, 0], q[:, 1]), kind="stable")
+    share = -(-T // n_domains)
+    b = np.arange(T)
+    pos = (b % n_domains) * share + b // n_domains
+    ok = pos < T
+    launch = np.empty(T, dtype=np.int64)
+    launch[ok] = order[pos[ok]]
+    taken = np.zeros(T, dtype=bool)
+    taken[pos[ok]] = True
+    launch[~ok] = order[~taken]
+    e0 = launch * TILE
+    return np.stack([e0, np.minimum(e0 + TILE, E), launch], axis=1).astype(np.int64)
+
+
 class HipBackend:
     """MI355X execution of the DG residual and the pseudo-time march."""
 
@@ -225,6 +269,15 @@ class HipBackend:
             nsb = int(self.lib.nz_superbee_work_doubles(self._pp))
             self._sb_work = torch.empty(nsb, **f64)
         self._flags = torch.zeros(E, dtype=torch.uint8, device=self.device)
+        # launch order of the element kernel's tiles: spatially coherent, and
+        # one contiguous run of it per XCD, so that the neighbour columns a
+        # tile gathers were just read through the same L2 (norm slots keep the
+        # element order, so results do not depend on the launch order)
+        self._tiles = None
+        if E >= LOCALITY_MIN_ELEMENTS:
+            tab = locality_tiles(np.asarray(ops.node_coords), np.asarray(topo.elem_nodes), E)
+            self._tiles = torch.from_numpy(tab.reshape(-1)).to(self.device)
+            P.tiles, P.n_tiles = C.c_void_p(self._tiles.data_ptr()), tab.shape[0]
 
     # ------------------------------------------------------------ helpers
     def _stream(self):
