@@ -237,9 +237,9 @@ struct Tile {
     static constexpr int FACE = 3 * NQF * QS + NQF * TE;
     // point states / xi-flux (two buffers, alternating rounds), eta-flux
     static constexpr int VOL = 3 * QR * QS;
-    // staging of U in and out: rows of one element; a stride of 4 x odd
-    // doubles keeps the per-component reads conflict-free
-    static constexpr int SIO = ((NS + 3) / 4) % 2 ? ((NS + 3) / 4) * 4 : ((NS + 3) / 4 + 1) * 4;
+    // staging of U in and out: the tile's rows exactly as in HBM, so the
+    // copies in and out are plain contiguous moves
+    static constexpr int SIO = NS;
     static constexpr int IO = TE * SIO;
     static constexpr int WORK = FACE > VOL ? (FACE > IO ? FACE : IO) : (VOL > IO ? VOL : IO);
     // staged once per tile: edge geometry of every element face (nx, ny,
@@ -407,6 +407,19 @@ __device__ uint8_t limit_column(double* col, int s, const double* mw, const doub
     return fl;
 }
 
+// the staged tile (rows of NS doubles, as in HBM) back to global memory
+template <class L>
+__device__ __forceinline__ void store_tile(double* __restrict__ out, const double* tile, int t,
+                                           int ne) {
+    constexpr int NU = TE * L::NS / DG_BS;
+    if (ne == TE) {
+#pragma unroll
+        for (int j = 0; j < NU; ++j) out[t + j * DG_BS] = tile[t + j * DG_BS];
+    } else {
+        for (int it = t; it < ne * L::NS; it += DG_BS) out[it] = tile[it];
+    }
+}
+
 template <class C, int FLUX, bool STAGE>
 __global__ __launch_bounds__(DG_BS) void k_dg(const nz_problem P, const double* __restrict__ T,
                                            const double* __restrict__ Uin,
@@ -475,10 +488,7 @@ __global__ __launch_bounds__(DG_BS) void k_dg(const nz_problem P, const double* 
             rg[j] = C::AFF && it < ne * 5 ? P.geo[e0 * 5 + it] : 0.0;
         }
 #pragma unroll
-        for (int j = 0; j < NU; ++j) {
-            const int it = t + j * DG_BS, r = it / NS, c = it - r * NS;
-            sW[r * L::SIO + c] = ru[j];
-        }
+        for (int j = 0; j < NU; ++j) sW[t + j * DG_BS] = ru[j];
 #pragma unroll
         for (int j = 0; j < NM; ++j) {
             const int it = t + j * DG_BS;
@@ -689,16 +699,11 @@ __global__ __launch_bounds__(DG_BS) void k_dg(const nz_problem P, const double* 
                     hy = gq[3];
                     wd = gq[4];
                 }
+                const double sc = fold ? 1.0 : wd;  // exact when folded
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
-                    double a = gx * F[c] + gy * G[c];
-                    double b = hx * F[c] + hy * G[c];
-                    if (!fold) {
-                        a *= wd;
-                        b *= wd;
-                    }
-                    sUq[at(qq, c, pe)] = a;
-                    sFy[at(qq, c, pe)] = b;
+                    sUq[at(qq, c, pe)] = (gx * F[c] + gy * G[c]) * sc;
+                    sFy[at(qq, c, pe)] = (hx * F[c] + hy * G[c]) * sc;
                 }
             }
         }
@@ -742,10 +747,7 @@ __global__ __launch_bounds__(DG_BS) void k_dg(const nz_problem P, const double* 
         for (int i = 0; i < N; ++i) sW[el * L::SIO + i * 4 + s] = osign * acc[i];
         if (s == 0 && live) dt[e] = wave;
         __syncthreads();
-        for (int it = t; it < TE * NS; it += DG_BS) {
-            const int r = it / NS, c = it - r * NS;
-            if (r < ne) Uout[(e0 + r) * NS + c] = sW[r * L::SIO + c];
-        }
+        store_tile<L>(Uout + e0 * NS, sW, t, ne);
         return;
     }
 
@@ -800,10 +802,7 @@ __global__ __launch_bounds__(DG_BS) void k_dg(const nz_problem P, const double* 
 #pragma unroll
     for (int i = 0; i < N; ++i) sW[el * L::SIO + i * 4 + s] = u[i];
     __syncthreads();
-    for (int it = t; it < TE * NS; it += DG_BS) {
-        const int r = it / NS, c = it - r * NS;
-        if (r < ne) Uout[(e0 + r) * NS + c] = sW[r * L::SIO + c];
-    }
+    store_tile<L>(Uout + e0 * NS, sW, t, ne);
 }
 
 // ------------------------------------------------- boundary-face pre-pass
