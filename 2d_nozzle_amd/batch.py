@@ -383,3 +383,143 @@ class BatchedSteady:
                 if results[g] is None:
                     finish(fam, d, False, done, float("nan"), "stopped")
         return results
+
+
+# ------------------------------------------------------------- batched adjoint
+@dataclass
+class DesignGradient:
+    design: Design
+    objective: float
+    gradient: dict
+    adjoint_iterations: int
+    adjoint_relative_residual: float
+    converged: bool
+
+
+def _design_coords(dsg: Design, area_ratio: float) -> np.ndarray:
+    """Node coordinates of a design with the logical grid frozen at the design."""
+    from .mesh import nozzle_node_coords
+
+    geom = dsg.geometry()
+    _, _, grids = build_nozzle_mesh(geom, kind=dsg.kind, nx=dsg.nx, nr=dsg.nr, refine=dsg.refine,
+                                    geometry_order=dsg.geometry_order)
+    p = dict(geom.params())
+    p["area_ratio"] = area_ratio
+    return nozzle_node_coords(grids["x_frac"], grids["r_frac"], p, geom.contour)
+
+
+def batched_gradients(results: list, flow: FlowConditions, opts: SolverOptions, *,
+                      objective: str = "thrust", tolerance: float = 1e-8,
+                      max_iterations: int = 100_000, check_interval: int = 100,
+                      fd_step: float = 1e-6, device=None) -> list:
+    """d objective / d area ratio for every converged design of a batch, from
+    steady adjoints marched together (``src/sensitivity.py:289-460``).
+
+    Designs are grouped in families as in the forward batch.  Each family's
+    adjoint march is the lone one (classical RK4 in pseudo-time with the
+    frozen local step of the converged state): element-local updates on a
+    block-diagonal mesh, so each design's iterates are those it has alone.
+    Every ``check_interval`` steps each design's defect ``rms(vjp(lam) -
+    seed) / rms(seed)`` is tested on its own slice; a design that passes is
+    copied out at that step.  The geometry cotangents of the whole family come
+    from one VJP launch and are split back per design for the host chain to
+    node coordinates and the design variable.
+    """
+    import torch
+
+    from .backend import HipBackend
+    from .sensitivity import edge_vjp, metric_vjp, objective_and_partials
+
+    out: list = [None] * len(results)
+    fams: dict = {}
+    for g, r in enumerate(results):
+        if r is None or not r.converged:
+            continue
+        o = r.operators
+        fams.setdefault((o.topology.kind, o.topology.geometry_order, o.ref.order), []).append(g)
+    for idx in fams.values():
+        ops_list = [results[g].operators for g in idx]
+        cat = concat_operators(ops_list)
+        bk = HipBackend(cat, flow, opts, device=device)
+        dev = bk.device
+        eoff, ioff, boff = cat.elem_offset, cat.interior_offset, cat.boundary_offset
+        NI, nqf = cat.n_interior, cat.ref.n_qface
+        parts = [objective_and_partials(results[g].U, results[g].operators, flow, objective)
+                 for g in idx]
+        u = torch.from_numpy(np.concatenate([results[g].U for g in idx])).to(dev)
+        s = torch.from_numpy(np.concatenate([p[1] for p in parts])).to(dev)
+        a = bk._adj_buffers()
+        st = bk._stream()
+        # frozen step from the converged states (element-local)
+        Fr = torch.empty_like(u)
+        wave = torch.empty(cat.n_elem, dtype=torch.float64, device=dev)
+        bk._chk(bk.lib.nz_rate(bk._pp, bk._p(u), bk._p(Fr), bk._p(wave), st), "nz_rate")
+        from .config import step_coefficient
+
+        fac = step_coefficient(cat.ref.order, opts.cfl, "ssprk3")
+        bk._chk(bk.lib.nz_time_step(bk._pp, bk._p(wave), fac, bk._p(a["dt"]), st), "nz_time_step")
+        lam, tmp, nxt = a["lam"]
+        lam.zero_()
+        sl = [slice(int(eoff[j]), int(eoff[j + 1])) for j in range(len(idx))]
+        norm0 = [float(torch.sqrt(torch.mean(s[sl[j]] * s[sl[j]])).item()) for j in range(len(idx))]
+        lam_out = torch.zeros_like(u)
+        its = [0] * len(idx)
+        rels = [float("nan")] * len(idx)
+        done = [norm0[j] == 0.0 for j in range(len(idx))]
+        for j in range(len(idx)):
+            if done[j]:
+                its[j], rels[j] = 0, 0.0
+        work, acc, dtp = bk._p(a["work"]), bk._p(a["acc"]), bk._p(a["dt"])
+        pu, ps = bk._p(u), bk._p(s)
+        dU = torch.empty_like(u)
+        it = 0
+        while it < max_iterations and not all(done):
+            n = min(check_interval, max_iterations - it)
+            for _ in range(n):
+                bk._chk(bk.lib.nz_adjoint_rk4_stage(bk._pp, pu, bk._p(lam), bk._p(lam), ps, dtp,
+                                                    acc, bk._p(tmp), 1, work, st), "rk4")
+                bk._chk(bk.lib.nz_adjoint_rk4_stage(bk._pp, pu, bk._p(tmp), bk._p(lam), ps, dtp,
+                                                    acc, bk._p(nxt), 2, work, st), "rk4")
+                bk._chk(bk.lib.nz_adjoint_rk4_stage(bk._pp, pu, bk._p(nxt), bk._p(lam), ps, dtp,
+                                                    acc, bk._p(tmp), 3, work, st), "rk4")
+                bk._chk(bk.lib.nz_adjoint_rk4_stage(bk._pp, pu, bk._p(tmp), bk._p(lam), ps, dtp,
+                                                    acc, bk._p(nxt), 4, work, st), "rk4")
+                lam, nxt = nxt, lam
+            it += n
+            bk._chk(bk.lib.nz_residual_vjp(bk._pp, pu, bk._p(lam), bk._p(dU), None, None, None,
+                                           work, st), "nz_residual_vjp")
+            d2 = (dU - s) ** 2
+            for j in range(len(idx)):
+                if done[j]:
+                    continue
+                r = float(torch.sqrt(torch.mean(d2[sl[j]])).item())
+                if not np.isfinite(r) or r <= tolerance * norm0[j]:
+                    lam_out[sl[j]] = lam[sl[j]]
+                    its[j], rels[j], done[j] = it, r / norm0[j], True
+        for j in range(len(idx)):
+            if not done[j]:
+                lam_out[sl[j]] = lam[sl[j]]
+                its[j] = it
+        # geometry cotangents of the whole family, split per design
+        rd = cat.ref
+        gm = torch.zeros((cat.n_elem, rd.n_qvol, 5), dtype=torch.float64, device=dev)
+        ge = torch.zeros((cat.n_edges_total, nqf, 3), dtype=torch.float64, device=dev)
+        gp = torch.zeros(max(cat.n_boundary * nqf, 1), dtype=torch.float64, device=dev)
+        bk._chk(bk.lib.nz_residual_vjp(bk._pp, pu, bk._p(lam_out), bk._p(dU), bk._p(gm),
+                                       bk._p(ge), bk._p(gp), work, st), "nz_residual_vjp")
+        gm, ge = gm.cpu().numpy(), ge.cpu().numpy()
+        for j, g in enumerate(idx):
+            ops = results[g].operators
+            ge_d = np.concatenate([ge[ioff[j]:ioff[j + 1]], ge[NI + boff[j]:NI + boff[j + 1]]])
+            J, _, J_edge = parts[j]
+            gX = edge_vjp(ops, J_edge) - metric_vjp(ops, gm[sl[j]]) - edge_vjp(ops, ge_d)
+            dsg = results[g].design
+            h = fd_step * max(1.0, abs(dsg.area_ratio))
+            dX = (_design_coords(dsg, dsg.area_ratio + h)
+                  - _design_coords(dsg, dsg.area_ratio - h)) / (2 * h)
+            conv = bool(np.isfinite(rels[j]) and rels[j] <= tolerance)
+            out[g] = DesignGradient(design=dsg, objective=J,
+                                    gradient={"area_ratio": float((gX * dX).sum())},
+                                    adjoint_iterations=its[j],
+                                    adjoint_relative_residual=rels[j], converged=conv)
+    return out

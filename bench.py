@@ -243,6 +243,35 @@ def sweep_bench(n: int, tol: float, n_lone: int, max_it: int = 50000) -> dict:
     lone_rate = len(pick) / t_lone
     print(f"sweep: one at a time {t_lone:.1f} s for {len(pick)} designs", file=sys.stderr,
           flush=True)
+    # thrust gradients w.r.t. the area ratio: batched adjoints of every
+    # converged design, against the lone forward + adjoint of the sampled ones
+    sens = _pkg("sensitivity")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    grads = batch.batched_gradients(res, flow, opts, tolerance=1e-8)
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
+    n_grad = sum(1 for gr in grads if gr is not None and gr.converged)
+    t_lone_g, n_lone_g, worst = 0.0, 0, 0.0
+    for g in pick:
+        if grads[g] is None:
+            continue
+        case = sens.DifferentiableCase(geometry=designs[g].geometry(), flow=flow,
+                                       order=designs[g].order, refine=designs[g].refine,
+                                       kind=designs[g].kind, options=opts)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, lg, _, _ = case.value_and_gradient("thrust", names=("area_ratio",),
+                                              U0=run.U0_list[g], adjoint_tolerance=1e-8)
+        torch.cuda.synchronize()
+        t_lone_g += time.perf_counter() - t0
+        n_lone_g += 1
+        worst = max(worst, abs(lg["area_ratio"] - grads[g].gradient["area_ratio"])
+                    / max(abs(lg["area_ratio"]), 1e-300))
+    grad_rate = n_grad / (t_batch + t_adj)
+    lone_grad_rate = n_lone_g / t_lone_g if t_lone_g > 0 else float("nan")
+    print(f"sweep: batched gradients {n_grad} in {t_batch + t_adj:.1f} s; lone {n_lone_g} in "
+          f"{t_lone_g:.1f} s", file=sys.stderr, flush=True)
     return {
         "designs": n, "elements": run.n_elements, "families": len(run.families),
         "tolerance": tol, "max_iterations": max_it, "converged": conv,
@@ -254,6 +283,11 @@ def sweep_bench(n: int, tol: float, n_lone: int, max_it: int = 50000) -> dict:
         "speedup": round((n / t_batch) / lone_rate, 2),
         "lone_bit_identical": f"{lone_match}/{len(pick)}",
         "host_setup_s": round(run.host_setup_s, 1),
+        "gradients": n_grad, "adjoint_batch_s": round(t_adj, 3),
+        "gradients_per_s": round(grad_rate, 4),
+        "lone_gradients_per_s": round(lone_grad_rate, 4),
+        "gradient_speedup": round(grad_rate / lone_grad_rate, 2),
+        "lone_gradient_max_rel_diff": worst,
     }
 
 

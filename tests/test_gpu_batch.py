@@ -70,3 +70,24 @@ def test_eager_and_graph_replay_agree(sweep):
     for a, b in zip(results, eager):
         assert a.iterations == b.iterations
         assert np.array_equal(a.U, b.U)
+
+
+def test_batched_gradients_match_lone_adjoints(sweep):
+    """Thrust gradients w.r.t. the area ratio from the batched adjoint march
+    agree with each design's own adjoint solve (DifferentiableCase)."""
+    run, results, flow, opts = sweep
+    batch, sens = nz_module("batch"), nz_module("sensitivity")
+    grads = batch.batched_gradients(results, flow, opts, tolerance=1e-8)
+    for g, (res, gr) in enumerate(zip(results, grads)):
+        assert gr is not None and gr.converged
+        case = sens.DifferentiableCase(geometry=res.design.geometry(), flow=flow,
+                                       order=res.design.order, refine=res.design.refine,
+                                       kind=res.design.kind, options=opts)
+        J, grad, fwd, info = case.value_and_gradient("thrust", names=("area_ratio",),
+                                                     U0=run.U0_list[g], adjoint_tolerance=1e-8)
+        print(f"design {g}: batch dJ/dAR {gr.gradient['area_ratio']:.10e} "
+              f"({gr.adjoint_iterations} it), lone {grad['area_ratio']:.10e} "
+              f"({info['adjoint_iterations']} it)", flush=True)
+        assert J == gr.objective
+        assert gr.adjoint_iterations == info["adjoint_iterations"]
+        assert gr.gradient["area_ratio"] == pytest.approx(grad["area_ratio"], rel=1e-10)
