@@ -411,115 +411,166 @@ def _design_coords(dsg: Design, area_ratio: float) -> np.ndarray:
 def batched_gradients(results: list, flow: FlowConditions, opts: SolverOptions, *,
                       objective: str = "thrust", tolerance: float = 1e-8,
                       max_iterations: int = 100_000, check_interval: int = 100,
-                      fd_step: float = 1e-6, device=None) -> list:
+                      fd_step: float = 1e-6, group_size: int = 8, device=None) -> list:
     """d objective / d area ratio for every converged design of a batch, from
     steady adjoints marched together (``src/sensitivity.py:289-460``).
 
-    Designs are grouped in families as in the forward batch.  Each family's
-    adjoint march is the lone one (classical RK4 in pseudo-time with the
-    frozen local step of the converged state): element-local updates on a
-    block-diagonal mesh, so each design's iterates are those it has alone.
-    Every ``check_interval`` steps each design's defect ``rms(vjp(lam) -
-    seed) / rms(seed)`` is tested on its own slice; a design that passes is
-    copied out at that step.  The geometry cotangents of the whole family come
-    from one VJP launch and are split back per design for the host chain to
-    node coordinates and the design variable.
+    Designs are grouped in families as in the forward batch, split into groups
+    of similar forward iteration counts, and the groups march concurrently,
+    each on its own stream.  Each family's adjoint march is
+    the lone one (classical RK4 in pseudo-time with the frozen local step of
+    the converged state): element-local updates on a block-diagonal mesh, so
+    each design's iterates are those it has alone.  Every ``check_interval``
+    steps each design's defect ``rms(vjp(lam) - seed) / rms(seed)`` is tested
+    on its own segment; a design that passes is copied out at that step.  The
+    geometry cotangents of a whole family come from one VJP launch and are
+    split back per design for the host chain to node coordinates and the
+    design variable.
     """
     import torch
 
     from .backend import HipBackend
+    from .config import step_coefficient
     from .sensitivity import edge_vjp, metric_vjp, objective_and_partials
 
     out: list = [None] * len(results)
-    fams: dict = {}
+    groups: dict = {}
     for g, r in enumerate(results):
         if r is None or not r.converged:
             continue
         o = r.operators
-        fams.setdefault((o.topology.kind, o.topology.geometry_order, o.ref.order), []).append(g)
-    for idx in fams.values():
-        ops_list = [results[g].operators for g in idx]
-        cat = concat_operators(ops_list)
-        bk = HipBackend(cat, flow, opts, device=device)
-        dev = bk.device
-        eoff, ioff, boff = cat.elem_offset, cat.interior_offset, cat.boundary_offset
-        NI, nqf = cat.n_interior, cat.ref.n_qface
-        parts = [objective_and_partials(results[g].U, results[g].operators, flow, objective)
-                 for g in idx]
-        u = torch.from_numpy(np.concatenate([results[g].U for g in idx])).to(dev)
-        s = torch.from_numpy(np.concatenate([p[1] for p in parts])).to(dev)
-        a = bk._adj_buffers()
-        st = bk._stream()
-        # frozen step from the converged states (element-local)
-        Fr = torch.empty_like(u)
-        wave = torch.empty(cat.n_elem, dtype=torch.float64, device=dev)
-        bk._chk(bk.lib.nz_rate(bk._pp, bk._p(u), bk._p(Fr), bk._p(wave), st), "nz_rate")
-        from .config import step_coefficient
+        groups.setdefault((o.topology.kind, o.topology.geometry_order, o.ref.order), []).append(g)
+    # Within a family, designs with similar forward iteration counts (a proxy
+    # for the adjoint's) march together in groups, so that a group stops soon
+    # after its slowest design instead of the family's.
+    parts = []
+    for idx in groups.values():
+        idx = sorted(idx, key=lambda g: results[g].iterations)
+        parts += [idx[k:k + group_size] for k in range(0, len(idx), group_size)]
 
-        fac = step_coefficient(cat.ref.order, opts.cfl, "ssprk3")
-        bk._chk(bk.lib.nz_time_step(bk._pp, bk._p(wave), fac, bk._p(a["dt"]), st), "nz_time_step")
-        lam, tmp, nxt = a["lam"]
-        lam.zero_()
-        sl = [slice(int(eoff[j]), int(eoff[j + 1])) for j in range(len(idx))]
-        norm0 = [float(torch.sqrt(torch.mean(s[sl[j]] * s[sl[j]])).item()) for j in range(len(idx))]
-        lam_out = torch.zeros_like(u)
-        its = [0] * len(idx)
-        rels = [float("nan")] * len(idx)
-        done = [norm0[j] == 0.0 for j in range(len(idx))]
-        for j in range(len(idx)):
-            if done[j]:
-                its[j], rels[j] = 0, 0.0
-        work, acc, dtp = bk._p(a["work"]), bk._p(a["acc"]), bk._p(a["dt"])
-        pu, ps = bk._p(u), bk._p(s)
-        dU = torch.empty_like(u)
-        it = 0
-        while it < max_iterations and not all(done):
-            n = min(check_interval, max_iterations - it)
+    fams = []
+    for idx in parts:
+        f = SimpleNamespace(idx=idx)
+        f.cat = concat_operators([results[g].operators for g in idx])
+        f.bk = bk = HipBackend(f.cat, flow, opts, device=device)
+        f.stream = torch.cuda.Stream(device=bk.device)
+        dev = bk.device
+        f.parts = [objective_and_partials(results[g].U, results[g].operators, flow, objective)
+                   for g in idx]
+        f.u = torch.from_numpy(np.concatenate([results[g].U for g in idx])).to(dev)
+        f.s = torch.from_numpy(np.concatenate([p[1] for p in f.parts])).to(dev)
+        eoff = f.cat.elem_offset
+        f.sl = [slice(int(eoff[j]), int(eoff[j + 1])) for j in range(len(idx))]
+        # per-design sums over contiguous element segments as a padded gather
+        # (graph-capturable, fixed order)
+        lens = np.diff(eoff)
+        kk = np.arange(int(lens.max()))
+        f.seg_idx = torch.tensor(np.where(kk[None, :] < lens[:, None], eoff[:-1, None] + kk[None, :],
+                                          0), dtype=torch.int64, device=dev)
+        f.seg_mask = torch.tensor(kk[None, :] < lens[:, None], dtype=torch.float64, device=dev)
+        f.count = torch.tensor(np.diff(eoff) * f.cat.ref.n_basis * 4.0, dtype=torch.float64,
+                               device=dev)
+        f.a = bk._adj_buffers()
+        f.lam_out = torch.zeros_like(f.u)
+        f.dU = torch.empty_like(f.u)
+        with torch.cuda.stream(f.stream):
+            st = bk._stream()
+            Fr = torch.empty_like(f.u)
+            wave = torch.empty(f.cat.n_elem, dtype=torch.float64, device=dev)
+            bk._chk(bk.lib.nz_rate(bk._pp, bk._p(f.u), bk._p(Fr), bk._p(wave), st), "nz_rate")
+            fac = step_coefficient(f.cat.ref.order, opts.cfl, "ssprk3")
+            bk._chk(bk.lib.nz_time_step(bk._pp, bk._p(wave), fac, bk._p(f.a["dt"]), st),
+                    "nz_time_step")
+            f.a["lam"][0].zero_()
+            e2 = (f.s * f.s).sum(dim=(1, 2))
+            f.norm0 = torch.sqrt((e2[f.seg_idx] * f.seg_mask).sum(dim=1) / f.count).cpu().numpy()
+        f.done = [bool(v == 0.0) for v in f.norm0]
+        f.its = [0] * len(idx)
+        f.rels = [0.0 if d else float("nan") for d in f.done]
+        fams.append(f)
+
+    def launch(f, n):
+        # an even chunk leaves the lambda buffers in their roles: capture it
+        # once per family and replay it
+        if n % 2 == 0:
+            if getattr(f, "graph", None) is None or f.graph_n != n:
+                f.stream.synchronize()
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g, stream=f.stream):
+                    launch_eager(f, n)
+                f.graph, f.graph_n = g, n
+            with torch.cuda.stream(f.stream):
+                f.graph.replay()
+        else:
+            launch_eager(f, n)
+
+    def launch_eager(f, n):
+        bk, (lam, tmp, nxt) = f.bk, f.a["lam"]
+        work, acc, dtp = bk._p(f.a["work"]), bk._p(f.a["acc"]), bk._p(f.a["dt"])
+        pu, ps = bk._p(f.u), bk._p(f.s)
+        with torch.cuda.stream(f.stream):
+            st = bk._stream()
             for _ in range(n):
-                bk._chk(bk.lib.nz_adjoint_rk4_stage(bk._pp, pu, bk._p(lam), bk._p(lam), ps, dtp,
-                                                    acc, bk._p(tmp), 1, work, st), "rk4")
-                bk._chk(bk.lib.nz_adjoint_rk4_stage(bk._pp, pu, bk._p(tmp), bk._p(lam), ps, dtp,
-                                                    acc, bk._p(nxt), 2, work, st), "rk4")
-                bk._chk(bk.lib.nz_adjoint_rk4_stage(bk._pp, pu, bk._p(nxt), bk._p(lam), ps, dtp,
-                                                    acc, bk._p(tmp), 3, work, st), "rk4")
-                bk._chk(bk.lib.nz_adjoint_rk4_stage(bk._pp, pu, bk._p(tmp), bk._p(lam), ps, dtp,
-                                                    acc, bk._p(nxt), 4, work, st), "rk4")
+                for stage, (li, lo) in enumerate(((lam, tmp), (tmp, nxt), (nxt, tmp), (tmp, nxt)),
+                                                 start=1):
+                    bk._chk(bk.lib.nz_adjoint_rk4_stage(bk._pp, pu, bk._p(li), bk._p(lam), ps,
+                                                        dtp, acc, bk._p(lo), stage, work, st),
+                            "rk4")
                 lam, nxt = nxt, lam
-            it += n
-            bk._chk(bk.lib.nz_residual_vjp(bk._pp, pu, bk._p(lam), bk._p(dU), None, None, None,
+            f.a["lam"] = [lam, tmp, nxt]
+            bk._chk(bk.lib.nz_residual_vjp(bk._pp, pu, bk._p(lam), bk._p(f.dU), None, None, None,
                                            work, st), "nz_residual_vjp")
-            d2 = (dU - s) ** 2
-            for j in range(len(idx)):
-                if done[j]:
+            d2 = ((f.dU - f.s) ** 2).sum(dim=(1, 2))
+            f.res_dev = torch.sqrt((d2[f.seg_idx] * f.seg_mask).sum(dim=1) / f.count)
+
+    it = 0
+    while it < max_iterations and any(not all(f.done) for f in fams):
+        n = min(check_interval, max_iterations - it)
+        live = [f for f in fams if not all(f.done)]
+        for f in live:
+            launch(f, n)
+        it += n
+        for f in live:
+            f.stream.synchronize()
+            res = f.res_dev.cpu().numpy()
+            lam = f.a["lam"][0]
+            for j in range(len(f.idx)):
+                if f.done[j]:
                     continue
-                r = float(torch.sqrt(torch.mean(d2[sl[j]])).item())
-                if not np.isfinite(r) or r <= tolerance * norm0[j]:
-                    lam_out[sl[j]] = lam[sl[j]]
-                    its[j], rels[j], done[j] = it, r / norm0[j], True
-        for j in range(len(idx)):
-            if not done[j]:
-                lam_out[sl[j]] = lam[sl[j]]
-                its[j] = it
+                r = float(res[j])
+                if not np.isfinite(r) or r <= tolerance * f.norm0[j]:
+                    f.lam_out[f.sl[j]] = lam[f.sl[j]]
+                    f.its[j], f.rels[j], f.done[j] = it, r / f.norm0[j], True
+    for f in fams:
+        lam = f.a["lam"][0]
+        for j in range(len(f.idx)):
+            if not f.done[j]:
+                f.lam_out[f.sl[j]] = lam[f.sl[j]]
+                f.its[j] = it
         # geometry cotangents of the whole family, split per design
-        rd = cat.ref
+        bk, cat = f.bk, f.cat
+        rd, nqf = cat.ref, cat.ref.n_qface
+        dev = bk.device
         gm = torch.zeros((cat.n_elem, rd.n_qvol, 5), dtype=torch.float64, device=dev)
         ge = torch.zeros((cat.n_edges_total, nqf, 3), dtype=torch.float64, device=dev)
         gp = torch.zeros(max(cat.n_boundary * nqf, 1), dtype=torch.float64, device=dev)
-        bk._chk(bk.lib.nz_residual_vjp(bk._pp, pu, bk._p(lam_out), bk._p(dU), bk._p(gm),
-                                       bk._p(ge), bk._p(gp), work, st), "nz_residual_vjp")
+        bk._chk(bk.lib.nz_residual_vjp(bk._pp, bk._p(f.u), bk._p(f.lam_out), bk._p(f.dU),
+                                       bk._p(gm), bk._p(ge), bk._p(gp), bk._p(f.a["work"]),
+                                       bk._stream()), "nz_residual_vjp")
         gm, ge = gm.cpu().numpy(), ge.cpu().numpy()
-        for j, g in enumerate(idx):
+        ioff, boff, NI = cat.interior_offset, cat.boundary_offset, cat.n_interior
+        for j, g in enumerate(f.idx):
             ops = results[g].operators
             ge_d = np.concatenate([ge[ioff[j]:ioff[j + 1]], ge[NI + boff[j]:NI + boff[j + 1]]])
-            J, _, J_edge = parts[j]
-            gX = edge_vjp(ops, J_edge) - metric_vjp(ops, gm[sl[j]]) - edge_vjp(ops, ge_d)
+            J, _, J_edge = f.parts[j]
+            gX = edge_vjp(ops, J_edge) - metric_vjp(ops, gm[f.sl[j]]) - edge_vjp(ops, ge_d)
             dsg = results[g].design
             h = fd_step * max(1.0, abs(dsg.area_ratio))
             dX = (_design_coords(dsg, dsg.area_ratio + h)
                   - _design_coords(dsg, dsg.area_ratio - h)) / (2 * h)
-            conv = bool(np.isfinite(rels[j]) and rels[j] <= tolerance)
+            conv = bool(np.isfinite(f.rels[j]) and f.rels[j] <= tolerance)
             out[g] = DesignGradient(design=dsg, objective=J,
                                     gradient={"area_ratio": float((gX * dX).sum())},
-                                    adjoint_iterations=its[j],
-                                    adjoint_relative_residual=rels[j], converged=conv)
+                                    adjoint_iterations=f.its[j],
+                                    adjoint_relative_residual=f.rels[j], converged=conv)
     return out
