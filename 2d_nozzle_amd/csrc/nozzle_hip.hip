@@ -232,11 +232,12 @@ template <class C>
 struct Tile {
     static constexpr int N = C::N, NQV = C::NQV, NQF = C::NQF, NF = C::NF, NS = C::NS;
     static constexpr int QR = DG_BS / TE;  // volume points per round: one per lane
-    // own trace / weighted flux (two buffers, alternating faces), neighbour
-    // trace, signal speeds
-    static constexpr int FACE = 3 * NQF * QS + NQF * TE;
-    // point states / xi-flux (two buffers, alternating rounds), eta-flux
-    static constexpr int VOL = 3 * QR * QS;
+    // own trace / weighted flux, neighbour trace, signal speeds.  One buffer
+    // each: a one-wave workgroup executes its LDS operations in program
+    // order, so the next face may overwrite rows the last lift has read.
+    static constexpr int FACE = 2 * NQF * QS + NQF * TE;
+    // point states / xi-flux, eta-flux
+    static constexpr int VOL = 2 * QR * QS;
     // staging of U in and out: the tile's rows exactly as in HBM, so the
     // copies in and out are plain contiguous moves
     static constexpr int SIO = NS;
@@ -248,6 +249,13 @@ struct Tile {
     static constexpr int GF = C::STR ? 3 : 3 * NQF;
     static constexpr int EGN = TE * NF * GF;
     static constexpr int GEN = C::AFF ? TE * 5 : 1;
+    // LDS bytes of one tile; when four tiles fit each SIMD's share of the
+    // CU's 160 KB, the register budget is capped at four waves per SIMD
+    // (128 VGPRs) so that the LDS and the registers allow the same
+    // occupancy: the fourth wave hides more of the latency than the few
+    // spilled registers cost.
+    static constexpr int LDS = 8 * (WORK + NF * N * NQF + EGN + GEN) + 8 * TE * NF;
+    static constexpr int WPE = LDS * 16 <= 160 * 1024 ? 4 : 1;
 };
 
 // component k of this lane's element (lanes 4e..4e+3 hold s = 0..3)
@@ -421,7 +429,8 @@ __device__ __forceinline__ void store_tile(double* __restrict__ out, const doubl
 }
 
 template <class C, int FLUX, bool STAGE>
-__global__ __launch_bounds__(DG_BS) void k_dg(const nz_problem P, const double* __restrict__ T,
+__global__ __launch_bounds__(DG_BS) __attribute__((amdgpu_waves_per_eu(Tile<C>::WPE)))
+void k_dg(const nz_problem P, const double* __restrict__ T,
                                            const double* __restrict__ Uin,
                                            const double* __restrict__ Un,
                                            double* __restrict__ Uout, double* __restrict__ dt,
@@ -530,14 +539,14 @@ __global__ __launch_bounds__(DG_BS) void k_dg(const nz_problem P, const double* 
     double wave = 0.0;
     __syncthreads();  // the staging rows are reused below
 
-    double* sTn = sW + 2 * NQF * QS;  // neighbour trace
-    double* sSp = sW + 3 * NQF * QS;  // [NQF][TE] signal speed
+    double* sTo = sW;                 // own trace, then weighted flux
+    double* sTn = sW + NQF * QS;      // neighbour trace
+    double* sSp = sW + 2 * NQF * QS;  // [NQF][TE] signal speed
     const int lift_off = fold ? P.off.lift : P.off.phi_face;
 
     // ---------------------------------------------------------------- faces
 #pragma unroll 1
     for (int f = 0; f < NF; ++f) {
-        double* sTo = sW + (f & 1) * NQF * QS;  // own trace, then weighted flux
         const int nb = sNb[el * NF + f], inf = sIn[el * NF + f];
         const bool has = live && nb >= 0;
         // the neighbour's column of component s, requested before the own
@@ -649,19 +658,19 @@ __global__ __launch_bounds__(DG_BS) void k_dg(const nz_problem P, const double* 
                 wave += smax * len;
             }
         }
-        // the next face writes the other own-trace buffer and the neighbour
-        // rows; the speeds are rewritten only after its first barrier
+        // the next face rewrites these rows after the reads above (one wave:
+        // LDS operations complete in program order)
     }
     __syncthreads();
 
     // --------------------------------------------------------------- volume
-    double* sFy = sW + 2 * L::QR * QS;  // eta-flux
+    double* sUq = sW;                   // point states, then xi-flux
+    double* sFy = sW + L::QR * QS;      // eta-flux
     // point-major volume tables: phi, dxi, deta, axi, aeta (N each) per point
     const double* vt = T + P.off.vol_t;
     const int vproj = fold ? 3 * N : N;
 #pragma unroll 1
     for (int q0 = 0; q0 < NQV; q0 += L::QR) {
-        double* sUq = sW + ((q0 / L::QR) & 1) * L::QR * QS;  // point states, then xi-flux
 #pragma unroll 1
         for (int qq = 0; qq < L::QR; ++qq) {
             if (q0 + qq < NQV) {
@@ -721,8 +730,7 @@ __global__ __launch_bounds__(DG_BS) void k_dg(const nz_problem P, const double* 
                 }
             }
         }
-        // the next round writes the other state buffer; the eta-flux rows are
-        // rewritten only after its first barrier
+        // the next round rewrites these rows after the reads above
     }
 
     // ------------------------------------------------------------ epilogue

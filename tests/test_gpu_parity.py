@@ -109,6 +109,24 @@ def test_steps_cfg1(scheme, steps, key):
     assert abs(res / float(g[f"{key}_res"]) - 1) <= 1e-12
 
 
+@pytest.mark.parametrize("flux", FLUXES)
+def test_steps_synth_p3_against_oracle(flux):
+    """The fused stage kernel at P3 on affine triangles (the instantiation
+    that marches the 1024x512 channel): two SSP-RK3 steps against the CPU
+    oracle, limiter on, with the limiter's counts."""
+    from oracle.oracle import CpuOracle
+
+    g = golden("synth_p3")
+    ops, flow = build_case("synth_p3")
+    flow = flow.replace(flux=flux)
+    bk = backend(ops, flow)
+    U, res = bk.run(g["U"], 2, "ssprk3")
+    Uo, reso = CpuOracle(ops, flow).run(g["U"], 2, "ssprk3", bk.dt_factor("ssprk3"))
+    print(flux, rel_err(U, Uo), res, reso, bk.limiter_activity())
+    assert rel_err(U, Uo) <= 1e-12
+    assert abs(res / reso - 1) <= 1e-12
+
+
 def test_steps_quad_p2_and_p0():
     g = golden("quad_q2_p2")
     ops, flow = build_case("quad_q2_p2", dense=False)
