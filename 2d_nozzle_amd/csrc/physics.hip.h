@@ -20,6 +20,46 @@ namespace nz {
 
 constexpr double kFloor = 1e-10;  // thermo.py:33
 
+// Reciprocal and square root of the flux formulas' operands, which the
+// floors keep positive and normal (densities and pressures >= 1e-10, sound
+// speeds, sums of square roots of densities).  The hardware estimates
+// (v_rcp_f64, v_rsq_f64) are refined by Newton-Raphson steps: for 1/x,
+// r <- r + r (1 - x r) twice; for sqrt(x), one coupled step on g ~ sqrt(x),
+// h ~ 1/(2 sqrt(x)) followed by two residual corrections g <- g + h (x - g^2).
+// Each lands within about an ulp of the correctly rounded result in 5 and
+// 9 instructions, against the IEEE sequences' 10 and 15, whose extra work
+// handles denormal, zero and infinite operands that cannot occur here.
+// NaN propagates.  The dual-number overloads differentiate 1/x and sqrt(x).
+__device__ __forceinline__ double recip(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+__device__ __forceinline__ double root(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    return fma(d, h, g);
+}
+
+template <int K>
+__device__ __forceinline__ Dual<K> recip(const Dual<K>& x) {
+    return 1.0 / x;
+}
+
+template <int K>
+__device__ __forceinline__ Dual<K> root(const Dual<K>& x) {
+    return sqrt(x);
+}
+
 template <class T>
 struct Prim {
     T r, u, v, p, H;
@@ -27,11 +67,11 @@ struct Prim {
 
 template <class T>
 __device__ __forceinline__ Prim<T> primitives(const T U[4], double g) {
-    // one correctly rounded reciprocal instead of three divisions; the
-    // results differ from the quotients by at most an ulp
+    // one reciprocal instead of three divisions; the results differ from
+    // the quotients by about an ulp
     Prim<T> w;
     w.r = fmax(U[0], kFloor);
-    const T ir = 1.0 / w.r;
+    const T ir = recip(w.r);
     w.u = U[1] * ir;
     w.v = U[2] * ir;
     w.p = fmax((g - 1.0) * (U[3] - 0.5 * w.r * (w.u * w.u + w.v * w.v)), kFloor);
@@ -64,18 +104,18 @@ __device__ __forceinline__ T flux_roe(const T UL[4], const T UR[4], const T& nx,
         Fc[k] = 0.5 * (FL[k] + FR[k]);
         d[k] = UR[k] - UL[k];
     }
-    const T sL = sqrt(L.r), sR = sqrt(R.r), iden = 1.0 / (sL + sR);
+    const T sL = root(L.r), sR = root(R.r), iden = recip(sL + sR);
     const T vx = (sL * L.u + sR * R.u) * iden;
     const T vy = (sL * L.v + sR * R.v) * iden;
     const T H = (sL * L.H + sR * R.H) * iden;
     const T q2 = vx * vx + vy * vy;
-    const T c = sqrt(fmax((g - 1.0) * (H - 0.5 * q2), kFloor));
-    const T ic = 1.0 / c;
+    const T c = root(fmax((g - 1.0) * (H - 0.5 * q2), kFloor));
+    const T ic = recip(c);
     const T vn = vx * nx + vy * ny;
     T l1 = fabs(vn + c), l2 = fabs(vn - c), l3 = fabs(vn);
     const T smax = fmax(l1, l2);
     const T eps = efix * c;
-    const T h = 0.5 / eps;
+    const T h = 0.5 * recip(eps);
     if (l1 < eps) l1 = (l1 * l1 + eps * eps) * h;
     if (l2 < eps) l2 = (l2 * l2 + eps * eps) * h;
     if (l3 < eps) l3 = (l3 * l3 + eps * eps) * h;
@@ -103,13 +143,13 @@ __device__ __forceinline__ T flux_hllc(const T UL[4], const T UR[4], const T& nx
                                        double g, double low_mach, T out[4]) {
     const Prim<T> L = primitives(UL, g), R = primitives(UR, g);
     const T vnL = L.u * nx + L.v * ny, vnR = R.u * nx + R.v * ny;
-    const T aL = sqrt(fmax(g * L.p / L.r, kFloor)), aR = sqrt(fmax(g * R.p / R.r, kFloor));
-    const T sL = sqrt(L.r), sR = sqrt(R.r), iden = 1.0 / (sL + sR);
+    const T aL = root(fmax(g * L.p / L.r, kFloor)), aR = root(fmax(g * R.p / R.r, kFloor));
+    const T sL = root(L.r), sR = root(R.r), iden = recip(sL + sR);
     const T vx = (sL * L.u + sR * R.u) * iden;
     const T vy = (sL * L.v + sR * R.v) * iden;
     const T H = (sL * L.H + sR * R.H) * iden;
     const T vnt = vx * nx + vy * ny;
-    const T at = sqrt(fmax((g - 1.0) * (H - 0.5 * (vx * vx + vy * vy)), kFloor));
+    const T at = root(fmax((g - 1.0) * (H - 0.5 * (vx * vx + vy * vy)), kFloor));
     T SL = fmin(vnL - aL, vnt - at);
     T SR = fmax(vnR + aR, vnt + at);
     if (low_mach > 0.0) {
@@ -155,7 +195,7 @@ __device__ __forceinline__ T flux_ausm(const T UL[4], const T UR[4], const T& nx
                                        double g, T out[4]) {
     const Prim<T> L = primitives(UL, g), R = primitives(UR, g);
     const T vnL = L.u * nx + L.v * ny, vnR = R.u * nx + R.v * ny;
-    const T aL = sqrt(g * L.p / L.r), aR = sqrt(g * R.p / R.r);
+    const T aL = root(g * L.p / L.r), aR = root(g * R.p / R.r);
     const T ah = 0.5 * (aL + aR), rh = 0.5 * (L.r + R.r);
     const T mL = vnL / ah, mR = vnR / ah;
     const T mb2 = 0.5 * (mL * mL + mR * mR);
