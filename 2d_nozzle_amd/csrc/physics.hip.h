@@ -62,7 +62,7 @@ __device__ __forceinline__ Dual<K> root(const Dual<K>& x) {
 
 template <class T>
 struct Prim {
-    T r, u, v, p, H;
+    T r, u, v, p, H, ir;  // ir = 1 / r
 };
 
 template <class T>
@@ -76,6 +76,7 @@ __device__ __forceinline__ Prim<T> primitives(const T U[4], double g) {
     w.v = U[2] * ir;
     w.p = fmax((g - 1.0) * (U[3] - 0.5 * w.r * (w.u * w.u + w.v * w.v)), kFloor);
     w.H = (U[3] + w.p) * ir;
+    w.ir = ir;
     return w;
 }
 
@@ -143,7 +144,7 @@ __device__ __forceinline__ T flux_hllc(const T UL[4], const T UR[4], const T& nx
                                        double g, double low_mach, T out[4]) {
     const Prim<T> L = primitives(UL, g), R = primitives(UR, g);
     const T vnL = L.u * nx + L.v * ny, vnR = R.u * nx + R.v * ny;
-    const T aL = root(fmax(g * L.p / L.r, kFloor)), aR = root(fmax(g * R.p / R.r, kFloor));
+    const T aL = root(fmax(g * L.p * L.ir, kFloor)), aR = root(fmax(g * R.p * R.ir, kFloor));
     const T sL = root(L.r), sR = root(R.r), iden = recip(sL + sR);
     const T vx = (sL * L.u + sR * R.u) * iden;
     const T vy = (sL * L.v + sR * R.v) * iden;
@@ -195,9 +196,10 @@ __device__ __forceinline__ T flux_ausm(const T UL[4], const T UR[4], const T& nx
                                        double g, T out[4]) {
     const Prim<T> L = primitives(UL, g), R = primitives(UR, g);
     const T vnL = L.u * nx + L.v * ny, vnR = R.u * nx + R.v * ny;
-    const T aL = root(g * L.p / L.r), aR = root(g * R.p / R.r);
+    const T aL = root(g * L.p * L.ir), aR = root(g * R.p * R.ir);
     const T ah = 0.5 * (aL + aR), rh = 0.5 * (L.r + R.r);
-    const T mL = vnL / ah, mR = vnR / ah;
+    const T iah = recip(ah);
+    const T mL = vnL * iah, mR = vnR * iah;
     const T mb2 = 0.5 * (mL * mL + mR * mR);
     // split Mach numbers M4+(mL), M4-(mR)
     T m4p, m4m;
@@ -205,7 +207,7 @@ __device__ __forceinline__ T flux_ausm(const T UL[4], const T UR[4], const T& nx
     else m4p = (0.25 * sq(mL + 1.0)) * (1.0 - 16.0 * 0.125 * (-0.25 * sq(mL - 1.0)));
     if (fabs(mR) >= 1.0) m4m = 0.5 * (mR - fabs(mR));
     else m4m = (-0.25 * sq(mR - 1.0)) * (1.0 + 16.0 * 0.125 * (0.25 * sq(mR + 1.0)));
-    const T uh = ah * (m4p + m4m) + 0.4 * fmax(1.0 - mb2, 0.0) * (L.p - R.p) / (rh * ah);
+    const T uh = ah * (m4p + m4m) + 0.4 * fmax(1.0 - mb2, 0.0) * (L.p - R.p) * recip(rh * ah);
     // pressure splittings P5+(mL), P5-(mR); the supersonic branch is (1 +- sign M)/2
     T pp, pm;
     if (fabs(mL) >= 1.0) pp = T(mL > 0.0 ? 1.0 : 0.0);
