@@ -45,6 +45,7 @@ class Problem(C.Structure):
         ("bisect_steps", C.c_int32), ("pad0", C.c_int32),
         ("wnorm", C.c_double * 4),
         ("tiles", C.c_void_p), ("n_tiles", C.c_int64),
+        ("vjp_tiles", C.c_void_p), ("n_vjp_tiles", C.c_int64),
     ]
 
 
@@ -64,6 +65,7 @@ SIGNATURES = {
     "nz_time_step": (C.c_int, [C.POINTER(Problem), P, C.c_double, P, P]),
     "nz_superbee_work_doubles": (C.c_int64, [C.POINTER(Problem)]),
     "nz_adjoint_work_doubles": (C.c_int64, [C.POINTER(Problem)]),
+    "nz_adjoint_tile_elements": (C.c_int64, [C.POINTER(Problem)]),
     "nz_residual_vjp": (C.c_int, [C.POINTER(Problem), P, P, P, P, P, P, P, P]),
     "nz_adjoint_rk4_stage": (C.c_int, [C.POINTER(Problem), P, P, P, P, P, P, P, C.c_int, P, P]),
     "nz_adjoint_defect": (C.c_int, [C.POINTER(Problem), P, P, P, P, P, P, P]),

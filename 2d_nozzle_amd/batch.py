@@ -411,19 +411,22 @@ def _design_coords(dsg: Design, area_ratio: float) -> np.ndarray:
 def batched_gradients(results: list, flow: FlowConditions, opts: SolverOptions, *,
                       objective: str = "thrust", tolerance: float = 1e-8,
                       max_iterations: int = 100_000, check_interval: int = 100,
-                      fd_step: float = 1e-6, group_size: int = 8, device=None) -> list:
+                      fd_step: float = 1e-6, group_size: int | None = None,
+                      device=None) -> list:
     """d objective / d area ratio for every converged design of a batch, from
     steady adjoints marched together (``src/sensitivity.py:289-460``).
 
-    Designs are grouped in families as in the forward batch, split into groups
-    of similar forward iteration counts, and the groups march concurrently,
-    each on its own stream.  Each family's adjoint march is
-    the lone one (classical RK4 in pseudo-time with the frozen local step of
-    the converged state): element-local updates on a block-diagonal mesh, so
-    each design's iterates are those it has alone.  Every ``check_interval``
-    steps each design's defect ``rms(vjp(lam) - seed) / rms(seed)`` is tested
-    on its own segment; a design that passes is copied out at that step.  The
-    geometry cotangents of a whole family come from one VJP launch and are
+    Designs are grouped in families as in the forward batch (optionally split
+    into groups of ``group_size`` with similar forward iteration counts), and
+    the groups march concurrently, each on its own stream.  Each group's
+    adjoint march is the lone one (classical RK4 in pseudo-time with the
+    frozen local step of the converged state): element-local updates on a
+    block-diagonal mesh, so each design's iterates are those it has alone.
+    Every ``check_interval`` steps each design's defect
+    ``rms(vjp(lam) - seed) / rms(seed)`` is tested on its own segment; a
+    design that passes is copied out at that step and its tiles leave the
+    adjoint kernels' tile table, so it costs nothing afterwards.  The
+    geometry cotangents of a whole group come from one VJP launch and are
     split back per design for the host chain to node coordinates and the
     design variable.
     """
@@ -446,7 +449,23 @@ def batched_gradients(results: list, flow: FlowConditions, opts: SolverOptions, 
     parts = []
     for idx in groups.values():
         idx = sorted(idx, key=lambda g: results[g].iterations)
-        parts += [idx[k:k + group_size] for k in range(0, len(idx), group_size)]
+        step = group_size or len(idx)
+        parts += [idx[k:k + step] for k in range(0, len(idx), step)]
+
+    def set_vjp_tiles(f):
+        # the live designs' elements in tiles of the adjoint kernels' width;
+        # a captured chunk used the old table
+        te, tiles = f.te, []
+        for j, d in enumerate(f.done):
+            if not d:
+                a, b = int(f.eoff[j]), int(f.eoff[j + 1])
+                for e0 in range(a, b, te):
+                    tiles += [e0, min(e0 + te, b)]
+        f.vt = torch.tensor(tiles if tiles else [0, 0], dtype=torch.int64, device=f.bk.device)
+        P = f.bk.P
+        P.vjp_tiles = C.c_void_p(f.vt.data_ptr())
+        P.n_vjp_tiles = len(tiles) // 2
+        f.graph = None
 
     fams = []
     for idx in parts:
@@ -459,7 +478,8 @@ def batched_gradients(results: list, flow: FlowConditions, opts: SolverOptions, 
                    for g in idx]
         f.u = torch.from_numpy(np.concatenate([results[g].U for g in idx])).to(dev)
         f.s = torch.from_numpy(np.concatenate([p[1] for p in f.parts])).to(dev)
-        eoff = f.cat.elem_offset
+        f.eoff = eoff = f.cat.elem_offset
+        f.te = int(bk.lib.nz_adjoint_tile_elements(bk._pp))
         f.sl = [slice(int(eoff[j]), int(eoff[j + 1])) for j in range(len(idx))]
         # per-design sums over contiguous element segments as a padded gather
         # (graph-capturable, fixed order)
@@ -487,6 +507,7 @@ def batched_gradients(results: list, flow: FlowConditions, opts: SolverOptions, 
         f.done = [bool(v == 0.0) for v in f.norm0]
         f.its = [0] * len(idx)
         f.rels = [0.0 if d else float("nan") for d in f.done]
+        set_vjp_tiles(f)
         fams.append(f)
 
     def launch(f, n):
@@ -534,6 +555,7 @@ def batched_gradients(results: list, flow: FlowConditions, opts: SolverOptions, 
             f.stream.synchronize()
             res = f.res_dev.cpu().numpy()
             lam = f.a["lam"][0]
+            stopped = False
             for j in range(len(f.idx)):
                 if f.done[j]:
                     continue
@@ -541,14 +563,18 @@ def batched_gradients(results: list, flow: FlowConditions, opts: SolverOptions, 
                 if not np.isfinite(r) or r <= tolerance * f.norm0[j]:
                     f.lam_out[f.sl[j]] = lam[f.sl[j]]
                     f.its[j], f.rels[j], f.done[j] = it, r / f.norm0[j], True
+                    stopped = True
+            if stopped and not all(f.done):
+                set_vjp_tiles(f)
     for f in fams:
         lam = f.a["lam"][0]
         for j in range(len(f.idx)):
             if not f.done[j]:
                 f.lam_out[f.sl[j]] = lam[f.sl[j]]
                 f.its[j] = it
-        # geometry cotangents of the whole family, split per design
+        # geometry cotangents of the whole group, split per design
         bk, cat = f.bk, f.cat
+        bk.P.vjp_tiles, bk.P.n_vjp_tiles = C.c_void_p(0), 0
         rd, nqf = cat.ref, cat.ref.n_qface
         dev = bk.device
         gm = torch.zeros((cat.n_elem, rd.n_qvol, 5), dtype=torch.float64, device=dev)

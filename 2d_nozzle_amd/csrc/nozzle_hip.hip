@@ -1074,8 +1074,15 @@ __global__ __launch_bounds__(BS) void k_vjp(const nz_problem P, const double* __
     const int t = threadIdx.x;
     const int el = t % TE;
     const int slot = wave_slot() / (TE / 64);
-    const int64_t e0 = static_cast<int64_t>(blockIdx.x) * TE;
-    const int ne = static_cast<int>(P.n_elem - e0 < TE ? P.n_elem - e0 : TE);
+    int64_t e0;
+    int ne;
+    if (P.vjp_tiles) {  // a subset of the mesh: the live designs of a batch
+        e0 = P.vjp_tiles[2 * blockIdx.x];
+        ne = static_cast<int>(P.vjp_tiles[2 * blockIdx.x + 1] - e0);
+    } else {
+        e0 = static_cast<int64_t>(blockIdx.x) * TE;
+        ne = static_cast<int>(P.n_elem - e0 < TE ? P.n_elem - e0 : TE);
+    }
     const bool live = el < ne;
     const int64_t e = e0 + el;
 
@@ -1524,6 +1531,7 @@ struct Launch {
         if (nbq > 0)
             hipLaunchKernelGGL((nz::k_bvjp<C>), dim3(static_cast<unsigned>((nbq + nz::BS - 1) / nz::BS)),
                                dim3(nz::BS), 0, st, P, P.tables, U, L, bv, gedge, gpb);
+        if (nblk <= 0) return;  // an empty tile table: nothing to march
         const dim3 g(static_cast<unsigned>(nblk));
 #define NZ_VJP(FL)                                                                             \
     hipLaunchKernelGGL((nz::k_vjp<C, FL>), g, dim3(nz::BS), 0, st, P, P.tables, U, L, bv, out, \
@@ -1577,10 +1585,15 @@ int resolve(const nz_problem* p, Ops* ops) {
         return fail(NZ_E_ARG, "boundary edges need bedge and the bflux workspace");
     if (p->tiles && (p->n_tiles < 0 || p->n_tiles > (int64_t{1} << 31)))
         return fail(NZ_E_ARG, "n_tiles out of range");
+    if (p->vjp_tiles && (p->n_vjp_tiles < 0 || p->n_vjp_tiles > (int64_t{1} << 31)))
+        return fail(NZ_E_ARG, "n_vjp_tiles out of range");
     return NZ_OK;
 }
 
 int64_t n_blocks(const nz_problem* p, const Ops& o) { return (p->n_elem + o.te - 1) / o.te; }
+int64_t n_blocks_vjp(const nz_problem* p, const Ops& o) {
+    return p->vjp_tiles ? p->n_vjp_tiles : n_blocks(p, o);
+}
 int64_t n_blocks_dg(const nz_problem* p) {
     return p->tiles ? p->n_tiles : (p->n_elem + nz::TE - 1) / nz::TE;
 }
@@ -1595,8 +1608,15 @@ const char* nz_last_error(void) { return g_err; }
 int64_t nz_scratch_doubles(const nz_problem* p) {
     Ops o;
     if (resolve(p, &o) != NZ_OK) return -1;
-    const int64_t a = n_blocks(p, o) > n_blocks_dg(p) ? n_blocks(p, o) : n_blocks_dg(p);
+    int64_t a = n_blocks(p, o) > n_blocks_dg(p) ? n_blocks(p, o) : n_blocks_dg(p);
+    if (n_blocks_vjp(p, o) > a) a = n_blocks_vjp(p, o);
     return a > 1024 ? a : 1024;
+}
+
+int64_t nz_adjoint_tile_elements(const nz_problem* p) {
+    Ops o;
+    if (resolve(p, &o) != NZ_OK) return -1;
+    return o.te;
 }
 
 int nz_residual(const nz_problem* p, const double* U, double* R, double* wave, void* stream) {
@@ -1707,7 +1727,7 @@ int nz_residual_vjp(const nz_problem* p, const double* U, const double* lam, dou
     if (rc) return rc;
     if (!U || !lam || !dU || !work) return fail(NZ_E_ARG, "null array");
     o.vjp(*p, U, lam, dU, gmet, gedge, gpb, work, nz::kAdjVjp, nullptr, nullptr, nullptr, nullptr,
-          nullptr, 0, n_blocks(p, o), static_cast<hipStream_t>(stream));
+          nullptr, 0, n_blocks_vjp(p, o), static_cast<hipStream_t>(stream));
     return check_launch("nz_residual_vjp");
 }
 
@@ -1723,7 +1743,7 @@ int nz_adjoint_rk4_stage(const nz_problem* p, const double* U, const double* lam
     if (lam_next == lam_in || lam_next == lam_base)
         return fail(NZ_E_ARG, "lam_next must not alias lam_in or lam_base");
     o.vjp(*p, U, lam_in, nullptr, nullptr, nullptr, nullptr, work, nz::kAdjRk, seed, lam_base, dt,
-          acc, lam_next, stage, n_blocks(p, o), static_cast<hipStream_t>(stream));
+          acc, lam_next, stage, n_blocks_vjp(p, o), static_cast<hipStream_t>(stream));
     return check_launch("nz_adjoint_rk4_stage");
 }
 
@@ -1734,7 +1754,7 @@ int nz_adjoint_defect(const nz_problem* p, const double* U, const double* lam, c
     if (rc) return rc;
     if (!U || !lam || !seed || !work || !scratch || !out) return fail(NZ_E_ARG, "null array");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const int64_t nb = n_blocks(p, o);
+    const int64_t nb = n_blocks_vjp(p, o);
     o.vjp(*p, U, lam, scratch, nullptr, nullptr, nullptr, work, nz::kAdjDefect, seed, nullptr,
           nullptr, nullptr, nullptr, 0, nb, st);
     hipLaunchKernelGGL(k_norm_finish, dim3(1), dim3(FIN_BS), 0, st, scratch, nb,

@@ -87,11 +87,26 @@ typedef struct {
      * meshes (designs) in one launch and to leave converged ones out. */
     const int64_t *tiles;
     int64_t n_tiles;
+    /* optional tile table for the adjoint kernels of nz_residual_vjp,
+     * nz_adjoint_rk4_stage and nz_adjoint_defect, one workgroup per entry:
+     * entry k covers elements [vjp_tiles[2k], vjp_tiles[2k+1]), at most
+     * nz_adjoint_tile_elements(p) of them.  Elements outside every entry are
+     * not touched (their outputs keep what they held); nz_adjoint_defect
+     * sums over the listed elements only.  The boundary pre-pass still
+     * covers every boundary edge.  NULL: consecutive tiles over the whole
+     * mesh.  Used to leave converged designs of a batch out of the adjoint
+     * march. */
+    const int64_t *vjp_tiles;
+    int64_t n_vjp_tiles;
 } nz_problem;
 
-/* Number of scratch doubles nz_* calls need for norm partials, and the
- * number of elements one workgroup handles. */
+/* Number of scratch doubles nz_* calls need for norm partials. */
 int64_t nz_scratch_doubles(const nz_problem *p);
+
+/* Elements per workgroup of the adjoint kernels: the widest entry a
+ * vjp_tiles table may hold.  No counterpart in the reference (its adjoint is
+ * a JAX vjp over the whole mesh, src/sensitivity.py:317). */
+int64_t nz_adjoint_tile_elements(const nz_problem *p);
 
 /* R(U) and the wave-speed sum, unfolded.
  * Replaces edge_pass + element_pass(apply_mass=False)
