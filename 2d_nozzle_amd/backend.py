@@ -520,31 +520,57 @@ class HipBackend:
             return self._out(dU, npy)
         return (self._out(dU, npy), self._out(gm, npy), self._out(ge, npy), float(gp.sum().item()))
 
-    def adjoint_solve(self, U, seed, *, tolerance: float = 1e-8, max_iterations: int = 100_000,
-                      check_interval: int = 100, verbose: bool = False):
-        """Solve ``(dR/dU)^T lam = seed`` by the reference's pseudo-time march.
-
-        Classical RK4 with the frozen local step of the converged state,
-        sized with the ``ssprk3`` calibration and without the positivity
-        factor; convergence when ``rms(vjp(lam) - seed) <= tolerance *
-        rms(seed)``, tested every ``check_interval`` steps
-        (``src/sensitivity.py:289-383``).  Returns
-        ``(lam, relative_residual, iterations, converged)``.
-        """
-        if max_iterations < 1 or check_interval < 1:
-            raise ValueError("max_iterations and check_interval must be at least 1")
+    def _frozen_step(self, u):
+        """The adjoint's local pseudo-time step: the forward step of the
+        converged state with the ``ssprk3`` calibration and without the
+        positivity factor (``src/sensitivity.py:330-343``), into the adjoint
+        buffers."""
         torch = self._torch
-        u, npy = self._in(U)
-        s, _ = self._in(seed)
         a = self._adj_buffers()
         st = self._stream()
-        # frozen step from the converged state
         Fr = torch.empty_like(u)
         wave = torch.empty(self.E, dtype=torch.float64, device=self.device)
         self._chk(self.lib.nz_rate(self._pp, self._p(u), self._p(Fr), self._p(wave), st), "nz_rate")
         fac = step_coefficient(self.ops.ref.order, self.opts.cfl, "ssprk3")
         self._chk(self.lib.nz_time_step(self._pp, self._p(wave), fac, self._p(a["dt"]), st),
                   "nz_time_step")
+        return a
+
+    def adjoint_solve(self, U, seed, *, tolerance: float = 1e-8, max_iterations: int = 100_000,
+                      check_interval: int = 100, verbose: bool = False, method: str = "rk4",
+                      restart: int = 60):
+        """Solve ``(dR/dU)^T lam = seed``; converged when
+        ``rms(vjp(lam) - seed) <= tolerance * rms(seed)``.
+
+        ``method="rk4"`` is the reference's pseudo-time march
+        (``src/sensitivity.py:289-383``): classical RK4 with the frozen local
+        step of the converged state, sized with the ``ssprk3`` calibration and
+        without the positivity factor, tested every ``check_interval`` steps;
+        ``iterations`` counts steps (four VJPs each).
+
+        ``method="gmres"`` solves the same linear system by restarted GMRES
+        (``restart`` vectors), right-preconditioned with that same local step
+        times the inverse mass matrix, so ``D = dt M^-1`` is the operator the
+        march integrates; GMRES minimises the true residual, whose norm is
+        tested after every VJP and recomputed at each restart.
+        ``iterations`` counts VJPs.  Same fixed point, same test: the
+        gradients agree with the march's to the tolerance.
+
+        Returns ``(lam, relative_residual, iterations, converged)``.
+        """
+        if max_iterations < 1 or check_interval < 1:
+            raise ValueError("max_iterations and check_interval must be at least 1")
+        if method not in ("rk4", "gmres"):
+            raise ValueError(f"unknown adjoint method {method!r}; choose rk4 or gmres")
+        torch = self._torch
+        u, npy = self._in(U)
+        s, _ = self._in(seed)
+        a = self._frozen_step(u)
+        st = self._stream()
+        if method == "gmres":
+            lam, rel, its, ok = self._adjoint_gmres(u, s, a, tolerance, max_iterations, restart,
+                                                    verbose)
+            return self._out(lam, npy), rel, its, ok
         lam, tmp, nxt = a["lam"]
         lam.zero_()
         norm0 = float(torch.sqrt(torch.mean(s * s)).item())
@@ -579,3 +605,77 @@ class HipBackend:
                 break
         a["lam"] = [lam, tmp, nxt]
         return self._out(lam.clone(), npy), res / norm0, it, converged
+
+    def _adjoint_gmres(self, u, s, a, tolerance, max_iterations, restart, verbose):
+        """Restarted GMRES on ``(dR/dU)^T D y = seed``, ``lam = D y``, with
+        ``D = dt M^-1`` (the frozen local step), classical Gram-Schmidt with
+        one reorthogonalisation and Givens rotations on the host."""
+        torch = self._torch
+        st = self._stream()
+        work, pu = self._p(a["work"]), self._p(u)
+        dt = a["dt"].view(-1, 1, 1)
+        m = max(int(restart), 1)
+
+        def jt(v):  # (dR/dU)^T v
+            out = torch.empty_like(v)
+            self._chk(self.lib.nz_residual_vjp(self._pp, pu, self._p(v), self._p(out), None, None,
+                                               None, work, st), "nz_residual_vjp")
+            return out
+
+        def prec(v):  # dt M^-1 v
+            out = torch.empty_like(v)
+            self._chk(self.lib.nz_inverse_mass(self._pp, self._p(v), self._p(out), st),
+                      "nz_inverse_mass")
+            return out.mul_(dt)
+
+        lam = torch.zeros_like(s)
+        bnorm = float(torch.linalg.vector_norm(s).item())
+        if bnorm == 0.0:
+            return lam, 0.0, 0, True
+        V = torch.empty((m + 1, s.numel()), dtype=torch.float64, device=s.device)
+        r, its, rel = s.clone(), 0, 1.0
+        while True:
+            beta = float(torch.linalg.vector_norm(r).item())
+            rel = beta / bnorm
+            if verbose:
+                print(f"    adjoint gmres {its:>7d}   {rel:.2e} relative (true residual)")
+            if not math.isfinite(rel) or rel <= tolerance or its >= max_iterations:
+                break
+            V[0] = r.view(-1) / beta
+            H = np.zeros((m + 1, m))
+            g = np.zeros(m + 1)
+            g[0] = beta
+            cs, sn = np.zeros(m), np.zeros(m)
+            k = 0
+            for j in range(m):
+                w = jt(prec(V[j].view_as(s))).view(-1)
+                its += 1
+                h = V[:j + 1] @ w
+                w -= V[:j + 1].T @ h
+                h2 = V[:j + 1] @ w
+                w -= V[:j + 1].T @ h2
+                hj = (h + h2).cpu().numpy()
+                hn = float(torch.linalg.vector_norm(w).item())
+                col = np.append(hj, hn)
+                for i in range(j):  # earlier rotations
+                    t = cs[i] * col[i] + sn[i] * col[i + 1]
+                    col[i + 1] = -sn[i] * col[i] + cs[i] * col[i + 1]
+                    col[i] = t
+                den = math.hypot(col[j], col[j + 1])
+                cs[j], sn[j] = (1.0, 0.0) if den == 0.0 else (col[j] / den, col[j + 1] / den)
+                col[j], col[j + 1] = den, 0.0
+                g[j + 1] = -sn[j] * g[j]
+                g[j] = cs[j] * g[j]
+                H[:j + 2, j] = col
+                k = j + 1
+                if hn > 0.0:
+                    V[j + 1] = w / hn
+                if hn == 0.0 or abs(g[j + 1]) <= tolerance * bnorm or its >= max_iterations:
+                    break
+            y = np.zeros(k)
+            for i in range(k - 1, -1, -1):  # back substitution
+                y[i] = (g[i] - H[i, i + 1:k] @ y[i + 1:k]) / H[i, i]
+            yt = torch.from_numpy(y).to(s.device)
+            lam += prec((V[:k].T @ yt).view_as(s))
+            r = s - jt(lam)
+        return lam, rel, its, bool(math.isfinite(rel) and rel <= tolerance)

@@ -32,6 +32,15 @@ struct Dual {
     }
 };
 
+// x with tangent direction k among the K carried (none if k is outside [0, K))
+template <int K>
+__device__ __forceinline__ Dual<K> seeded(double x, int k) {
+    Dual<K> r(x);
+#pragma unroll
+    for (int j = 0; j < K; ++j) r.d[j] = j == k ? 1.0 : 0.0;
+    return r;
+}
+
 template <int K>
 __device__ __forceinline__ Dual<K> operator+(const Dual<K>& a, const Dual<K>& b) {
     Dual<K> r;

@@ -1050,7 +1050,7 @@ __global__ __launch_bounds__(BS) void k_bvjp(const nz_problem P, const double* _
     if (gpb) gpb[b * C::NQF + q] = v[6];
 }
 
-template <class C, int FLUX>
+template <class C, int FLUX, bool GEO>
 __global__ __launch_bounds__(BS) void k_vjp(const nz_problem P, const double* __restrict__ T,
                                             const double* __restrict__ U,
                                             const double* __restrict__ Lin,
@@ -1064,6 +1064,9 @@ __global__ __launch_bounds__(BS) void k_vjp(const nz_problem P, const double* __
                                             int rk_stage) {
     constexpr int N = C::N, NQV = C::NQV, NQF = C::NQF, NF = C::NF, NS = C::NS;
     constexpr int TE = C::TE, TP = C::TP, SLOTS = C::SLOTS;
+    // tangent directions per dual pass: all of them (the owner's state, and
+    // the normal when geometry cotangents are asked for) in one pass
+    constexpr int KB = GEO ? 6 : 4;
     __shared__ double sU[NS * TP];
     __shared__ double sL[NS * TP];
     __shared__ double sW[C::AWORK];
@@ -1108,7 +1111,9 @@ __global__ __launch_bounds__(BS) void k_vjp(const nz_problem P, const double* __
     double* sTn = sW;                     // [NQF][4][TP] neighbour U trace
     double* sLn = sW + NQF * 4 * TP;      // [NQF][4][TP] neighbour lambda trace
     double* sV = sW + 2 * NQF * 4 * TP;   // [NQF][4][TP] a . dFhat/dU_own
+#pragma unroll 1
     for (int f = 0; f < NF; ++f) {
+#pragma unroll 1
         for (int s = slot; s < 4; s += SLOTS) {
             double tu[NQF], tl[NQF];
 #pragma unroll
@@ -1138,6 +1143,7 @@ __global__ __launch_bounds__(BS) void k_vjp(const nz_problem P, const double* __
             }
         }
         __syncthreads();
+#pragma unroll 1
         for (int q = slot; q < NQF; q += SLOTS) {
             if (live) {
                 const int nb = P.fe_nbr[e * NF + f];
@@ -1158,37 +1164,50 @@ __global__ __launch_bounds__(BS) void k_vjp(const nz_problem P, const double* __
                     const double* eg = C::STR ? P.edge_geo + k * 3 : P.edge_geo + (k * NQF + q) * 3;
                     const double wq = T[P.off.w_face + q];
                     const double jw = eg[2] * wq;
-                    using D = Dual<6>;
-                    D ul[4], ur[4];
-                    double dl[4];  // Lambda_owner - Lambda_neighbour
+                    // Tangents: the owner's state, and the normal when its
+                    // cotangent is asked for, KB directions per pass (each
+                    // pass re-evaluates the flux with a KB-wide dual).
+                    constexpr int KD = GEO ? 6 : 4;
+                    using D = Dual<KB>;
+                    double dl[4], un[4];  // dl = Lambda_owner - Lambda_neighbour
                     for (int s = 0; s < 4; ++s) {
-                        const double un = sTn[(q * 4 + s) * TP + el];
+                        un[s] = sTn[(q * 4 + s) * TP + el];
                         const double ln = sLn[(q * 4 + s) * TP + el];
-                        if (side == 0) {
-                            ul[s] = D::seed(uo[s], s);
-                            ur[s] = D(un);
-                            dl[s] = lo[s] - ln;
-                        } else {
-                            ul[s] = D(un);
-                            ur[s] = D::seed(uo[s], s);
-                            dl[s] = ln - lo[s];
-                        }
+                        dl[s] = side == 0 ? lo[s] - ln : ln - lo[s];
                     }
-                    const D nx = D::seed(eg[0], 4), ny = D::seed(eg[1], 5);
-                    D fx[4];
-                    interface_flux<FLUX>(P, ul, ur, nx, ny, fx);
-                    double g[6] = {0, 0, 0, 0, 0, 0}, gj = 0.0;
-                    for (int s = 0; s < 4; ++s) {
-                        const double a = jw * dl[s];
-                        for (int d = 0; d < 6; ++d) g[d] += a * fx[s].d[d];
-                        gj += dl[s] * fx[s].v;
+                    double g[KD], gj = 0.0;
+#pragma unroll
+                    for (int d0 = 0; d0 < KD; d0 += KB) {
+                        D ul[4], ur[4], fx[4];
+#pragma unroll
+                        for (int s = 0; s < 4; ++s) {
+                            const D own = seeded<KB>(uo[s], s - d0), nbr(un[s]);
+                            ul[s] = side == 0 ? own : nbr;
+                            ur[s] = side == 0 ? nbr : own;
+                        }
+                        if constexpr (GEO) {
+                            const D nx = seeded<KB>(eg[0], 4 - d0), ny = seeded<KB>(eg[1], 5 - d0);
+                            interface_flux<FLUX>(P, ul, ur, nx, ny, fx);
+                        } else {
+                            interface_flux<FLUX>(P, ul, ur, eg[0], eg[1], fx);
+                        }
+#pragma unroll
+                        for (int j = 0; j < KB; ++j) {
+                            double a = 0.0;
+                            for (int s = 0; s < 4; ++s) a += (jw * dl[s]) * fx[s].d[j];
+                            g[d0 + j] = a;
+                        }
+                        if (d0 == 0)
+                            for (int s = 0; s < 4; ++s) gj += dl[s] * fx[s].v;
                     }
                     for (int d = 0; d < 4; ++d) v[d] = g[d];
-                    if (gedge && side == 0) {
-                        double* ge = gedge + (k * NQF + q) * 3;
-                        ge[0] = g[4];
-                        ge[1] = g[5];
-                        ge[2] = gj * wq;
+                    if constexpr (GEO) {
+                        if (gedge && side == 0) {
+                            double* ge = gedge + (k * NQF + q) * 3;
+                            ge[0] = g[4];
+                            ge[1] = g[5];
+                            ge[2] = gj * wq;
+                        }
                     }
                 } else {
                     const double* b = bv + ((k - P.n_int) * NQF + q) * 4;
@@ -1216,6 +1235,7 @@ __global__ __launch_bounds__(BS) void k_vjp(const nz_problem P, const double* __
 
     // --------------------------------------------------------------- volume
     double* sGq = sW;  // [QR][4][TP] cotangent on the point state
+#pragma unroll 1
     for (int q0 = 0; q0 < NQV; q0 += C::QR) {
         {
             const int qq = slot, q = q0 + qq;
@@ -1246,26 +1266,41 @@ __global__ __launch_bounds__(BS) void k_vjp(const nz_problem P, const double* __
                     ey = gq[3];
                     wd = gq[4];
                 }
-                using D = Dual<4>;
-                D ud[4];
-                for (int s = 0; s < 4; ++s) ud[s] = D::seed(uq[s], s);
-                const Prim<D> w = primitives(ud, P.gamma);
-                const D F[4] = {w.r * w.u, w.r * w.u * w.u + w.p, w.r * w.u * w.v, w.r * w.u * w.H};
-                const D G[4] = {w.r * w.v, w.r * w.u * w.v, w.r * w.v * w.v + w.p, w.r * w.v * w.H};
                 double g[4] = {0, 0, 0, 0};
                 double gm[5] = {0, 0, 0, 0, 0};
+                double bx[4], by[4];
                 for (int s = 0; s < 4; ++s) {
-                    const double bx = wd * (ax[s] * xx + ay[s] * ex);
-                    const double by = wd * (ax[s] * xy + ay[s] * ey);
-                    for (int d = 0; d < 4; ++d) g[d] -= bx * F[s].d[d] + by * G[s].d[d];
-                    gm[0] -= wd * ax[s] * F[s].v;
-                    gm[1] -= wd * ax[s] * G[s].v;
-                    gm[2] -= wd * ay[s] * F[s].v;
-                    gm[3] -= wd * ay[s] * G[s].v;
-                    gm[4] -= ax[s] * (xx * F[s].v + xy * G[s].v) + ay[s] * (ex * F[s].v + ey * G[s].v);
+                    bx[s] = wd * (ax[s] * xx + ay[s] * ex);
+                    by[s] = wd * (ax[s] * xy + ay[s] * ey);
+                }
+                using D = Dual<KB>;
+                double Fv[4], Gv[4];
+#pragma unroll
+                for (int d0 = 0; d0 < 4; d0 += KB) {  // KB tangent directions per pass
+                    D ud[4];
+                    for (int s = 0; s < 4; ++s) ud[s] = seeded<KB>(uq[s], s - d0);
+                    const Prim<D> w = primitives(ud, P.gamma);
+                    const D F[4] = {w.r * w.u, w.r * w.u * w.u + w.p, w.r * w.u * w.v, w.r * w.u * w.H};
+                    const D G[4] = {w.r * w.v, w.r * w.u * w.v, w.r * w.v * w.v + w.p, w.r * w.v * w.H};
+                    for (int s = 0; s < 4; ++s)
+                        for (int j = 0; j < KB; ++j) g[d0 + j] -= bx[s] * F[s].d[j] + by[s] * G[s].d[j];
+                    if (d0 == 0)
+                        for (int s = 0; s < 4; ++s) {
+                            Fv[s] = F[s].v;
+                            Gv[s] = G[s].v;
+                        }
+                }
+                for (int s = 0; s < 4; ++s) {
+                    if constexpr (GEO) {
+                        gm[0] -= wd * ax[s] * Fv[s];
+                        gm[1] -= wd * ax[s] * Gv[s];
+                        gm[2] -= wd * ay[s] * Fv[s];
+                        gm[3] -= wd * ay[s] * Gv[s];
+                        gm[4] -= ax[s] * (xx * Fv[s] + xy * Gv[s]) + ay[s] * (ex * Fv[s] + ey * Gv[s]);
+                    }
                 }
                 for (int d = 0; d < 4; ++d) sGq[(qq * 4 + d) * TP + el] = g[d];
-                if (gmet)
+                if (GEO && gmet)
                     for (int d = 0; d < 5; ++d) gmet[(e * NQV + q) * 5 + d] = gm[d];
             }
         }
@@ -1533,9 +1568,16 @@ struct Launch {
                                dim3(nz::BS), 0, st, P, P.tables, U, L, bv, gedge, gpb);
         if (nblk <= 0) return;  // an empty tile table: nothing to march
         const dim3 g(static_cast<unsigned>(nblk));
-#define NZ_VJP(FL)                                                                             \
-    hipLaunchKernelGGL((nz::k_vjp<C, FL>), g, dim3(nz::BS), 0, st, P, P.tables, U, L, bv, out, \
-                       gmet, gedge, mode, seed, Lbase, dt, acc, Lnext, stage)
+        const bool geo = gmet || gedge;  // geometry cotangents: the normal is differentiated too
+#define NZ_VJP(FL)                                                                                \
+    do {                                                                                          \
+        if (geo)                                                                                  \
+            hipLaunchKernelGGL((nz::k_vjp<C, FL, true>), g, dim3(nz::BS), 0, st, P, P.tables, U,  \
+                               L, bv, out, gmet, gedge, mode, seed, Lbase, dt, acc, Lnext, stage); \
+        else                                                                                      \
+            hipLaunchKernelGGL((nz::k_vjp<C, FL, false>), g, dim3(nz::BS), 0, st, P, P.tables, U, \
+                               L, bv, out, gmet, gedge, mode, seed, Lbase, dt, acc, Lnext, stage); \
+    } while (0)
         if (P.flux_id == 1) NZ_VJP(1);
         else if (P.flux_id == 2) NZ_VJP(2);
         else NZ_VJP(0);

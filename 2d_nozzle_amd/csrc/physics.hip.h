@@ -9,7 +9,9 @@
 //
 // Every function is a template over the scalar type: `double` in the
 // forward kernels, forward-mode `Dual<K>` in the adjoint kernels, so the
-// flux Jacobians are those of exactly the arithmetic evaluated here.
+// flux Jacobians are those of exactly the arithmetic evaluated here.  The
+// interface fluxes take the normal as a type of its own: `double` when only
+// the state is differentiated, a dual when the normal is as well.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -80,8 +82,8 @@ __device__ __forceinline__ Prim<T> primitives(const T U[4], double g) {
     return w;
 }
 
-template <class T, class S>
-__device__ __forceinline__ void normal_flux(const Prim<T>& w, const S& nx, const S& ny, T f[4]) {
+template <class T, class N_>
+__device__ __forceinline__ void normal_flux(const Prim<T>& w, const N_& nx, const N_& ny, T f[4]) {
     const T vn = w.u * nx + w.v * ny;
     f[0] = w.r * vn;
     f[1] = w.r * w.u * vn + w.p * nx;
@@ -90,8 +92,8 @@ __device__ __forceinline__ void normal_flux(const Prim<T>& w, const S& nx, const
 }
 
 // Roe with the Harten-Hyman entropy fix (fluxes.py:38-103).
-template <class T>
-__device__ __forceinline__ T flux_roe(const T UL[4], const T UR[4], const T& nx, const T& ny,
+template <class T, class N_>
+__device__ __forceinline__ T flux_roe(const T UL[4], const T UR[4], const N_& nx, const N_& ny,
                                       double g, double efix, T out[4]) {
     // Ordered so that few values are live at once: the central flux and the
     // jumps are formed first and the one-sided states die with the Roe means.
@@ -139,8 +141,8 @@ __device__ __forceinline__ T away_from_zero(const T& x) {
 }
 
 // HLLC with Batten's wave speeds and the optional low-Mach scaling (fluxes.py:106-255).
-template <class T>
-__device__ __forceinline__ T flux_hllc(const T UL[4], const T UR[4], const T& nx, const T& ny,
+template <class T, class N_>
+__device__ __forceinline__ T flux_hllc(const T UL[4], const T UR[4], const N_& nx, const N_& ny,
                                        double g, double low_mach, T out[4]) {
     const Prim<T> L = primitives(UL, g), R = primitives(UR, g);
     const T vnL = L.u * nx + L.v * ny, vnR = R.u * nx + R.v * ny;
@@ -191,8 +193,8 @@ __device__ __forceinline__ T sq(const T& x) {
     return x * x;
 }
 
-template <class T>
-__device__ __forceinline__ T flux_ausm(const T UL[4], const T UR[4], const T& nx, const T& ny,
+template <class T, class N_>
+__device__ __forceinline__ T flux_ausm(const T UL[4], const T UR[4], const N_& nx, const N_& ny,
                                        double g, T out[4]) {
     const Prim<T> L = primitives(UL, g), R = primitives(UR, g);
     const T vnL = L.u * nx + L.v * ny, vnR = R.u * nx + R.v * ny;
@@ -314,9 +316,9 @@ __device__ __forceinline__ T flux_outflow(const T U[4], const T& nx, const T& ny
 }
 
 // FLUX: 0 roe, 1 hllc, 2 ausmup (src/backends/numba_backend.py:59-67)
-template <int FLUX, class T>
+template <int FLUX, class T, class N_>
 __device__ __forceinline__ T interface_flux(const nz_problem& P, const T UL[4], const T UR[4],
-                                            const T& nx, const T& ny, T out[4]) {
+                                            const N_& nx, const N_& ny, T out[4]) {
     if constexpr (FLUX == 1) return flux_hllc(UL, UR, nx, ny, P.gamma, P.low_mach, out);
     else if constexpr (FLUX == 2) return flux_ausm(UL, UR, nx, ny, P.gamma, out);
     else return flux_roe(UL, UR, nx, ny, P.gamma, P.efix, out);

@@ -221,8 +221,14 @@ class DifferentiableCase:
 
     def value_and_gradient(self, objective: str = "thrust", params=None, *,
                            names: Sequence[str] | None = None, U0, adjoint_tolerance: float = 1e-8,
-                           fd_step: float = 1e-6, verbose: bool = False):
-        """``(J, {name: dJ/dname}, forward_result, adjoint_info)``."""
+                           fd_step: float = 1e-6, verbose: bool = False,
+                           adjoint_method: str = "rk4"):
+        """``(J, {name: dJ/dname}, forward_result, adjoint_info)``.
+
+        ``adjoint_method``: ``"rk4"``, the reference's pseudo-time march, or
+        ``"gmres"``, restarted GMRES on the same system (see
+        ``HipBackend.adjoint_solve``).
+        """
         from .backend import HipBackend
 
         base = self._merged(params)
@@ -236,7 +242,7 @@ class DifferentiableCase:
         J, J_U, J_edge = objective_and_partials(fwd.U, ops, flow, objective)
         bk = HipBackend(ops, flow, self.options)
         lam, rel, its, ok = bk.adjoint_solve(fwd.U, J_U, tolerance=adjoint_tolerance,
-                                             verbose=verbose)
+                                             verbose=verbose, method=adjoint_method)
         if not ok:
             raise RuntimeError(f"the adjoint march stalled at relative residual {rel:.2e} after "
                                f"{its} iterations")

@@ -133,3 +133,22 @@ def test_gradient_against_central_differences():
         assert abs(fd - grad[k]) <= 1e-4 * abs(grad[k])
     # the reference's recorded length derivative (docs/validation/gradients.md:28)
     assert abs(grad["length"] - 1.353855e-2) <= 6e-7 * 1.353855e-2
+
+
+def test_gmres_adjoint_matches_the_march():
+    """Restarted GMRES on the same linear system: the captured derivatives to
+    their printed digits, and the march's gradient to the adjoint tolerance,
+    in far fewer VJPs than the march's four per step."""
+    dc, g = _case()
+    names = ("area_ratio", "throat_x", "theta_exit", "inlet_half_height")
+    _, grad_rk, _, info_rk = dc.value_and_gradient("thrust", names=names, U0=g["U0"])
+    _, grad_gm, _, info_gm = dc.value_and_gradient("thrust", names=names, U0=g["U0"],
+                                                   adjoint_method="gmres")
+    print("rk4", info_rk, "gmres", info_gm)
+    assert info_gm["adjoint_relative_residual"] <= 1e-8
+    assert info_gm["adjoint_iterations"] < 4 * info_rk["adjoint_iterations"]
+    captured = {"area_ratio": 9.772082e-03, "throat_x": -1.490300e-02,
+                "theta_exit": 7.602688e-03, "inlet_half_height": -1.524949e+00}
+    for k, v in captured.items():
+        assert abs(grad_gm[k] - v) <= 6e-7 * abs(v) + 1e-12, (k, grad_gm[k], v)
+        assert abs(grad_gm[k] - grad_rk[k]) <= 1e-6 * abs(grad_rk[k]), (k, grad_gm[k], grad_rk[k])
