@@ -258,6 +258,18 @@ struct Tile {
     static constexpr int WPE = LDS * 16 <= 160 * 1024 ? 4 : 1;
 };
 
+// LDS hand-off between the lanes of a one-wave workgroup.  A wave's LDS
+// operations complete in program order, so lanes see each other's earlier
+// writes without waiting for them; what is needed is that the compiler keep
+// the accesses on their side of this point (a wavefront-scope fence), not a
+// workgroup barrier, whose wait for every outstanding load and store would
+// also stall the global and scalar loads issued ahead of their use.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // component k of this lane's element (lanes 4e..4e+3 hold s = 0..3)
 __device__ __forceinline__ double quad_get(double v, int k) {
     return __shfl(v, (static_cast<int>(threadIdx.x) & ~3) | k, 64);
@@ -513,7 +525,7 @@ void k_dg(const nz_problem P, const double* __restrict__ T,
         for (int j = 0; j < NG; ++j)
             if (t + j * DG_BS < L::GEN) sGe[t + j * DG_BS] = rg[j];
     }
-    __syncthreads();
+    wave_sync();
     // edge geometry of the tile's faces, gathered by edge id
     {
         constexpr int NE_ = (L::EGN + DG_BS - 1) / DG_BS;
@@ -537,7 +549,7 @@ void k_dg(const nz_problem P, const double* __restrict__ T,
         acc[i] = 0.0;
     }
     double wave = 0.0;
-    __syncthreads();  // the staging rows are reused below
+    wave_sync();  // the staging rows are reused below
 
     double* sTo = sW;                 // own trace, then weighted flux
     double* sTn = sW + NQF * QS;      // neighbour trace
@@ -596,7 +608,7 @@ void k_dg(const nz_problem P, const double* __restrict__ T,
 #pragma unroll
             for (int q = 0; q < NQF; ++q) sTn[q * QS + cs] = tn[q];
         }
-        __syncthreads();
+        wave_sync();
         // (q,e): the flux at edge point q, one item per lane
         for (int it = t; it < NQF * TE; it += DG_BS) {
             const int q = it / TE, pe = it % TE;
@@ -635,7 +647,7 @@ void k_dg(const nz_problem P, const double* __restrict__ T,
                 sSp[q * TE + pe] = sp;
             }
         }
-        __syncthreads();
+        wave_sync();
         // (e,s): lift, and the element's wave-speed sum
         {
             double fw[NQF];
@@ -661,7 +673,7 @@ void k_dg(const nz_problem P, const double* __restrict__ T,
         // the next face rewrites these rows after the reads above (one wave:
         // LDS operations complete in program order)
     }
-    __syncthreads();
+    wave_sync();
 
     // --------------------------------------------------------------- volume
     double* sUq = sW;                   // point states, then xi-flux
@@ -681,7 +693,7 @@ void k_dg(const nz_problem P, const double* __restrict__ T,
                 sUq[qq * QS + cs] = a;
             }
         }
-        __syncthreads();
+        wave_sync();
         {
             const int qq = t / TE, pe = t % TE, q = q0 + qq;
             const int64_t ep = e0 + pe;
@@ -716,7 +728,7 @@ void k_dg(const nz_problem P, const double* __restrict__ T,
                 }
             }
         }
-        __syncthreads();
+        wave_sync();
 #pragma unroll 1
         for (int qq = 0; qq < L::QR; ++qq) {
             const int q = q0 + qq;
@@ -748,13 +760,13 @@ void k_dg(const nz_problem P, const double* __restrict__ T,
 #pragma unroll
         for (int i = 0; i < N; ++i) acc[i] = r[i];
     }
-    __syncthreads();  // last reads of the volume rows are done; reuse sW for staging
+    wave_sync();  // last reads of the volume rows are done; reuse sW for staging
     if (!STAGE) {
         const double osign = mode == kRes ? 1.0 : -1.0;  // rate = -M^-1 R
 #pragma unroll
         for (int i = 0; i < N; ++i) sW[el * L::SIO + i * 4 + s] = osign * acc[i];
         if (s == 0 && live) dt[e] = wave;
-        __syncthreads();
+        wave_sync();
         store_tile<L>(Uout + e0 * NS, sW, t, ne);
         return;
     }
@@ -809,7 +821,7 @@ void k_dg(const nz_problem P, const double* __restrict__ T,
     }
 #pragma unroll
     for (int i = 0; i < N; ++i) sW[el * L::SIO + i * 4 + s] = u[i];
-    __syncthreads();
+    wave_sync();
     store_tile<L>(Uout + e0 * NS, sW, t, ne);
 }
 
