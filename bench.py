@@ -9,6 +9,7 @@ element kernel) with the state resident in HBM.  It is timed for each of the
 Roe, HLLC and AUSM+-up fluxes; ``value`` is the Roe figure.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-march]
+                    [--shocked] [--adjoint] [--adjoint-stages K] [--sweep N]
 
 Prints one JSON line.  ``value`` is Gdof-stage/s = E N / t_eval / 1e9.
 ``roofline`` prices the residual against HBM: the algorithmic bytes of one
@@ -20,6 +21,10 @@ evaluation when a counter profile of this workload is recorded in
 reports the fused SSP-RK3 stage loop on the same mesh and state (positivity
 limiter on).  ``cpu_baseline`` times the CPU oracle (an OpenMP restatement of
 the reference's compiled kernels) on a bounded sample of the same workload.
+The optional legs add the shocked quadrilateral march (``--shocked``), the
+Bezier-wall design gradients (``--adjoint``), the per-step cost of the
+adjoint march on that mesh (``--adjoint-stages``) and the batched design
+sweep with its gradients (``--sweep``), each under a key of its own.
 """
 
 from __future__ import annotations
@@ -139,6 +144,41 @@ def shocked_bench(refine: int, steps: int) -> dict:
         "elements_scaled_per_stage": round(cnt[0] / (3 * (steps + 1)), 1),
         "residual_norm_last": float(bk._res.item()),
     }
+
+
+def adjoint_stage_bench(refine: int, steps: int) -> dict:
+    """Per-step cost of the adjoint march against the forward march on the
+    Bezier-wall mesh (tri P2, 14 x 7 base grid, refine 5: 200,704 elements),
+    on the quasi-1D state after 50 forward steps with a seeded random
+    cotangent: ``steps`` SSP-RK3 steps, then ``steps`` RK4 adjoint steps
+    (four residual-VJP launches each), each after a warm-up of its own."""
+    import torch
+
+    wl, cf, ini, mesh, opm = (_pkg("workloads"), _pkg("config"), _pkg("initial"), _pkg("mesh"),
+                              _pkg("operators"))
+    geom = wl.bezier9_geometry(0)
+    topo, coords, _ = mesh.build_nozzle_mesh(geom, kind="tri", nx=14, nr=7, refine=refine)
+    ops = opm.build_operators(topo, coords, 2, dense=False)
+    flow = cf.FlowConditions()
+    bk = _pkg("backend").HipBackend(ops, flow, cf.SolverOptions())
+    U, _ = bk.run(ini.initial_state(ops, flow, geom, "quasi1d"), 50, "ssprk3")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    U, _ = bk.run(U, steps, "ssprk3")
+    torch.cuda.synchronize()
+    t_f = (time.perf_counter() - t0) / steps
+    seed = np.random.default_rng(0).standard_normal(U.shape) * 1e-3
+    bk.adjoint_solve(U, seed, tolerance=1e-30, max_iterations=steps // 2, check_interval=steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bk.adjoint_solve(U, seed, tolerance=1e-30, max_iterations=steps, check_interval=steps)
+    torch.cuda.synchronize()
+    t_a = (time.perf_counter() - t0) / steps
+    E, N = ops.n_elem, ops.ref.n_basis
+    return {"workload": f"bezier9-tri-p2-14x7-refine{refine}", "elements": E, "steps": steps,
+            "forward_ms_per_step": round(t_f * 1e3, 4), "adjoint_ms_per_step": round(t_a * 1e3, 4),
+            "forward_gdof_stage_s": round(3 * E * N / t_f / 1e9, 3),
+            "adjoint_gdof_stage_s": round(4 * E * N / t_a / 1e9, 3)}
 
 
 def adjoint_bench(refine: int) -> dict:
@@ -324,6 +364,8 @@ def main() -> int:
     ap.add_argument("--adjoint", action="store_true",
                     help="also time the design-gradient case (degree-9 Bezier wall, tri P2)")
     ap.add_argument("--adjoint-refine", type=int, default=5)
+    ap.add_argument("--adjoint-stages", type=int, default=0,
+                    help="also time this many adjoint and forward steps on the Bezier-wall mesh")
     ap.add_argument("--sweep", type=int, default=0,
                     help="also march this many designs of the mixed sweep (seed 1) in one batch")
     ap.add_argument("--sweep-tol", type=float, default=1e-10)
@@ -434,6 +476,8 @@ def main() -> int:
         out["shocked"] = shocked_bench(args.shocked_refine, max(2, args.steps // 2))
     if args.adjoint:
         out["adjoint"] = adjoint_bench(args.adjoint_refine)
+    if args.adjoint_stages > 0:
+        out["adjoint_stages"] = adjoint_stage_bench(args.adjoint_refine, args.adjoint_stages)
     if args.sweep > 0:
         out["sweep"] = sweep_bench(args.sweep, args.sweep_tol, args.sweep_lone, args.sweep_maxit)
     if not args.no_cpu:
