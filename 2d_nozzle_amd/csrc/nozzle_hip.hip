@@ -1188,6 +1188,16 @@ __global__ __launch_bounds__(BS) void k_vjp(const nz_problem P, const double* __
                         dl[s] = side == 0 ? lo[s] - ln : ln - lo[s];
                     }
                     double g[KD], gj = 0.0;
+                    if constexpr (FLUX == 0 && !GEO) {
+                        // Roe: one reverse-mode pass for the owner's cotangent
+                        double uL[4], uR[4], ob[4];
+                        for (int s = 0; s < 4; ++s) {
+                            uL[s] = side == 0 ? uo[s] : un[s];
+                            uR[s] = side == 0 ? un[s] : uo[s];
+                            ob[s] = jw * dl[s];
+                        }
+                        roe_vjp(uL, uR, eg[0], eg[1], P.gamma, P.efix, side == 0, ob, g);
+                    } else {
 #pragma unroll
                     for (int d0 = 0; d0 < KD; d0 += KB) {
                         D ul[4], ur[4], fx[4];
@@ -1211,6 +1221,7 @@ __global__ __launch_bounds__(BS) void k_vjp(const nz_problem P, const double* __
                         }
                         if (d0 == 0)
                             for (int s = 0; s < 4; ++s) gj += dl[s] * fx[s].v;
+                    }
                     }
                     for (int d = 0; d < 4; ++d) v[d] = g[d];
                     if constexpr (GEO) {
@@ -1285,22 +1296,24 @@ __global__ __launch_bounds__(BS) void k_vjp(const nz_problem P, const double* __
                     bx[s] = wd * (ax[s] * xx + ay[s] * ex);
                     by[s] = wd * (ax[s] * xy + ay[s] * ey);
                 }
-                using D = Dual<KB>;
                 double Fv[4], Gv[4];
-#pragma unroll
-                for (int d0 = 0; d0 < 4; d0 += KB) {  // KB tangent directions per pass
+                if constexpr (GEO) {
+                    // the fluxes' values are needed for the metric cotangents
+                    using D = Dual<4>;
                     D ud[4];
-                    for (int s = 0; s < 4; ++s) ud[s] = seeded<KB>(uq[s], s - d0);
+                    for (int s = 0; s < 4; ++s) ud[s] = D::seed(uq[s], s);
                     const Prim<D> w = primitives(ud, P.gamma);
                     const D F[4] = {w.r * w.u, w.r * w.u * w.u + w.p, w.r * w.u * w.v, w.r * w.u * w.H};
                     const D G[4] = {w.r * w.v, w.r * w.u * w.v, w.r * w.v * w.v + w.p, w.r * w.v * w.H};
-                    for (int s = 0; s < 4; ++s)
-                        for (int j = 0; j < KB; ++j) g[d0 + j] -= bx[s] * F[s].d[j] + by[s] * G[s].d[j];
-                    if (d0 == 0)
-                        for (int s = 0; s < 4; ++s) {
-                            Fv[s] = F[s].v;
-                            Gv[s] = G[s].v;
-                        }
+                    for (int s = 0; s < 4; ++s) {
+                        for (int d = 0; d < 4; ++d) g[d] -= bx[s] * F[s].d[d] + by[s] * G[s].d[d];
+                        Fv[s] = F[s].v;
+                        Gv[s] = G[s].v;
+                    }
+                } else {  // reverse mode: one pass instead of four tangents
+                    double ub[4];
+                    euler_flux_vjp(uq, P.gamma, bx, by, ub);
+                    for (int d = 0; d < 4; ++d) g[d] = -ub[d];
                 }
                 for (int s = 0; s < 4; ++s) {
                     if constexpr (GEO) {

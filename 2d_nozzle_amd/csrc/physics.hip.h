@@ -315,6 +315,225 @@ __device__ __forceinline__ T flux_outflow(const T U[4], const T& nx, const T& ny
     return fabs(o.u * nx + o.v * ny) + sqrt(g * o.p / o.r);
 }
 
+// Reverse mode of the Euler fluxes at a volume point: with F(U), G(U) as
+// evaluated by primitives() and the point-flux formulas, returns
+// ub = (dF/dU)^T fb + (dG/dU)^T gb.  Each statement of the forward evaluation
+// is adjointed in reverse order, so the result is the derivative of exactly
+// that arithmetic, floors included: a floored density or pressure passes no
+// derivative (the taken branch of fmax).
+__device__ __forceinline__ void euler_flux_vjp(const double U[4], double g, const double fb[4],
+                                               const double gb[4], double ub[4]) {
+    // forward
+    const bool rok = U[0] >= kFloor;
+    const double r = rok ? U[0] : kFloor;
+    const double ir = recip(r);
+    const double u = U[1] * ir, v = U[2] * ir;
+    const double k = u * u + v * v;
+    const double pq = (g - 1.0) * (U[3] - 0.5 * r * k);
+    const bool pok = pq >= kFloor;
+    const double p = pok ? pq : kFloor;
+    const double H = (U[3] + p) * ir;
+    // F = (r u, r u u + p, r u v, r u H), G = (r v, r u v, r v v + p, r v H)
+    double rb = fb[0] * u + fb[1] * u * u + fb[2] * u * v + fb[3] * u * H
+              + gb[0] * v + gb[1] * u * v + gb[2] * v * v + gb[3] * v * H;
+    double ubar = fb[0] * r + fb[1] * 2.0 * r * u + fb[2] * r * v + fb[3] * r * H + gb[1] * r * v;
+    double vbar = fb[2] * r * u + gb[0] * r + gb[1] * r * u + gb[2] * 2.0 * r * v + gb[3] * r * H;
+    double pb = fb[1] + gb[2];
+    const double Hb = fb[3] * r * u + gb[3] * r * v;
+    // H = (U3 + p) ir
+    double u3b = Hb * ir;
+    pb += Hb * ir;
+    double irb = Hb * (U[3] + p);
+    // p = max((g - 1)(U3 - r k / 2), floor)
+    if (pok) {
+        const double qb = (g - 1.0) * pb;
+        u3b += qb;
+        rb -= 0.5 * k * qb;
+        ubar -= r * u * qb;
+        vbar -= r * v * qb;
+    }
+    // u = U1 ir, v = U2 ir
+    const double u1b = ubar * ir, u2b = vbar * ir;
+    irb += ubar * U[1] + vbar * U[2];
+    // ir = 1 / r
+    rb -= irb * ir * ir;
+    ub[0] = rok ? rb : 0.0;
+    ub[1] = u1b;
+    ub[2] = u2b;
+    ub[3] = u3b;
+}
+
+// Reverse mode of primitives(): the cotangents (rb, ub, vb, pb, Hb) of the
+// primitive values pulled back to the conserved state U.
+__device__ __forceinline__ void primitives_vjp(const double U[4], double g, double rb, double ub,
+                                               double vb, double pb, double Hb, double out[4]) {
+    const bool rok = U[0] >= kFloor;
+    const double r = rok ? U[0] : kFloor;
+    const double ir = recip(r);
+    const double u = U[1] * ir, v = U[2] * ir;
+    const double k = u * u + v * v;
+    const double pq = (g - 1.0) * (U[3] - 0.5 * r * k);
+    const double p = pq >= kFloor ? pq : kFloor;
+    double u3b = Hb * ir;
+    pb += Hb * ir;
+    double irb = Hb * (U[3] + p);
+    if (pq >= kFloor) {
+        const double qb = (g - 1.0) * pb;
+        u3b += qb;
+        rb -= 0.5 * k * qb;
+        ub -= r * u * qb;
+        vb -= r * v * qb;
+    }
+    irb += ub * U[1] + vb * U[2];
+    rb -= irb * ir * ir;
+    out[0] = rok ? rb : 0.0;
+    out[1] = ub * ir;
+    out[2] = vb * ir;
+    out[3] = u3b;
+}
+
+// Reverse mode of flux_roe with respect to one side's state (the left one
+// when own_left): given the cotangent ob of the flux, returns
+// ob^T dF/dU_own.  The statements of flux_roe are adjointed in reverse
+// order, branch by branch as evaluated (entropy fixes, floors, |.|), so the
+// result is the derivative of exactly that arithmetic, in one backward pass
+// rather than four forward tangents.
+__device__ __forceinline__ void roe_vjp(const double UL[4], const double UR[4], double nx,
+                                        double ny, double g, double efix, bool own_left,
+                                        const double ob[4], double out[4]) {
+    // ---- forward, keeping what the backward pass reads
+    const Prim<double> L = primitives(UL, g), R = primitives(UR, g);
+    const double vnL = L.u * nx + L.v * ny, vnR = R.u * nx + R.v * ny;
+    double d[4];
+    for (int k = 0; k < 4; ++k) d[k] = UR[k] - UL[k];
+    const double sL = root(L.r), sR = root(R.r), iden = recip(sL + sR);
+    const double nvx = sL * L.u + sR * R.u, nvy = sL * L.v + sR * R.v, nH = sL * L.H + sR * R.H;
+    const double vx = nvx * iden, vy = nvy * iden, H = nH * iden;
+    const double q2 = vx * vx + vy * vy;
+    const double cq = (g - 1.0) * (H - 0.5 * q2);
+    const bool cok = cq >= kFloor;
+    const double c = root(cok ? cq : kFloor);
+    const double ic = recip(c);
+    const double vn = vx * nx + vy * ny;
+    const double a1 = vn + c, a2 = vn - c;
+    const double l1o = fabs(a1), l2o = fabs(a2), l3o = fabs(vn);
+    const double eps = efix * c;
+    const double re = recip(eps);
+    const double h = 0.5 * re;
+    const bool f1 = l1o < eps, f2 = l2o < eps, f3 = l3o < eps;
+    const double l1 = f1 ? (l1o * l1o + eps * eps) * h : l1o;
+    const double l2 = f2 ? (l2o * l2o + eps * eps) * h : l2o;
+    const double l3 = f3 ? (l3o * l3o + eps * eps) * h : l3o;
+    const double G1 = (g - 1.0) * (0.5 * q2 * d[0] - vx * d[1] - vy * d[2] + d[3]);
+    const double G2 = -vn * d[0] + d[1] * nx + d[2] * ny;
+    const double s1 = 0.5 * (l1 + l2), s2 = 0.5 * (l1 - l2);
+    const double G1c = G1 * ic;
+    const double t1 = G1c * ic, t2 = G2 * ic, sm = s1 - l3;
+    const double C1 = t1 * sm + t2 * s2;
+    const double C2 = G1c * s2 + sm * G2;
+    // ---- backward
+    double db[4];
+    double l3b = -0.5 * (ob[0] * d[0] + ob[1] * d[1] + ob[2] * d[2] + ob[3] * d[3]);
+    for (int k = 0; k < 4; ++k) db[k] = -0.5 * l3 * ob[k];
+    const double C1b = -0.5 * (ob[0] + ob[1] * vx + ob[2] * vy + ob[3] * H);
+    const double C2b = -0.5 * (ob[1] * nx + ob[2] * ny + ob[3] * vn);
+    double vxb = -0.5 * ob[1] * C1, vyb = -0.5 * ob[2] * C1, Hb = -0.5 * ob[3] * C1;
+    double vnb = -0.5 * ob[3] * C2;
+    // C2 = G1c s2 + sm G2
+    double G1cb = C2b * s2, s2b = C2b * G1c, smb = C2b * G2, G2b = C2b * sm;
+    // C1 = t1 sm + t2 s2
+    const double t1b = C1b * sm, t2b = C1b * s2;
+    smb += C1b * t1;
+    s2b += C1b * t2;
+    G1cb += t1b * ic;
+    double icb = t1b * G1c + t2b * G2;
+    G2b += t2b * ic;
+    // G1c = G1 ic
+    const double G1b = G1cb * ic;
+    icb += G1cb * G1;
+    // sm = s1 - l3; s1 = (l1 + l2) / 2; s2 = (l1 - l2) / 2
+    const double s1b = smb;
+    l3b -= smb;
+    double l1b = 0.5 * (s1b + s2b), l2b = 0.5 * (s1b - s2b);
+    // G2 = -vn d0 + d1 nx + d2 ny
+    vnb -= G2b * d[0];
+    db[0] -= G2b * vn;
+    db[1] += G2b * nx;
+    db[2] += G2b * ny;
+    // G1 = (g - 1)(q2 d0 / 2 - vx d1 - vy d2 + d3)
+    const double gG = (g - 1.0) * G1b;
+    double q2b = 0.5 * gG * d[0];
+    db[0] += 0.5 * gG * q2;
+    vxb -= gG * d[1];
+    db[1] -= gG * vx;
+    vyb -= gG * d[2];
+    db[2] -= gG * vy;
+    db[3] += gG;
+    // entropy fixes l = (lo^2 + eps^2) h
+    double epsb = 0.0, hb = 0.0;
+    if (f1) {
+        epsb += l1b * 2.0 * eps * h;
+        hb += l1b * (l1o * l1o + eps * eps);
+        l1b = l1b * 2.0 * l1o * h;
+    }
+    if (f2) {
+        epsb += l2b * 2.0 * eps * h;
+        hb += l2b * (l2o * l2o + eps * eps);
+        l2b = l2b * 2.0 * l2o * h;
+    }
+    if (f3) {
+        epsb += l3b * 2.0 * eps * h;
+        hb += l3b * (l3o * l3o + eps * eps);
+        l3b = l3b * 2.0 * l3o * h;
+    }
+    // h = re / 2, re = 1 / eps, eps = efix c
+    epsb -= 0.5 * hb * re * re;
+    double cb = epsb * efix;
+    // l1 = |vn + c|, l2 = |vn - c|, l3 = |vn|
+    const double a1b = a1 < 0.0 ? -l1b : l1b, a2b = a2 < 0.0 ? -l2b : l2b;
+    vnb += a1b + a2b + (vn < 0.0 ? -l3b : l3b);
+    cb += a1b - a2b;
+    // vn = vx nx + vy ny
+    vxb += vnb * nx;
+    vyb += vnb * ny;
+    // ic = 1 / c
+    cb -= icb * ic * ic;
+    // c = sqrt(max(cq, floor))
+    if (cok) {
+        const double cqb = cb * 0.5 * ic;
+        Hb += (g - 1.0) * cqb;
+        q2b -= 0.5 * (g - 1.0) * cqb;
+    }
+    // q2 = vx^2 + vy^2
+    vxb += 2.0 * vx * q2b;
+    vyb += 2.0 * vy * q2b;
+    // vx = nvx iden, vy = nvy iden, H = nH iden
+    const double nvxb = vxb * iden, nvyb = vyb * iden, nHb = Hb * iden;
+    const double idenb = vxb * nvx + vyb * nvy + Hb * nH;
+    // iden = 1 / (sL + sR)
+    const double sumb = -idenb * iden * iden;
+    // the own side only from here
+    const Prim<double>& W = own_left ? L : R;
+    const double sW = own_left ? sL : sR, vnW = own_left ? vnL : vnR;
+    double sWb = sumb + nvxb * W.u + nvyb * W.v + nHb * W.H;
+    double rb = sWb * 0.5 / sW;  // s = sqrt(r)
+    double ub = nvxb * sW, vb = nvyb * sW, Wb = nHb * sW;
+    // the central part (F_own / 2) of the flux: normal_flux(W)
+    double fb[4];
+    for (int k = 0; k < 4; ++k) fb[k] = 0.5 * ob[k];
+    rb += fb[0] * vnW + fb[1] * W.u * vnW + fb[2] * W.v * vnW + fb[3] * W.H * vnW;
+    ub += fb[1] * W.r * vnW;
+    vb += fb[2] * W.r * vnW;
+    Wb += fb[3] * W.r * vnW;
+    const double pb = fb[1] * nx + fb[2] * ny;
+    const double vnWb = fb[0] * W.r + fb[1] * W.r * W.u + fb[2] * W.r * W.v + fb[3] * W.r * W.H;
+    ub += vnWb * nx;
+    vb += vnWb * ny;
+    primitives_vjp(own_left ? UL : UR, g, rb, ub, vb, pb, Wb, out);
+    // d = UR - UL
+    for (int k = 0; k < 4; ++k) out[k] += own_left ? -db[k] : db[k];
+}
+
 // FLUX: 0 roe, 1 hllc, 2 ausmup (src/backends/numba_backend.py:59-67)
 template <int FLUX, class T, class N_>
 __device__ __forceinline__ T interface_flux(const nz_problem& P, const T UL[4], const T UR[4],
