@@ -63,6 +63,13 @@ struct Cfg {
     static constexpr int A_FACE = 3 * NQF * 4 * TP;
     static constexpr int AWORK = A_FACE > NS * TP ? (A_FACE > W_VOL ? A_FACE : W_VOL)
                                                  : (NS * TP > W_VOL ? NS * TP : W_VOL);
+    // LDS bytes of one adjoint workgroup (four waves, one per SIMD) and the
+    // waves per SIMD that the CU's 160 KB allows, at most three (168 VGPRs:
+    // the reverse-mode Roe kernel needs about that many)
+    static constexpr int ALDS = 8 * (2 * NS * TP + AWORK + PF + 5 * TE) + 4 * NF * TE;
+    static constexpr int AWPE = (160 * 1024) / ALDS < 1   ? 1
+                                : (160 * 1024) / ALDS > 3 ? 3
+                                                          : (160 * 1024) / ALDS;
 };
 
 // ---------------------------------------------------------------- limiter
@@ -1062,8 +1069,13 @@ __global__ __launch_bounds__(BS) void k_bvjp(const nz_problem P, const double* _
     if (gpb) gpb[b * C::NQF + q] = v[6];
 }
 
+// The register budget of the reverse-mode Roe instantiations is held to the
+// occupancy the LDS allows (three waves per SIMD at P2: no spills, and 19 %
+// faster than the unconstrained allocation's two); the dual-number
+// instantiations need far more registers and are left unconstrained.
 template <class C, int FLUX, bool GEO>
-__global__ __launch_bounds__(BS) void k_vjp(const nz_problem P, const double* __restrict__ T,
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(FLUX == 0 && !GEO ? C::AWPE : 1)))
+void k_vjp(const nz_problem P, const double* __restrict__ T,
                                             const double* __restrict__ U,
                                             const double* __restrict__ Lin,
                                             const double* __restrict__ bv,
