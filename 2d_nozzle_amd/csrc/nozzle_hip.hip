@@ -1,28 +1,29 @@
-// MI355X (gfx950) kernels for the DG residual, the fused SSP-RK3/Euler stage
-// and the Zhang-Shu positivity limiter, plus the C ABI declared in
-// include/nozzle_hip.h.
+// MI355X (gfx950) kernels for the DG residual, the fused SSP-RK3/Euler stage,
+// the Zhang-Shu positivity limiter and the residual VJP, plus the C ABI
+// declared in include/nozzle_hip.h.
 //
-// Work decomposition.  One workgroup of 256 threads (four wave64s) owns a
-// tile of TE consecutive elements (TE = 64, or 128 when N <= 4).  The tile's
-// state is staged once in LDS, transposed to [component][element] with a
-// padded row so that a wave reading one component of 64 elements hits 64
-// consecutive doubles.  Every phase then maps the wave index to a
-// wave-uniform table index (component s, quadrature point q or basis
-// function i) and the lane to the element, so reference tables are read with
-// scalar loads and LDS traffic is conflict-free:
+// Work decomposition.
 //
-//   faces   for each local face f: (e,s) items gather the neighbour's trace
-//           at the face points straight from HBM; (e,q) items form the own
-//           trace from LDS and evaluate the Riemann or boundary flux;
-//           (e,i) items lift the weighted flux into register accumulators.
-//           Every interior flux is evaluated by both elements of the edge
-//           with identical operands, so both see the same bits and the
-//           assembly stays a pure gather with no flux table in HBM.
-//   volume  rounds of BS/TE quadrature points: (e,q) items form the
-//           contravariant Euler fluxes, (e,i) items project them.
-//   stage   the rate, the Runge-Kutta combination, the element-local
-//           positivity limiter, the time step and the residual-norm partials
-//           are fused; the new state leaves through LDS in coalesced rows.
+//   k_dg    the residual / rate / fused stage.  One wave (64 lanes) per
+//           workgroup owns a tile of 16 elements, four lanes per element (one
+//           per conserved variable).  Everything linear in U (traces,
+//           volume-point interpolation, lift, projection, the stage update,
+//           the limiter's probe sums) is separable by component, so each lane
+//           keeps its element's column U[:, s] and its accumulator in
+//           registers and reads reference tables with scalar loads; only the
+//           pointwise physics (Riemann and Euler fluxes, one (point, element)
+//           item per lane) goes through LDS.  Every interior flux is
+//           evaluated by both elements of the edge with identical operands,
+//           so both see the same bits and the assembly is a pure gather with
+//           no flux table in HBM.  See the comment above k_dg.
+//   k_vjp   the residual VJP and the adjoint RK4 stage.  One workgroup of 256
+//           threads (four waves) owns a tile of TE elements (64, or 128 when
+//           N <= 4), staged in LDS as [component][element] rows; each phase
+//           maps the wave to a wave-uniform index (component, point or basis
+//           function) and the lane to the element.
+//   others  small elementwise or per-element kernels (boundary pre-passes,
+//           standalone limiter, superbee passes, inverse mass, time step,
+//           norm finishing) on 256-thread workgroups.
 //
 // Affine triangles (Q = 1) fold M^-1 = M_ref^-1 / det into reference tables,
 // so an element carries only (xi_x, xi_y, eta_x, eta_y, det).  Curved or
