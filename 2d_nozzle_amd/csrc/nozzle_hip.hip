@@ -439,12 +439,14 @@ __device__ uint8_t limit_column(double* col, int s, const double* mw, const doub
 template <class L>
 __device__ __forceinline__ void store_tile(double* __restrict__ out, const double* tile, int t,
                                            int ne) {
+    // streaming stores: the tile is not read again by this launch, so it is
+    // kept out of the way of the neighbour gathers in L2
     constexpr int NU = TE * L::NS / DG_BS;
     if (ne == TE) {
 #pragma unroll
-        for (int j = 0; j < NU; ++j) out[t + j * DG_BS] = tile[t + j * DG_BS];
+        for (int j = 0; j < NU; ++j) __builtin_nontemporal_store(tile[t + j * DG_BS], &out[t + j * DG_BS]);
     } else {
-        for (int it = t; it < ne * L::NS; it += DG_BS) out[it] = tile[it];
+        for (int it = t; it < ne * L::NS; it += DG_BS) __builtin_nontemporal_store(tile[it], &out[it]);
     }
 }
 
