@@ -266,6 +266,16 @@ struct Tile {
     static constexpr int WPE = LDS * 16 <= 160 * 1024 ? 4 : 1;
 };
 
+// Adds the number of active lanes whose pred holds to *c with one atomic per
+// wave.  Same-address atomics from every lane would serialise at the memory
+// side (the counter is shared by all XCDs).  Call from converged code.
+__device__ __forceinline__ void wave_count(unsigned long long* c, bool pred) {
+    const unsigned long long m = __ballot(pred);
+    const unsigned long long act = __ballot(true);
+    if (m && static_cast<int>(__lane_id()) == __ffsll(static_cast<long long>(act)) - 1)
+        atomicAdd(c, static_cast<unsigned long long>(__popcll(m)));
+}
+
 // LDS hand-off between the lanes of a one-wave workgroup.  A wave's LDS
 // operations complete in program order, so lanes see each other's earlier
 // writes without waiting for them; what is needed is that the compiler keep
@@ -316,40 +326,59 @@ __device__ __forceinline__ double probe_value(const double* col, int k,
 template <class C>
 struct Probes {
     static constexpr int NP = C::NQV + C::NF * C::NQF;
-    // the probe sums do not change while theta is searched: keep them in
-    // registers when they fit beside the element's column
-    static constexpr bool CACHE = NP <= 32 && C::N <= 10;
+    // probes per lane: lane s of an element holds the four conserved
+    // variables of probes s, s + 4, s + 8, ...
+    static constexpr int NK = (NP + 3) / 4;
 };
 
-// min density and pressure of ub + theta (V - ub) over the probes; every lane
-// of the element returns the same values.  pv: cached probe sums, or null to
-// recompute them from col.
+// The probe states of this lane's share, formed once per limited element:
+// every lane sums its own component at each probe (no contraction), and the
+// lane that owns probe k (k mod 4) collects the four components.
 template <class C>
-__device__ __forceinline__ void probe_minmax(const double* pv, const double* col, const double ub[4],
-                                             double theta, const double* __restrict__ T,
-                                             const nz_problem& P, int sidebits, double* rmin,
-                                             double* pmin) {
-#pragma clang fp contract(off)
+__device__ __forceinline__ void probe_share(const double* col, int s, const double* __restrict__ T,
+                                            const nz_problem& P, int sidebits,
+                                            double V[Probes<C>::NK][4]) {
     constexpr int NP = Probes<C>::NP;
-    double rm = 1e300, pm = 1e300;
-    const double g = P.gamma;
-    auto one = [&](double v) {
-        double V[4];
-        for (int c = 0; c < 4; ++c) V[c] = quad_get(v, c);
-        for (int c = 0; c < 4; ++c) V[c] = ub[c] + theta * (V[c] - ub[c]);
-        rm = V[0] < rm ? V[0] : rm;
-        const double r = V[0] > 1e-300 ? V[0] : 1e-300;
-        const double pp = (g - 1.0) * (V[3] - 0.5 * (V[1] * V[1] + V[2] * V[2]) / r);
-        pm = pp < pm ? pp : pm;
-    };
-    if constexpr (Probes<C>::CACHE) {
-        (void)col;
 #pragma unroll
-        for (int k = 0; k < NP; ++k) one(pv[k]);
-    } else {
-        (void)pv;
-#pragma unroll 1
-        for (int k = 0; k < NP; ++k) one(probe_value<C>(col, k, T, P, sidebits));
+    for (int k = 0; k < NP; ++k) {
+        const double v = probe_value<C>(col, k, T, P, sidebits);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const double x = quad_get(v, c);
+            if ((k & 3) == s) V[k >> 2][c] = x;
+        }
+        // one probe's table column in scalar registers at a time
+        __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int kk = 0; kk < Probes<C>::NK; ++kk)
+        if (4 * kk + s >= NP)  // no such probe: repeat one that exists
+            for (int c = 0; c < 4; ++c) V[kk][c] = V[0][c];
+}
+
+// min density and pressure of ub + theta (V - ub) over the element's probes;
+// every lane of the element returns the same values.  Each lane scans its
+// share and the four lanes combine with two shuffles; min is exact, so the
+// result is the same as a scan over all probes in order.
+template <class C>
+__device__ __forceinline__ void probe_minmax(const double V[Probes<C>::NK][4], const double ub[4],
+                                             double theta, double g, double* rmin, double* pmin) {
+#pragma clang fp contract(off)
+    double rm = 1e300, pm = 1e300;
+#pragma unroll
+    for (int kk = 0; kk < Probes<C>::NK; ++kk) {
+        double W[4];
+        for (int c = 0; c < 4; ++c) W[c] = ub[c] + theta * (V[kk][c] - ub[c]);
+        rm = W[0] < rm ? W[0] : rm;
+        const double r = W[0] > 1e-300 ? W[0] : 1e-300;
+        const double pp = (g - 1.0) * (W[3] - 0.5 * (W[1] * W[1] + W[2] * W[2]) / r);
+        pm = pp < pm ? pp : pm;
+    }
+#pragma unroll
+    for (int o = 1; o < 4; o <<= 1) {
+        const double ro = __shfl_xor(rm, o, 64), po = __shfl_xor(pm, o, 64);
+        rm = ro < rm ? ro : rm;
+        pm = po < pm ? po : pm;
     }
     *rmin = rm;
     *pmin = pm;
@@ -402,13 +431,10 @@ __device__ uint8_t limit_column(double* col, int s, const double* mw, const doub
             if (p_lo >= eps_p) return fl;
         }
     }
-    double pv[Probes<C>::CACHE ? Probes<C>::NP : 1];
-    if constexpr (Probes<C>::CACHE) {
-#pragma unroll
-        for (int k = 0; k < Probes<C>::NP; ++k) pv[k] = probe_value<C>(col, k, T, P, sidebits);
-    }
+    double V[Probes<C>::NK][4];
+    probe_share<C>(col, s, T, P, sidebits, V);
     double rmin, pmin;
-    probe_minmax<C>(pv, col, ub, 1.0, T, P, sidebits, &rmin, &pmin);
+    probe_minmax<C>(V, ub, 1.0, P.gamma, &rmin, &pmin);
     if (rmin >= eps_rho && pmin >= eps_p) return fl;
     fl |= 2;
     double theta = 1.0;
@@ -417,14 +443,14 @@ __device__ uint8_t limit_column(double* col, int s, const double* mw, const doub
         theta = gap <= 0.0 ? 0.0 : fmin(fmax((ub[0] - eps_rho) / gap, 0.0), 1.0);
     }
     double rr, phi;
-    probe_minmax<C>(pv, col, ub, theta, T, P, sidebits, &rr, &phi);
+    probe_minmax<C>(V, ub, theta, P.gamma, &rr, &phi);
     if (phi < eps_p) {
         double lo = 0.0, hi = theta;
 #pragma unroll 1
         for (int it = 0; it < P.bisect_steps; ++it) {
             const double mid = 0.5 * (lo + hi);
             double pm;
-            probe_minmax<C>(pv, col, ub, mid, T, P, sidebits, &rr, &pm);
+            probe_minmax<C>(V, ub, mid, P.gamma, &rr, &pm);
             if (pm >= eps_p) lo = mid;
             else hi = mid;
         }
@@ -811,6 +837,7 @@ void k_dg(const nz_problem P, const double* __restrict__ T,
     }
     if (P.limiter == 1 && N > 1) {  // superbee runs as separate passes
         uint8_t fl = 0;
+        int scaled = 0;
         if (live) {
             int sidebits = 0;
 #pragma unroll
@@ -818,12 +845,11 @@ void k_dg(const nz_problem P, const double* __restrict__ T,
             double mw[N];
 #pragma unroll
             for (int i = 0; i < N; ++i) mw[i] = C::AFF ? T[P.off.mean_w + i] : P.mean_w[e * N + i];
-            int scaled = 0;
             fl = limit_column<C>(u, s, mw, T, P, sidebits, &scaled);
-            if (s == 0 && counters) {
-                if (scaled) atomicAdd(&counters[0], 1ull);
-                if (fl & 1) atomicAdd(&counters[1], 1ull);
-            }
+        }
+        if (counters) {
+            wave_count(&counters[0], live && s == 0 && scaled);
+            wave_count(&counters[1], live && s == 0 && (fl & 1));
         }
         if (flags && live && s == 0) flags[e] = fl;
     } else if (flags && live && s == 0) {
@@ -887,8 +913,8 @@ __global__ __launch_bounds__(BS) void k_limit(const nz_problem P, const double* 
         int scaled = 0;
         fl = limit_element<C>(U + e * C::NS, 1, mw, tb, side, P, &scaled);
         if (counters) {
-            if (scaled) atomicAdd(&counters[0], 1ull);
-            if (fl & 1) atomicAdd(&counters[1], 1ull);
+            wave_count(&counters[0], scaled != 0);
+            wave_count(&counters[1], (fl & 1) != 0);
         }
     }
     if (flags) flags[e] = fl;
@@ -998,8 +1024,8 @@ __global__ __launch_bounds__(BS) void k_sb_apply(const nz_problem P, const doubl
                 double* x = &U[(e * C::N + i) * 4 + s];
                 *x = ub[s] + theta * (*x - ub[s]);
             }
-        if (counters) atomicAdd(&counters[0], 1ull);
     }
+    if (counters) wave_count(&counters[0], theta < 1.0);
 }
 
 // ---------------------------------------------------------------- adjoint
