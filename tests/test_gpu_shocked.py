@@ -56,8 +56,10 @@ def test_limiter_flags_match_oracle_on_a_shocked_march(shocked):
     _stage(nolim, u, ustar, None)  # the stage's state before limiting
     ulim = torch.empty_like(u)
     flags = torch.zeros(ops.n_elem, dtype=torch.uint8, device=u.device)
+    lim._counters.zero_()
     _stage(lim, u, ulim, flags)  # the fused stage limiter on the same input
     torch.cuda.synchronize()
+    counts = lim._counters.cpu().tolist()
     Ustar = ustar.cpu().numpy()
     ref_U, ref_flags, (n_scaled, n_rep) = positivity_limit(Ustar, ops, flow)
     g_flags = flags.cpu().numpy()
@@ -65,6 +67,8 @@ def test_limiter_flags_match_oracle_on_a_shocked_march(shocked):
           f"scaled {n_scaled}, repaired {n_rep}", flush=True)
     assert n_scaled > 0  # the shock keeps the limiter busy
     assert np.array_equal(g_flags, ref_flags)
+    # the activity counters (summed per wave on the device) count the same elements
+    assert (int(counts[0]), int(counts[1])) == (int(n_scaled), int(n_rep)), counts
     assert rel_err(ulim.cpu().numpy(), ref_U) < 1e-13
     # the standalone device limiter on the same input
     lu, lflags, _ = lim.limit_flags(Ustar)
