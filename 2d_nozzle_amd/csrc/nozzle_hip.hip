@@ -719,7 +719,7 @@ void k_dg(const nz_problem P, const double* __restrict__ T,
     const int vproj = fold ? 3 * N : N;
 #pragma unroll 1
     for (int q0 = 0; q0 < NQV; q0 += L::QR) {
-#pragma unroll 1
+#pragma unroll 2  // two points' table rows in flight
         for (int qq = 0; qq < L::QR; ++qq) {
             if (q0 + qq < NQV) {
                 double a = 0.0;
