@@ -64,13 +64,6 @@ struct Cfg {
     static constexpr int A_FACE = 3 * NQF * 4 * TP;
     static constexpr int AWORK = A_FACE > NS * TP ? (A_FACE > W_VOL ? A_FACE : W_VOL)
                                                  : (NS * TP > W_VOL ? NS * TP : W_VOL);
-    // LDS bytes of one adjoint workgroup (four waves, one per SIMD) and the
-    // waves per SIMD that the CU's 160 KB allows, at most three (168 VGPRs:
-    // the reverse-mode Roe kernel needs about that many)
-    static constexpr int ALDS = 8 * (2 * NS * TP + AWORK + PF + 5 * TE) + 4 * NF * TE;
-    static constexpr int AWPE = (160 * 1024) / ALDS < 1   ? 1
-                                : (160 * 1024) / ALDS > 3 ? 3
-                                                          : (160 * 1024) / ALDS;
 };
 
 // ---------------------------------------------------------------- limiter
@@ -1098,13 +1091,8 @@ __global__ __launch_bounds__(BS) void k_bvjp(const nz_problem P, const double* _
     if (gpb) gpb[b * C::NQF + q] = v[6];
 }
 
-// The register budget of the reverse-mode Roe instantiations is held to the
-// occupancy the LDS allows (three waves per SIMD at P2: no spills, and 19 %
-// faster than the unconstrained allocation's two); the dual-number
-// instantiations need far more registers and are left unconstrained.
 template <class C, int FLUX, bool GEO>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(FLUX == 0 && !GEO ? C::AWPE : 1)))
-void k_vjp(const nz_problem P, const double* __restrict__ T,
+__global__ __launch_bounds__(BS) void k_vjp(const nz_problem P, const double* __restrict__ T,
                                             const double* __restrict__ U,
                                             const double* __restrict__ Lin,
                                             const double* __restrict__ bv,
@@ -1229,16 +1217,7 @@ void k_vjp(const nz_problem P, const double* __restrict__ T,
                         dl[s] = side == 0 ? lo[s] - ln : ln - lo[s];
                     }
                     double g[KD], gj = 0.0;
-                    if constexpr (FLUX == 0 && !GEO) {
-                        // Roe: one reverse-mode pass for the owner's cotangent
-                        double uL[4], uR[4], ob[4];
-                        for (int s = 0; s < 4; ++s) {
-                            uL[s] = side == 0 ? uo[s] : un[s];
-                            uR[s] = side == 0 ? un[s] : uo[s];
-                            ob[s] = jw * dl[s];
-                        }
-                        roe_vjp(uL, uR, eg[0], eg[1], P.gamma, P.efix, side == 0, ob, g);
-                    } else {
+                    {
 #pragma unroll
                     for (int d0 = 0; d0 < KD; d0 += KB) {
                         D ul[4], ur[4], fx[4];
@@ -1453,6 +1432,405 @@ void k_vjp(const nz_problem P, const double* __restrict__ T,
     }
 }
 
+// ------------------------------------------- residual VJP on wave tiles
+// (dR/dU)^T lambda with the Roe flux, state cotangents only, on the mapping
+// of k_dg: one wave per 16-element tile, four lanes per element, each lane
+// holding its element's columns U[:, s] and lambda[:, s] and its accumulator
+// in registers and reading reference tables with scalar loads.  Per face the
+// lanes form the own and neighbour traces of U and lambda in canonical point
+// order; one (point, element) item per lane pulls jw (lambda_own -
+// lambda_nbr) back through the Roe flux with roe_vjp, and the element lifts
+// the result with the side-0 face table.  Boundary faces take the pre-pass
+// values (k_bvjp).  Per volume round the lanes interpolate U and the
+// reference gradients of lambda to four points; one item per lane pulls the
+// contravariant cotangents back through the Euler fluxes (euler_flux_vjp) and
+// the element projects them with phi.  Modes and outputs as k_vjp.
+template <class C>
+struct VTile {
+    static constexpr int N = C::N, NQV = C::NQV, NQF = C::NQF, NF = C::NF, NS = C::NS;
+    static constexpr int QR = DG_BS / TE;
+    static constexpr int FACE = 4 * NQF * QS;  // own U, own lambda, neighbour U, neighbour lambda
+    static constexpr int VOL = 3 * QR * QS;    // U, dlambda/dxi, dlambda/deta at the round's points
+    static constexpr int IO = 2 * TE * NS;     // the U and lambda tiles as in HBM
+    static constexpr int WORK = FACE > VOL ? (FACE > IO ? FACE : IO) : (VOL > IO ? VOL : IO);
+    static constexpr int GF = C::STR ? 3 : 3 * NQF;
+    static constexpr int EGN = TE * NF * GF;
+    static constexpr int GEN = C::AFF ? TE * 5 : 1;
+    // register budget: three waves per SIMD (168 VGPRs) up to P2 where the
+    // LDS allows them (measured 2 % faster than two waves at P2), two above
+    static constexpr int LDS = 8 * (WORK + NF * N * NQF + EGN + GEN + TE) + 8 * TE * NF;
+    static constexpr int WPE = N <= 6 && LDS * 12 <= 160 * 1024 ? 3 : 2;
+};
+
+template <class C>
+__global__ __launch_bounds__(DG_BS) __attribute__((amdgpu_waves_per_eu(VTile<C>::WPE)))
+void k_vjpw(const nz_problem P, const double* __restrict__ T,
+                                              const double* __restrict__ U,
+                                              const double* __restrict__ Lin,
+                                              const double* __restrict__ bv,
+                                              double* __restrict__ out, int mode,
+                                              const double* __restrict__ seed,
+                                              const double* __restrict__ Lbase,
+                                              const double* __restrict__ dt,
+                                              double* __restrict__ acc_rk,
+                                              double* __restrict__ Lnext, int rk_stage) {
+    using L = VTile<C>;
+    constexpr int N = C::N, NQV = C::NQV, NQF = C::NQF, NF = C::NF, NS = C::NS;
+    constexpr int NPF = NF * N * NQF;
+    constexpr int NU = TE * NS / DG_BS;  // = N
+    __shared__ double sW[L::WORK];
+    __shared__ double sPF[NPF];
+    __shared__ double sEG[L::EGN];
+    __shared__ double sGe[L::GEN];
+    __shared__ double sDt[TE];
+    __shared__ int sNb[TE * NF], sIn[TE * NF];
+
+    const int t = threadIdx.x;
+    const int el = t >> 2, s = t & 3;
+    int64_t e0;
+    int ne;
+    if (P.vjp_tiles) {
+        e0 = P.vjp_tiles[2 * blockIdx.x];
+        ne = static_cast<int>(P.vjp_tiles[2 * blockIdx.x + 1] - e0);
+    } else {
+        e0 = static_cast<int64_t>(blockIdx.x) * TE;
+        ne = static_cast<int>(P.n_elem - e0 < TE ? P.n_elem - e0 : TE);
+    }
+    const bool live = el < ne;
+    const int64_t e = e0 + el;
+    const int cs = at(0, s, el);
+
+    // ---- stage U and lambda (coalesced), the maps, the face table, the metrics
+    {
+        constexpr int NM = (TE * NF + DG_BS - 1) / DG_BS;
+        constexpr int NT = (NPF + DG_BS - 1) / DG_BS;
+        constexpr int NG = (L::GEN + DG_BS - 1) / DG_BS;
+        double ru[NU], rl[NU], rt[NT], rg[NG];
+        int rn[NM], ri[NM];
+#pragma unroll
+        for (int j = 0; j < NU; ++j) {
+            const int it = t + j * DG_BS, r = it / NS;
+            ru[j] = r < ne ? U[e0 * NS + it] : 0.0;
+            rl[j] = r < ne ? Lin[e0 * NS + it] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < NM; ++j) {
+            const int it = t + j * DG_BS;
+            const bool ok = it < ne * NF;
+            rn[j] = ok ? P.fe_nbr[e0 * NF + it] : -1;
+            ri[j] = ok ? P.fe_info[e0 * NF + it] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+            const int it = t + j * DG_BS;
+            rt[j] = it < NPF ? T[P.off.phi_face + it] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < NG; ++j) {
+            const int it = t + j * DG_BS;
+            rg[j] = C::AFF && it < ne * 5 ? P.geo[e0 * 5 + it] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < NU; ++j) {
+            sW[t + j * DG_BS] = ru[j];
+            sW[TE * NS + t + j * DG_BS] = rl[j];
+        }
+#pragma unroll
+        for (int j = 0; j < NM; ++j) {
+            const int it = t + j * DG_BS;
+            if (it < TE * NF) {
+                sNb[it] = rn[j];
+                sIn[it] = ri[j];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NT; ++j)
+            if (t + j * DG_BS < NPF) sPF[t + j * DG_BS] = rt[j];
+#pragma unroll
+        for (int j = 0; j < NG; ++j)
+            if (t + j * DG_BS < L::GEN) sGe[t + j * DG_BS] = rg[j];
+    }
+    wave_sync();
+    {
+        constexpr int NE_ = (L::EGN + DG_BS - 1) / DG_BS;
+        double rq[NE_];
+#pragma unroll
+        for (int j = 0; j < NE_; ++j) {
+            const int it = t + j * DG_BS;
+            const int r = it / L::GF, jj = it - r * L::GF;
+            const bool ok = it < L::EGN && r < ne * NF;
+            const int64_t k = ok ? (sIn[r] >> 3) : 0;
+            rq[j] = ok ? P.edge_geo[k * L::GF + jj] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < NE_; ++j)
+            if (t + j * DG_BS < L::EGN) sEG[t + j * DG_BS] = rq[j];
+    }
+    double u[N], l[N], acc[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        u[i] = sW[el * NS + i * 4 + s];
+        l[i] = sW[TE * NS + el * NS + i * 4 + s];
+        acc[i] = 0.0;
+    }
+    wave_sync();
+
+    double* sUo = sW;                 // own U trace, then the pulled-back cotangent
+    double* sLo = sW + NQF * QS;      // own lambda trace
+    double* sUn = sW + 2 * NQF * QS;  // neighbour U trace
+    double* sLn = sW + 3 * NQF * QS;  // neighbour lambda trace
+
+    // ---------------------------------------------------------------- faces
+#pragma unroll 1
+    for (int f = 0; f < NF; ++f) {
+        const int nb = sNb[el * NF + f], inf = sIn[el * NF + f];
+        const bool has = live && nb >= 0;
+        double xn[N], yn[N];
+        {
+            const int64_t base = (has ? static_cast<int64_t>(nb) : e0) * NS + s;
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                xn[i] = U[base + i * 4];
+                yn[i] = Lin[base + i * 4];
+            }
+        }
+        {
+            const double* b = T + P.off.phi_face + f * N * NQF;
+#pragma unroll
+            for (int q = 0; q < NQF; ++q) {
+                double a = 0.0, c = 0.0;
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    a += b[i * NQF + q] * u[i];
+                    c += b[i * NQF + q] * l[i];
+                }
+                sUo[q * QS + cs] = a;
+                sLo[q * QS + cs] = c;
+            }
+        }
+        {
+            double tu[NQF], tl[NQF];
+#pragma unroll
+            for (int q = 0; q < NQF; ++q) tu[q] = tl[q] = 0.0;
+            const int key = has ? (inf & 3) : -1;
+            const int key0 = __builtin_amdgcn_readfirstlane(key);
+            if (__all(key == key0) && key0 >= 0) {
+                const double* ph = T + P.off.phi_face + key0 * N * NQF;
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+#pragma unroll
+                    for (int q = 0; q < NQF; ++q) {
+                        tu[q] += ph[i * NQF + q] * xn[i];
+                        tl[q] += ph[i * NQF + q] * yn[i];
+                    }
+                }
+            } else if (has) {
+                const double* ph = sPF + key * N * NQF;
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+#pragma unroll
+                    for (int q = 0; q < NQF; ++q) {
+                        tu[q] += ph[i * NQF + q] * xn[i];
+                        tl[q] += ph[i * NQF + q] * yn[i];
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < NQF; ++q) {
+                sUn[q * QS + cs] = tu[q];
+                sLn[q * QS + cs] = tl[q];
+            }
+        }
+        wave_sync();
+        // (q,e): the cotangent at edge point q, one item per lane
+        for (int it = t; it < NQF * TE; it += DG_BS) {
+            const int q = it / TE, pe = it % TE;
+            if (pe < ne) {
+                const int nbp = sNb[pe * NF + f], infp = sIn[pe * NF + f];
+                const int sd = (infp >> 2) & 1;
+                const int64_t k = infp >> 3;
+                const int qo = sd ? NQF - 1 - q : q;  // this element's canonical index
+                double v[4];
+                if (nbp >= 0) {
+                    const int qn = NQF - 1 - qo;
+                    const double* eg = sEG + (pe * NF + f) * L::GF + (C::STR ? 0 : q * 3);
+                    const double jw = eg[2] * T[P.off.w_face + q];
+                    double uL[4], uR[4], ob[4];
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        const double uo = sUo[at(qo, c, pe)], un = sUn[at(qn, c, pe)];
+                        const double lo = sLo[at(qo, c, pe)], ln = sLn[at(qn, c, pe)];
+                        uL[c] = sd == 0 ? uo : un;
+                        uR[c] = sd == 0 ? un : uo;
+                        ob[c] = jw * (sd == 0 ? lo - ln : ln - lo);
+                    }
+                    roe_vjp(uL, uR, eg[0], eg[1], P.gamma, P.efix, sd == 0, ob, v);
+                } else {  // boundary faces: the pre-pass's values
+                    const double* b = bv + ((k - P.n_int) * NQF + q) * 4;
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) v[c] = b[c];
+                }
+#pragma unroll
+                for (int c = 0; c < 4; ++c) sUo[at(qo, c, pe)] = v[c];
+            }
+        }
+        wave_sync();
+        {
+            double fw[NQF];
+#pragma unroll
+            for (int q = 0; q < NQF; ++q) fw[q] = sUo[q * QS + cs];
+            const double* lb = T + P.off.phi_face + f * N * NQF;
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+#pragma unroll
+                for (int q = 0; q < NQF; ++q) acc[i] += lb[i * NQF + q] * fw[q];
+            }
+        }
+        // the next face rewrites these rows after the reads above
+    }
+    wave_sync();
+
+    // --------------------------------------------------------------- volume
+    {
+        double* sQu = sW;                   // U at the round's points, then the cotangent
+        double* sQx = sW + L::QR * QS;      // dlambda/dxi
+        double* sQy = sW + 2 * L::QR * QS;  // dlambda/deta
+        const double* vt = T + P.off.vol_t;  // per point: phi, dxi, deta, ... (N each)
+#pragma unroll 1
+        for (int q0 = 0; q0 < NQV; q0 += L::QR) {
+#pragma unroll 1
+            for (int qq = 0; qq < L::QR; ++qq) {
+                if (q0 + qq < NQV) {
+                    const double* ph = vt + (q0 + qq) * 5 * N;
+                    double a = 0.0, bx = 0.0, by = 0.0;
+#pragma unroll
+                    for (int i = 0; i < N; ++i) {
+                        a += ph[i] * u[i];
+                        bx += ph[N + i] * l[i];
+                        by += ph[2 * N + i] * l[i];
+                    }
+                    sQu[qq * QS + cs] = a;
+                    sQx[qq * QS + cs] = bx;
+                    sQy[qq * QS + cs] = by;
+                }
+            }
+            wave_sync();
+            {
+                const int qq = t / TE, pe = t % TE, q = q0 + qq;
+                if (q < NQV && pe < ne) {
+                    double uq[4], fb[4], gb[4], ub[4];
+                    double xx, xy, ex, ey, wd;
+                    if (C::AFF) {
+                        const double* ge = sGe + pe * 5;
+                        xx = ge[0];
+                        xy = ge[1];
+                        ex = ge[2];
+                        ey = ge[3];
+                        wd = T[P.off.w_vol + q] * ge[4];
+                    } else {
+                        const double* gq = P.geo + ((e0 + pe) * NQV + q) * 5;
+                        xx = gq[0];
+                        xy = gq[1];
+                        ex = gq[2];
+                        ey = gq[3];
+                        wd = gq[4];
+                    }
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        uq[c] = sQu[at(qq, c, pe)];
+                        const double ax = sQx[at(qq, c, pe)], ay = sQy[at(qq, c, pe)];
+                        fb[c] = wd * (ax * xx + ay * ex);
+                        gb[c] = wd * (ax * xy + ay * ey);
+                    }
+                    euler_flux_vjp(uq, P.gamma, fb, gb, ub);
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) sQu[at(qq, c, pe)] = -ub[c];
+                }
+            }
+            wave_sync();
+#pragma unroll 1
+            for (int qq = 0; qq < L::QR; ++qq) {
+                const int q = q0 + qq;
+                if (q < NQV) {
+                    const double a = sQu[qq * QS + cs];
+                    const double* ph = vt + q * 5 * N;
+#pragma unroll
+                    for (int i = 0; i < N; ++i) acc[i] += ph[i] * a;
+                }
+            }
+            // the next round rewrites these rows after the reads above
+        }
+    }
+    wave_sync();
+
+    // ------------------------------------------------------------ epilogue
+    if (mode == kAdjVjp) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) sW[el * NS + i * 4 + s] = acc[i];
+        wave_sync();
+        if (ne == TE) {
+#pragma unroll
+            for (int j = 0; j < NU; ++j) out[e0 * NS + t + j * DG_BS] = sW[t + j * DG_BS];
+        } else {
+            for (int it = t; it < ne * NS; it += DG_BS) out[e0 * NS + it] = sW[it];
+        }
+        return;
+    }
+    {  // the seed tile, coalesced
+        double rs[NU];
+#pragma unroll
+        for (int j = 0; j < NU; ++j) {
+            const int it = t + j * DG_BS;
+            rs[j] = it / NS < ne ? seed[e0 * NS + it] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < NU; ++j) sW[TE * NS + t + j * DG_BS] = rs[j];
+        if (s == 0) sDt[el] = live && mode == kAdjRk ? dt[e] : 0.0;
+    }
+    wave_sync();
+    double d[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) d[i] = acc[i] - sW[TE * NS + el * NS + i * 4 + s];
+    if (mode == kAdjDefect) {  // partial sums of (vjp - seed)^2, one per tile
+        double sq = 0.0;
+        if (live)
+            for (int i = 0; i < N; ++i) sq += d[i] * d[i];
+        for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o, 64);
+        if (t == 0) out[blockIdx.x] = sq;
+        return;
+    }
+    // adjoint pseudo-time rate -M^-1 (vjp - seed) and one classical RK4
+    // stage (src/sensitivity.py:344-357), the update in coalesced rows
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        double r = 0.0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            const double m = C::AFF ? T[P.off.minv_ref + i * N + j] / sGe[el * 5 + 4]
+                                    : (live ? P.minv[(e * N + i) * N + j] : 0.0);
+            r += m * d[j];
+        }
+        sW[el * NS + i * 4 + s] = -r;
+    }
+    wave_sync();
+    for (int it = t; it < ne * NS; it += DG_BS) {
+        const int64_t x = e0 * NS + it;
+        const double kv = sW[it], h = sDt[it / NS], lb = Lbase[x];
+        if (rk_stage == 1) {
+            acc_rk[x] = kv;
+            Lnext[x] = lb + 0.5 * h * kv;
+        } else if (rk_stage == 2) {
+            acc_rk[x] = acc_rk[x] + 2.0 * kv;
+            Lnext[x] = lb + 0.5 * h * kv;
+        } else if (rk_stage == 3) {
+            acc_rk[x] = acc_rk[x] + 2.0 * kv;
+            Lnext[x] = lb + h * kv;
+        } else {
+            Lnext[x] = lb + h / 6.0 * (acc_rk[x] + kv);
+        }
+    }
+}
+
 // ------------------------------------------------------------ small kernels
 template <class C>
 __global__ __launch_bounds__(BS) void k_inverse_mass(const nz_problem P, const double* __restrict__ T,
@@ -1633,8 +2011,16 @@ struct Launch {
             hipLaunchKernelGGL((nz::k_bvjp<C>), dim3(static_cast<unsigned>((nbq + nz::BS - 1) / nz::BS)),
                                dim3(nz::BS), 0, st, P, P.tables, U, L, bv, gedge, gpb);
         if (nblk <= 0) return;  // an empty tile table: nothing to march
-        const dim3 g(static_cast<unsigned>(nblk));
         const bool geo = gmet || gedge;  // geometry cotangents: the normal is differentiated too
+        if (P.flux_id == 0 && !geo) {  // Roe, state cotangents: the wave-tile kernel
+            hipLaunchKernelGGL((nz::k_vjpw<C>), dim3(static_cast<unsigned>(nblk)), dim3(nz::DG_BS), 0,
+                               st, P, P.tables, U, L, bv, out, mode, seed, Lbase, dt, acc, Lnext,
+                               stage);
+            return;
+        }
+        // its own grid: entries of a tile table, or tiles of C::TE elements
+        const dim3 g(static_cast<unsigned>(P.vjp_tiles ? P.n_vjp_tiles
+                                                       : (P.n_elem + C::TE - 1) / C::TE));
 #define NZ_VJP(FL)                                                                                \
     do {                                                                                          \
         if (geo)                                                                                  \
@@ -1646,7 +2032,9 @@ struct Launch {
     } while (0)
         if (P.flux_id == 1) NZ_VJP(1);
         else if (P.flux_id == 2) NZ_VJP(2);
-        else NZ_VJP(0);
+        else  // Roe with geometry cotangents (state-only Roe went to k_vjpw)
+            hipLaunchKernelGGL((nz::k_vjp<C, 0, true>), g, dim3(nz::BS), 0, st, P, P.tables, U, L,
+                               bv, out, gmet, gedge, mode, seed, Lbase, dt, acc, Lnext, stage);
 #undef NZ_VJP
     }
     static Ops ops() { return Ops{&dg, &limit, &inverse_mass, &superbee, &vjp, C::TE}; }
@@ -1699,8 +2087,11 @@ int resolve(const nz_problem* p, Ops* ops) {
 }
 
 int64_t n_blocks(const nz_problem* p, const Ops& o) { return (p->n_elem + o.te - 1) / o.te; }
+// elements per workgroup of the state-cotangent VJP launches: the wave-tile
+// kernel's 16 for Roe, the four-wave kernel's tile otherwise
+int64_t vjp_te(const nz_problem* p, const Ops& o) { return p->flux_id == 0 ? nz::TE : o.te; }
 int64_t n_blocks_vjp(const nz_problem* p, const Ops& o) {
-    return p->vjp_tiles ? p->n_vjp_tiles : n_blocks(p, o);
+    return p->vjp_tiles ? p->n_vjp_tiles : (p->n_elem + vjp_te(p, o) - 1) / vjp_te(p, o);
 }
 int64_t n_blocks_dg(const nz_problem* p) {
     return p->tiles ? p->n_tiles : (p->n_elem + nz::TE - 1) / nz::TE;
@@ -1724,7 +2115,7 @@ int64_t nz_scratch_doubles(const nz_problem* p) {
 int64_t nz_adjoint_tile_elements(const nz_problem* p) {
     Ops o;
     if (resolve(p, &o) != NZ_OK) return -1;
-    return o.te;
+    return vjp_te(p, o);
 }
 
 int nz_residual(const nz_problem* p, const double* U, double* R, double* wave, void* stream) {
