@@ -326,22 +326,26 @@ struct Probes {
 
 // The probe states of this lane's share, formed once per limited element:
 // every lane sums its own component at each probe (no contraction), and the
-// lane that owns probe k (k mod 4) collects the four components.
+// lane that owns probe k (k mod 4) collects the four components.  The probe
+// loop is not unrolled, so one probe's table column is in scalar registers at
+// a time; the store into the share goes through a wave-uniform test per slot.
 template <class C>
 __device__ __forceinline__ void probe_share(const double* col, int s, const double* __restrict__ T,
                                             const nz_problem& P, int sidebits,
                                             double V[Probes<C>::NK][4]) {
     constexpr int NP = Probes<C>::NP;
-#pragma unroll
+#pragma unroll 1
     for (int k = 0; k < NP; ++k) {
         const double v = probe_value<C>(col, k, T, P, sidebits);
+        double x[4];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const double x = quad_get(v, c);
-            if ((k & 3) == s) V[k >> 2][c] = x;
-        }
-        // one probe's table column in scalar registers at a time
-        __builtin_amdgcn_sched_barrier(0);
+        for (int c = 0; c < 4; ++c) x[c] = quad_get(v, c);
+        const bool mine = (k & 3) == s;
+        const int kk = k >> 2;
+#pragma unroll
+        for (int j = 0; j < Probes<C>::NK; ++j)
+            if (j == kk && mine)
+                for (int c = 0; c < 4; ++c) V[j][c] = x[c];
     }
 #pragma unroll
     for (int kk = 0; kk < Probes<C>::NK; ++kk)
