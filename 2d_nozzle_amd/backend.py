@@ -77,6 +77,10 @@ def _tables(rd) -> tuple[np.ndarray, dict]:
 TILE = 16                      # elements per tile of the element kernel
 XCDS = 8                       # L2 domains a launch is dealt round-robin across
 LOCALITY_MIN_ELEMENTS = 1 << 14
+# (element kind, geometry order, order) -> the residual-VJP tile shape the
+# autotuner measured fastest with the Roe flux in this process (0 one-wave
+# tiles of 16 elements, 1 four-wave tiles); see HipBackend.tune_vjp
+_VJP_KERNEL: dict = {}
 
 
 def _morton2(ix: np.ndarray, iy: np.ndarray) -> np.ndarray:
@@ -248,6 +252,8 @@ class HipBackend:
         P.bisect_steps = 12
         for s in range(4):
             P.wnorm[s] = w[s]
+        self._vjp_key = (topo.kind, int(getattr(topo, "geometry_order", 1)), int(rd.order))
+        P.vjp_kernel = _VJP_KERNEL.get(self._vjp_key, 0) if P.flux_id == 0 else 0
         self.P = P
         self._pp = C.byref(P)
 
@@ -520,6 +526,47 @@ class HipBackend:
             return self._out(dU, npy)
         return (self._out(dU, npy), self._out(gm, npy), self._out(ge, npy), float(gp.sum().item()))
 
+    def tune_vjp(self, U, lam) -> int:
+        """Autotuner for the residual-VJP tile shape with the Roe flux: times
+        the state-cotangent VJP on this mesh with one-wave tiles of 16 elements
+        and with four-wave tiles of 64 (128 when N <= 4), keeps the faster for
+        this (element kind, geometry order, order) for the rest of the process
+        (so every backend of a family, batched or alone, uses the same one),
+        and returns it (0 or 1).  The two agree to round-off."""
+        if self.P.flux_id != 0:
+            return 0
+        if self._vjp_key in _VJP_KERNEL:
+            self.P.vjp_kernel = _VJP_KERNEL[self._vjp_key]
+            return self.P.vjp_kernel
+        torch = self._torch
+        u, _ = self._in(U)
+        lm, _ = self._in(lam)
+        a = self._adj_buffers()
+        out = torch.empty_like(u)
+        st = torch.cuda.current_stream(self.device)
+        cost = []
+        for choice in (0, 1):
+            self.P.vjp_kernel = choice
+
+            def vjp():
+                self._chk(self.lib.nz_residual_vjp(self._pp, self._p(u), self._p(lm), self._p(out),
+                                                   None, None, None, self._p(a["work"]),
+                                                   self._stream()), "nz_residual_vjp")
+
+            for _ in range(2):
+                vjp()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            for _ in range(5):
+                vjp()
+            e1.record(st)
+            e1.synchronize()
+            cost.append(e0.elapsed_time(e1))
+        best = int(cost[1] < cost[0])
+        _VJP_KERNEL[self._vjp_key] = best
+        self.P.vjp_kernel = best
+        return best
+
     def _frozen_step(self, u):
         """The adjoint's local pseudo-time step: the forward step of the
         converged state with the ``ssprk3`` calibration and without the
@@ -565,6 +612,7 @@ class HipBackend:
         torch = self._torch
         u, npy = self._in(U)
         s, _ = self._in(seed)
+        self.tune_vjp(u, s)
         a = self._frozen_step(u)
         st = self._stream()
         if method == "gmres":

@@ -476,9 +476,10 @@ def batched_gradients(results: list, flow: FlowConditions, opts: SolverOptions, 
         dev = bk.device
         f.parts = [objective_and_partials(results[g].U, results[g].operators, flow, objective)
                    for g in idx]
+        f.eoff = eoff = f.cat.elem_offset
         f.u = torch.from_numpy(np.concatenate([results[g].U for g in idx])).to(dev)
         f.s = torch.from_numpy(np.concatenate([p[1] for p in f.parts])).to(dev)
-        f.eoff = eoff = f.cat.elem_offset
+        bk.tune_vjp(f.u, f.s)  # the tile shape fixes the tile table's width
         f.te = int(bk.lib.nz_adjoint_tile_elements(bk._pp))
         f.sl = [slice(int(eoff[j]), int(eoff[j + 1])) for j in range(len(idx))]
         # per-design sums over contiguous element segments as a padded gather

@@ -64,6 +64,12 @@ struct Cfg {
     static constexpr int A_FACE = 3 * NQF * 4 * TP;
     static constexpr int AWORK = A_FACE > NS * TP ? (A_FACE > W_VOL ? A_FACE : W_VOL)
                                                  : (NS * TP > W_VOL ? NS * TP : W_VOL);
+    // LDS bytes of one adjoint workgroup (four waves, one per SIMD) and the
+    // waves per SIMD that the CU's 160 KB allows, at most three
+    static constexpr int ALDS = 8 * (2 * NS * TP + AWORK + PF + 5 * TE) + 4 * NF * TE;
+    static constexpr int AWPE = (160 * 1024) / ALDS < 1   ? 1
+                                : (160 * 1024) / ALDS > 3 ? 3
+                                                          : (160 * 1024) / ALDS;
 };
 
 // ---------------------------------------------------------------- limiter
@@ -1095,8 +1101,13 @@ __global__ __launch_bounds__(BS) void k_bvjp(const nz_problem P, const double* _
     if (gpb) gpb[b * C::NQF + q] = v[6];
 }
 
+// The register budget of the reverse-mode Roe instantiation is held to the
+// occupancy the LDS allows (three waves per SIMD at P2: no spills, and 19 %
+// faster than the unconstrained allocation's two); the dual-number
+// instantiations need far more registers and are left unconstrained.
 template <class C, int FLUX, bool GEO>
-__global__ __launch_bounds__(BS) void k_vjp(const nz_problem P, const double* __restrict__ T,
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(FLUX == 0 && !GEO ? C::AWPE : 1)))
+void k_vjp(const nz_problem P, const double* __restrict__ T,
                                             const double* __restrict__ U,
                                             const double* __restrict__ Lin,
                                             const double* __restrict__ bv,
@@ -1221,7 +1232,16 @@ __global__ __launch_bounds__(BS) void k_vjp(const nz_problem P, const double* __
                         dl[s] = side == 0 ? lo[s] - ln : ln - lo[s];
                     }
                     double g[KD], gj = 0.0;
-                    {
+                    if constexpr (FLUX == 0 && !GEO) {
+                        // Roe: one reverse-mode pass for the owner's cotangent
+                        double uL[4], uR[4], ob[4];
+                        for (int s = 0; s < 4; ++s) {
+                            uL[s] = side == 0 ? uo[s] : un[s];
+                            uR[s] = side == 0 ? un[s] : uo[s];
+                            ob[s] = jw * dl[s];
+                        }
+                        roe_vjp(uL, uR, eg[0], eg[1], P.gamma, P.efix, side == 0, ob, g);
+                    } else {
 #pragma unroll
                     for (int d0 = 0; d0 < KD; d0 += KB) {
                         D ul[4], ur[4], fx[4];
@@ -2016,7 +2036,7 @@ struct Launch {
                                dim3(nz::BS), 0, st, P, P.tables, U, L, bv, gedge, gpb);
         if (nblk <= 0) return;  // an empty tile table: nothing to march
         const bool geo = gmet || gedge;  // geometry cotangents: the normal is differentiated too
-        if (P.flux_id == 0 && !geo) {  // Roe, state cotangents: the wave-tile kernel
+        if (P.flux_id == 0 && !geo && P.vjp_kernel == 0) {  // Roe, state cotangents: wave tiles
             hipLaunchKernelGGL((nz::k_vjpw<C>), dim3(static_cast<unsigned>(nblk)), dim3(nz::DG_BS), 0,
                                st, P, P.tables, U, L, bv, out, mode, seed, Lbase, dt, acc, Lnext,
                                stage);
@@ -2036,9 +2056,7 @@ struct Launch {
     } while (0)
         if (P.flux_id == 1) NZ_VJP(1);
         else if (P.flux_id == 2) NZ_VJP(2);
-        else  // Roe with geometry cotangents (state-only Roe went to k_vjpw)
-            hipLaunchKernelGGL((nz::k_vjp<C, 0, true>), g, dim3(nz::BS), 0, st, P, P.tables, U, L,
-                               bv, out, gmet, gedge, mode, seed, Lbase, dt, acc, Lnext, stage);
+        else NZ_VJP(0);
 #undef NZ_VJP
     }
     static Ops ops() { return Ops{&dg, &limit, &inverse_mass, &superbee, &vjp, C::TE}; }
@@ -2087,13 +2105,16 @@ int resolve(const nz_problem* p, Ops* ops) {
         return fail(NZ_E_ARG, "n_tiles out of range");
     if (p->vjp_tiles && (p->n_vjp_tiles < 0 || p->n_vjp_tiles > (int64_t{1} << 31)))
         return fail(NZ_E_ARG, "n_vjp_tiles out of range");
+    if (p->vjp_kernel != 0 && p->vjp_kernel != 1) return fail(NZ_E_ARG, "vjp_kernel must be 0 or 1");
     return NZ_OK;
 }
 
 int64_t n_blocks(const nz_problem* p, const Ops& o) { return (p->n_elem + o.te - 1) / o.te; }
 // elements per workgroup of the state-cotangent VJP launches: the wave-tile
 // kernel's 16 for Roe, the four-wave kernel's tile otherwise
-int64_t vjp_te(const nz_problem* p, const Ops& o) { return p->flux_id == 0 ? nz::TE : o.te; }
+int64_t vjp_te(const nz_problem* p, const Ops& o) {
+    return p->flux_id == 0 && p->vjp_kernel == 0 ? nz::TE : o.te;
+}
 int64_t n_blocks_vjp(const nz_problem* p, const Ops& o) {
     return p->vjp_tiles ? p->n_vjp_tiles : (p->n_elem + vjp_te(p, o) - 1) / vjp_te(p, o);
 }

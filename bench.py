@@ -340,6 +340,9 @@ def sweep_bench(n: int, tol: float, n_lone: int, max_it: int = 50000) -> dict:
         "lone_converged_gradients_per_s": round(lone_grad_rate, 4),
         "gradient_speedup_converged_only": round(grad_rate / lone_grad_rate, 2),
         "lone_gradient_max_rel_diff": worst,
+        # the residual-VJP tile shape the autotuner kept per (kind, geometry order, order)
+        "adjoint_tile_shapes": {f"{k[0]}-q{k[1]}-p{k[2]}": ("wave16" if v == 0 else "block")
+                                for k, v in sorted(_pkg("backend")._VJP_KERNEL.items())},
     }
 
 

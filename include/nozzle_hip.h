@@ -98,6 +98,12 @@ typedef struct {
      * march. */
     const int64_t *vjp_tiles;
     int64_t n_vjp_tiles;
+    /* the state-cotangent VJP kernel with the Roe flux: 0 one-wave tiles of
+     * 16 elements (the element kernel's mapping), 1 four-wave tiles of 64 (or
+     * 128 when N <= 4); the two agree to round-off.  HipBackend picks it per
+     * (element kind, order, geometry order) by timing both. */
+    int32_t vjp_kernel;
+    int32_t pad1;
 } nz_problem;
 
 /* Number of scratch doubles nz_* calls need for norm partials. */

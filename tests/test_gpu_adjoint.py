@@ -152,3 +152,22 @@ def test_gmres_adjoint_matches_the_march():
     for k, v in captured.items():
         assert abs(grad_gm[k] - v) <= 6e-7 * abs(v) + 1e-12, (k, grad_gm[k], v)
         assert abs(grad_gm[k] - grad_rk[k]) <= 1e-6 * abs(grad_rk[k]), (k, grad_gm[k], grad_rk[k])
+
+
+@pytest.mark.parametrize("name,key", [("synth_p3", "U"), ("quad_q2_p2", "U0"), ("cfg1", "Ur")])
+def test_vjp_tile_shapes_agree(name, key):
+    """The two residual-VJP tile shapes (one-wave tiles of 16 elements, four-wave
+    tiles) give the same Roe VJP to round-off, and the autotuner picks one."""
+    g = golden(name)
+    ops, flow = build_case(name, dense=False)
+    bk = backend(ops, flow.replace(flux="roe"))
+    U = g[key]
+    lam = np.random.default_rng(5).standard_normal(U.shape)
+    outs = []
+    for choice in (0, 1):
+        bk.P.vjp_kernel = choice
+        outs.append(np.asarray(bk.residual_vjp(U, lam)))
+    err = float(np.abs(outs[0] - outs[1]).max() / np.abs(outs[1]).max())
+    print(name, "tile shapes differ by", err)
+    assert err <= 1e-12
+    assert bk.tune_vjp(U, lam) in (0, 1)
