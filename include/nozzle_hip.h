@@ -109,9 +109,11 @@ typedef struct {
 /* Number of scratch doubles nz_* calls need for norm partials. */
 int64_t nz_scratch_doubles(const nz_problem *p);
 
-/* Elements per workgroup of the adjoint kernels: the widest entry a
- * vjp_tiles table may hold.  No counterpart in the reference (its adjoint is
- * a JAX vjp over the whole mesh, src/sensitivity.py:317). */
+/* Elements per workgroup of the state-cotangent adjoint launches: the
+ * widest entry a vjp_tiles table may hold (16 with the Roe flux and
+ * vjp_kernel 0, else the four-wave tile of 64, or 128 when N <= 4).  No
+ * counterpart in the reference (its adjoint is a JAX vjp over the whole mesh,
+ * src/sensitivity.py:317). */
 int64_t nz_adjoint_tile_elements(const nz_problem *p);
 
 /* R(U) and the wave-speed sum, unfolded.
