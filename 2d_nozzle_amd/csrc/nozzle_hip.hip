@@ -516,6 +516,9 @@ void k_dg(const nz_problem P, const double* __restrict__ T,
     const bool fold = C::AFF && (STAGE || mode != kRes);  // affine rate: M^-1 folded into tables
     const int cs = at(0, s, el);  // this lane's slot at point 0
 
+    // A wave staging its tile runs at raised priority, so its loads are issued
+    // ahead of the other waves' arithmetic and their latency overlaps it.
+    __builtin_amdgcn_s_setprio(2);
     // ---- stage the tile (coalesced), the face maps, the face table and the
     // affine metrics; every load is issued before the first LDS write
     {
@@ -587,6 +590,7 @@ void k_dg(const nz_problem P, const double* __restrict__ T,
         u[i] = sW[el * L::SIO + i * 4 + s];
         acc[i] = 0.0;
     }
+    __builtin_amdgcn_s_setprio(0);
     double wave = 0.0;
     wave_sync();  // the staging rows are reused below
 
