@@ -1528,6 +1528,7 @@ void k_vjpw(const nz_problem P, const double* __restrict__ T,
     const int64_t e = e0 + el;
     const int cs = at(0, s, el);
 
+    __builtin_amdgcn_s_setprio(2);  // staging at raised priority, as in k_dg
     // ---- stage U and lambda (coalesced), the maps, the face table, the metrics
     {
         constexpr int NM = (TE * NF + DG_BS - 1) / DG_BS;
@@ -1594,6 +1595,7 @@ void k_vjpw(const nz_problem P, const double* __restrict__ T,
         for (int j = 0; j < NE_; ++j)
             if (t + j * DG_BS < L::EGN) sEG[t + j * DG_BS] = rq[j];
     }
+    __builtin_amdgcn_s_setprio(0);
     double u[N], l[N], acc[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) {
