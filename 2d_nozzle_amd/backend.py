@@ -252,8 +252,9 @@ class HipBackend:
         P.bisect_steps = 12
         for s in range(4):
             P.wnorm[s] = w[s]
-        self._vjp_key = (topo.kind, int(getattr(topo, "geometry_order", 1)), int(rd.order))
-        P.vjp_kernel = _VJP_KERNEL.get(self._vjp_key, 0) if P.flux_id == 0 else 0
+        self._vjp_key = (topo.kind, int(getattr(topo, "geometry_order", 1)), int(rd.order),
+                         int(P.flux_id))
+        P.vjp_kernel = _VJP_KERNEL.get(self._vjp_key, 0)
         self.P = P
         self._pp = C.byref(P)
 
@@ -527,14 +528,12 @@ class HipBackend:
         return (self._out(dU, npy), self._out(gm, npy), self._out(ge, npy), float(gp.sum().item()))
 
     def tune_vjp(self, U, lam) -> int:
-        """Autotuner for the residual-VJP tile shape with the Roe flux: times
-        the state-cotangent VJP on this mesh with one-wave tiles of 16 elements
-        and with four-wave tiles of 64 (128 when N <= 4), keeps the faster for
-        this (element kind, geometry order, order) for the rest of the process
-        (so every backend of a family, batched or alone, uses the same one),
-        and returns it (0 or 1).  The two agree to round-off."""
-        if self.P.flux_id != 0:
-            return 0
+        """Autotuner for the residual-VJP tile shape: times the state-cotangent
+        VJP on this mesh with one-wave tiles of 16 elements and with four-wave
+        tiles of 64 (128 when N <= 4), keeps the faster for this (element kind,
+        geometry order, order, flux) for the rest of the process (so every
+        backend of a family, batched or alone, uses the same one), and returns
+        it (0 or 1).  The two agree to round-off."""
         if self._vjp_key in _VJP_KERNEL:
             self.P.vjp_kernel = _VJP_KERNEL[self._vjp_key]
             return self.P.vjp_kernel

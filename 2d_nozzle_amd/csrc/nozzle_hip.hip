@@ -1461,14 +1461,16 @@ void k_vjp(const nz_problem P, const double* __restrict__ T,
 }
 
 // ------------------------------------------- residual VJP on wave tiles
-// (dR/dU)^T lambda with the Roe flux, state cotangents only, on the mapping
-// of k_dg: one wave per 16-element tile, four lanes per element, each lane
-// holding its element's columns U[:, s] and lambda[:, s] and its accumulator
-// in registers and reading reference tables with scalar loads.  Per face the
-// lanes form the own and neighbour traces of U and lambda in canonical point
-// order; one (point, element) item per lane pulls jw (lambda_own -
-// lambda_nbr) back through the Roe flux with roe_vjp, and the element lifts
-// the result with the side-0 face table.  Boundary faces take the pre-pass
+// (dR/dU)^T lambda, state cotangents only, on the mapping of k_dg: one wave
+// per 16-element tile, four lanes per element, each lane holding its
+// element's columns U[:, s] and lambda[:, s] and its accumulator in registers
+// and reading reference tables with scalar loads.  Per face the lanes form
+// the own and neighbour traces of U and lambda in canonical point order; one
+// (point, element) item per lane pulls jw (lambda_own - lambda_nbr) back
+// through the interface flux (Roe: reverse mode with roe_vjp; HLLC and
+// AUSM+-up: forward mode, the owner's four state tangents as one Dual<4>
+// pass, as k_vjp), and the element lifts the result with the side-0 face
+// table.  Boundary faces take the pre-pass
 // values (k_bvjp).  Per volume round the lanes interpolate U and the
 // reference gradients of lambda to four points; one item per lane pulls the
 // contravariant cotangents back through the Euler fluxes (euler_flux_vjp) and
@@ -1490,8 +1492,8 @@ struct VTile {
     static constexpr int WPE = N <= 6 && LDS * 12 <= 160 * 1024 ? 3 : 2;
 };
 
-template <class C>
-__global__ __launch_bounds__(DG_BS) __attribute__((amdgpu_waves_per_eu(VTile<C>::WPE)))
+template <class C, int FLUX>
+__global__ __launch_bounds__(DG_BS) __attribute__((amdgpu_waves_per_eu(FLUX == 0 ? VTile<C>::WPE : 1)))
 void k_vjpw(const nz_problem P, const double* __restrict__ T,
                                               const double* __restrict__ U,
                                               const double* __restrict__ Lin,
@@ -1694,7 +1696,24 @@ void k_vjpw(const nz_problem P, const double* __restrict__ T,
                         uR[c] = sd == 0 ? un : uo;
                         ob[c] = jw * (sd == 0 ? lo - ln : ln - lo);
                     }
-                    roe_vjp(uL, uR, eg[0], eg[1], P.gamma, P.efix, sd == 0, ob, v);
+                    if constexpr (FLUX == 0) {
+                        roe_vjp(uL, uR, eg[0], eg[1], P.gamma, P.efix, sd == 0, ob, v);
+                    } else {  // HLLC, AUSM+-up: the owner's four tangents as duals
+                        using D = Dual<4>;
+                        D dL[4], dR[4], fx[4];
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) {
+                            dL[c] = sd == 0 ? D::seed(uL[c], c) : D(uL[c]);
+                            dR[c] = sd == 0 ? D(uR[c]) : D::seed(uR[c], c);
+                        }
+                        interface_flux<FLUX>(P, dL, dR, eg[0], eg[1], fx);
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            double a = 0.0;
+                            for (int c = 0; c < 4; ++c) a += ob[c] * fx[c].d[j];
+                            v[j] = a;
+                        }
+                    }
                 } else {  // boundary faces: the pre-pass's values
                     const double* b = bv + ((k - P.n_int) * NQF + q) * 4;
 #pragma unroll
@@ -2042,10 +2061,14 @@ struct Launch {
                                dim3(nz::BS), 0, st, P, P.tables, U, L, bv, gedge, gpb);
         if (nblk <= 0) return;  // an empty tile table: nothing to march
         const bool geo = gmet || gedge;  // geometry cotangents: the normal is differentiated too
-        if (P.flux_id == 0 && !geo && P.vjp_kernel == 0) {  // Roe, state cotangents: wave tiles
-            hipLaunchKernelGGL((nz::k_vjpw<C>), dim3(static_cast<unsigned>(nblk)), dim3(nz::DG_BS), 0,
-                               st, P, P.tables, U, L, bv, out, mode, seed, Lbase, dt, acc, Lnext,
-                               stage);
+        if (!geo && P.vjp_kernel == 0) {  // state cotangents on wave tiles
+#define NZ_VJPW(FL)                                                                               \
+    hipLaunchKernelGGL((nz::k_vjpw<C, FL>), dim3(static_cast<unsigned>(nblk)), dim3(nz::DG_BS), 0, \
+                       st, P, P.tables, U, L, bv, out, mode, seed, Lbase, dt, acc, Lnext, stage)
+            if (P.flux_id == 1) NZ_VJPW(1);
+            else if (P.flux_id == 2) NZ_VJPW(2);
+            else NZ_VJPW(0);
+#undef NZ_VJPW
             return;
         }
         // its own grid: entries of a tile table, or tiles of C::TE elements
@@ -2117,10 +2140,8 @@ int resolve(const nz_problem* p, Ops* ops) {
 
 int64_t n_blocks(const nz_problem* p, const Ops& o) { return (p->n_elem + o.te - 1) / o.te; }
 // elements per workgroup of the state-cotangent VJP launches: the wave-tile
-// kernel's 16 for Roe, the four-wave kernel's tile otherwise
-int64_t vjp_te(const nz_problem* p, const Ops& o) {
-    return p->flux_id == 0 && p->vjp_kernel == 0 ? nz::TE : o.te;
-}
+// kernel's 16, or the four-wave kernel's tile
+int64_t vjp_te(const nz_problem* p, const Ops& o) { return p->vjp_kernel == 0 ? nz::TE : o.te; }
 int64_t n_blocks_vjp(const nz_problem* p, const Ops& o) {
     return p->vjp_tiles ? p->n_vjp_tiles : (p->n_elem + vjp_te(p, o) - 1) / vjp_te(p, o);
 }

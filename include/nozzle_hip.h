@@ -98,10 +98,10 @@ typedef struct {
      * march. */
     const int64_t *vjp_tiles;
     int64_t n_vjp_tiles;
-    /* the state-cotangent VJP kernel with the Roe flux: 0 one-wave tiles of
-     * 16 elements (the element kernel's mapping), 1 four-wave tiles of 64 (or
-     * 128 when N <= 4); the two agree to round-off.  HipBackend picks it per
-     * (element kind, order, geometry order) by timing both. */
+    /* the state-cotangent VJP kernel: 0 one-wave tiles of 16 elements (the
+     * element kernel's mapping), 1 four-wave tiles of 64 (or 128 when
+     * N <= 4); the two agree to round-off.  HipBackend picks it per (element
+     * kind, order, geometry order, flux) by timing both. */
     int32_t vjp_kernel;
     int32_t pad1;
 } nz_problem;
@@ -110,8 +110,8 @@ typedef struct {
 int64_t nz_scratch_doubles(const nz_problem *p);
 
 /* Elements per workgroup of the state-cotangent adjoint launches: the
- * widest entry a vjp_tiles table may hold (16 with the Roe flux and
- * vjp_kernel 0, else the four-wave tile of 64, or 128 when N <= 4).  No
+ * widest entry a vjp_tiles table may hold (16 with vjp_kernel 0, else
+ * the four-wave tile of 64, or 128 when N <= 4).  No
  * counterpart in the reference (its adjoint is a JAX vjp over the whole mesh,
  * src/sensitivity.py:317). */
 int64_t nz_adjoint_tile_elements(const nz_problem *p);
