@@ -97,6 +97,75 @@ def edge_vjp(ops, gedge: np.ndarray) -> np.ndarray:
     return gX
 
 
+# ------------------------------------------------------------------ autograd
+_DEVICE_RESIDUAL = None
+
+
+def _device_residual():
+    """The ``torch.autograd.Function`` behind :func:`residual`, built on first use."""
+    global _DEVICE_RESIDUAL
+    if _DEVICE_RESIDUAL is not None:
+        return _DEVICE_RESIDUAL
+    import torch
+    from torch.autograd.function import once_differentiable
+
+    class DeviceResidual(torch.autograd.Function):
+        """``R(U, X)`` by ``nz_residual``; backward by ``nz_residual_vjp``,
+        which re-evaluates the flux Jacobians per face point instead of storing
+        them.  The node-coordinate cotangent chains the kernel's metric and
+        edge cotangents back to the nodes (``metric_vjp``, ``edge_vjp``)."""
+
+        @staticmethod
+        def forward(ctx, U, X, bk):
+            R, _ = bk.residual(U)
+            ctx.bk = bk
+            ctx.x_device = None if X is None else X.device
+            ctx.save_for_backward(U)
+            return R
+
+        @staticmethod
+        @once_differentiable
+        def backward(ctx, gR):
+            (U,) = ctx.saved_tensors
+            bk = ctx.bk
+            gR = gR.to(dtype=torch.float64).contiguous()
+            if not ctx.needs_input_grad[1]:
+                return bk.residual_vjp(U, gR), None, None
+            dU, gmet, gedge, _ = bk.residual_vjp(U, gR, geometry=True)
+            gX = metric_vjp(bk.ops, gmet.cpu().numpy()) + edge_vjp(bk.ops, gedge.cpu().numpy())
+            gX = torch.from_numpy(gX).to(device=ctx.x_device)
+            return (dU if ctx.needs_input_grad[0] else None), gX, None
+
+    _DEVICE_RESIDUAL = DeviceResidual
+    return DeviceResidual
+
+
+def residual(backend, U, X=None):
+    """The unfolded DG residual ``R(U)`` (``(E, N, 4)``, the reference's
+    ``DifferentiableCase.residual``, ``src/sensitivity.py:230-234``) as a
+    differentiable torch operation on the device.
+
+    ``backend`` is a ``HipBackend``; ``U`` an ``(E, N, 4)`` float64 tensor on
+    its device.  ``X`` (optional, ``(n_nodes, 2)`` float64, any device) is the
+    node-coordinate tensor the backend's operators were built from: the
+    forward pass uses the backend's metrics, so ``X`` only carries the
+    gradient ``d(<g, R>)/dX`` back to the caller; coordinates that differ from
+    the backend's raise ``ValueError``.  Backward gives ``(dR/dU)^T g`` and,
+    when ``X`` requires it, ``(dR/dX)^T g``; double backward is not supported.
+    """
+    import torch
+
+    if not isinstance(U, torch.Tensor) or U.dtype != torch.float64 or U.device != backend.device:
+        raise ValueError(f"U must be a float64 tensor on {backend.device}")
+    fn = _device_residual()
+    if X is None:
+        return fn.apply(U, None, backend)
+    Xref = np.asarray(backend.ops.node_coords)
+    if tuple(X.shape) != Xref.shape or not np.array_equal(X.detach().cpu().numpy(), Xref):
+        raise ValueError("X differs from the node coordinates the backend was built from")
+    return fn.apply(U, X, backend)
+
+
 # ------------------------------------------------------------------ objectives
 def _trace(U, ops, tag):
     topo = ops.topology

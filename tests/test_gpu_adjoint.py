@@ -5,6 +5,9 @@
   kind (FD truncation bounds the tolerance at 1e-6 relative);
 * geometry and back-pressure cotangents against central differences of the
   residual under node motion and back-pressure change;
+* the residual as a torch autograd operation (``sensitivity.residual``):
+  backward equal to the direct VJP calls, node gradient against central
+  differences;
 * the captured sensitivity case of the reference (smooth contour, p=1,
   refine 0, p_b/p_t = 0.15, tolerance 1e-11, Roe): thrust 0.0458513361 and
   the adjoint derivatives printed in docs/validation/gradients.md:12-20, to
@@ -85,6 +88,61 @@ def test_geometry_and_back_pressure_cotangents(name, key):
            - lamR(X, flow.replace(back_pressure_ratio=pb - hp))) / (2 * hp)
     print(name, "back pressure", fdp, gpb)
     assert abs(fdp - gpb) <= 1e-6 * max(abs(gpb), 1e-3 * np.abs(lam).max())
+
+
+@pytest.mark.parametrize("name,key", [("cfg1", "U0"), ("quad_q2_p2", "U0"),
+                                      ("tri_p2_q2_backflow", "U")])
+def test_autograd_residual(name, key):
+    """``sensitivity.residual`` as a torch autograd operation: forward is the
+    device residual, backward the device VJP bit for bit, and the node
+    gradient matches central differences of <g, R> under node motion."""
+    import torch
+
+    op = nz_module("operators")
+    sens = nz_module("sensitivity")
+    g = golden(name)
+    ops, flow = build_case(name, dense=False)
+    bk = backend(ops, flow)
+    rng = np.random.default_rng(13)
+    U = torch.tensor(g[key], device=bk.device, requires_grad=True)
+    X = torch.tensor(np.asarray(ops.node_coords), dtype=torch.float64, requires_grad=True)
+    gR = torch.tensor(rng.standard_normal(g[key].shape), device=bk.device)
+    R = sens.residual(bk, U, X)
+    assert torch.equal(R, bk.residual(U.detach())[0])
+    (R * gR).sum().backward()
+    # with X the geometry launch runs (dual-number Jacobians): same kernel, same bits
+    assert torch.equal(U.grad, bk.residual_vjp(U.detach(), gR, geometry=True)[0])
+    assert X.grad.shape == X.shape and X.grad.device == X.device
+    # state only: X left out, the state-cotangent launch (reverse-mode Roe).
+    # The two launches differentiate the Roe flux in different orders; they
+    # agree to 3e-16 on the regular states.  The backflow state has elements
+    # whose pressure is negative at every point (floored); there the Roe-mean
+    # sound speed is floored to 1e-5, the wave decomposition divides by c^2
+    # and cancels, rounding differences are amplified, and the two differ by
+    # 3e-9 relative.
+    U2 = U.detach().clone().requires_grad_(True)
+    (sens.residual(bk, U2) * gR).sum().backward()
+    assert torch.equal(U2.grad, bk.residual_vjp(U.detach(), gR))
+    tol = 1e-8 if "backflow" in name else 1e-15
+    assert float((U2.grad - U.grad).abs().max() / U.grad.abs().max()) <= tol
+    Xn = np.asarray(ops.node_coords)
+    dX = rng.standard_normal(Xn.shape) * 1e-2
+    h = 1e-6
+    lam = gR.cpu().numpy()
+
+    def lamR(Xp):
+        o = op.build_operators(ops.topology, Xp, ops.ref.order, dense=False)
+        Rp, _ = backend(o, flow).residual(g[key])
+        return float((lam * Rp).sum())
+
+    fd = (lamR(Xn + h * dX) - lamR(Xn - h * dX)) / (2 * h)
+    an = float((X.grad.numpy() * dX).sum())
+    print(name, "autograd geometry", fd, an)
+    assert abs(fd - an) <= 1e-6 * abs(an)
+    with pytest.raises(ValueError, match="differs"):
+        sens.residual(bk, U.detach(), X.detach() + 1.0)
+    with pytest.raises(ValueError, match="float64 tensor"):
+        sens.residual(bk, U.detach().cpu())
 
 
 def _case():
