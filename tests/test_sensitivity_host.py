@@ -77,3 +77,22 @@ def test_objective_partials(objective):
     fd = (J_of(X + h * dX) - J_of(X - h * dX)) / (2 * h)
     an = (sens.edge_vjp(ops, dG) * dX).sum()
     assert abs(fd - an) <= 1e-6 * max(abs(fd), 1e-12)
+
+
+def test_autograd_residual_rejects_host_inputs():
+    """``sensitivity.residual`` refuses a state that is not a float64 tensor
+    on the backend's device, before any device call (host logic only)."""
+    import torch
+
+    sens = nz_module("sensitivity")
+
+    class _Stub:  # only the attributes the input checks read
+        device = torch.device("cuda", 0)
+
+    U = np.zeros((4, 3, 4))
+    with pytest.raises(ValueError, match="float64 tensor"):
+        sens.residual(_Stub(), U)
+    with pytest.raises(ValueError, match="float64 tensor"):
+        sens.residual(_Stub(), torch.zeros((4, 3, 4), dtype=torch.float64))
+    with pytest.raises(ValueError, match="float64 tensor"):
+        sens.residual(_Stub(), torch.zeros((4, 3, 4), dtype=torch.float32))
