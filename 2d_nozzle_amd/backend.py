@@ -172,6 +172,9 @@ class HipBackend:
             if minv is None:
                 wdet = rd.w_vol[None, :] * det
                 minv = np.linalg.inv(np.einsum("iq,jq,eq->eij", rd.phi_vol, rd.phi_vol, wdet))
+            # the device keeps the upper triangles, row-major (i <= j)
+            iu = np.triu_indices(rd.n_basis)
+            minv = np.asarray(minv)[:, iu[0], iu[1]]
             mean_w = np.asarray(ops.mean_weights)
         nrm = np.asarray(ops.edge_normal)
         jac = np.asarray(ops.edge_jac)

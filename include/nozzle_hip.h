@@ -57,7 +57,8 @@ typedef struct {
     const double *geo;      /* affine: (E, 5) = xi_x, xi_y, eta_x, eta_y, det
                                else:   (E, nqv, 5) = xi_x, xi_y, eta_x, eta_y, w*det */
     const double *area;     /* (E,) element area, sum_q w_q det_q                   */
-    const double *minv;     /* (E, N, N) inverse mass matrix (non-affine only)      */
+    const double *minv;     /* (E, N(N+1)/2) inverse mass matrices (non-affine only):
+                             * upper triangles, row-major, entry (i, j) for i <= j */
     const double *mean_w;   /* (E, N) cell-average weights (non-affine only)        */
     const double *edge_geo; /* straight: (F, 3) = nx, ny, |dX/dsigma|;  else (F, nqf, 3) */
     const int32_t *fe_nbr;  /* (E, nface) element across the face, or -tag (1..4)   */
