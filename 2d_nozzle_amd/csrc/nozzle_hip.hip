@@ -20,14 +20,19 @@
 //           threads (four waves) owns a tile of TE elements (64, or 128 when
 //           N <= 4), staged in LDS as [component][element] rows; each phase
 //           maps the wave to a wave-uniform index (component, point or basis
-//           function) and the lane to the element.
+//           function) and the lane to the element.  It serves the launch
+//           that also returns geometry cotangents, and the state-cotangent
+//           launches when the tile-shape autotuner picks four-wave tiles.
+//   k_vjpw  the state-cotangent VJP and the adjoint RK4 stage on k_dg's
+//           mapping (one wave per 16-element tile, four lanes per element).
 //   others  small elementwise or per-element kernels (boundary pre-passes,
 //           standalone limiter, superbee passes, inverse mass, time step,
 //           norm finishing) on 256-thread workgroups.
 //
 // Affine triangles (Q = 1) fold M^-1 = M_ref^-1 / det into reference tables,
 // so an element carries only (xi_x, xi_y, eta_x, eta_y, det).  Curved or
-// quadrilateral elements carry per-point metrics and apply their own M^-1.
+// quadrilateral elements carry per-point metrics and apply their own M^-1,
+// stored as its upper triangle.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
