@@ -197,3 +197,31 @@ def test_superbee_then_positivity_and_steps():
     U, res = bk.run(g["Ub"], 3, "ssprk3")
     Uo, reso = CpuOracle(ops, flow).run(g["Ub"], 3, "ssprk3", bk.dt_factor(), "superbee", 0.0)
     assert rel_err(U, Uo) <= 1e-12 and abs(res / reso - 1) <= 1e-12
+
+
+@pytest.mark.parametrize("q", [1, 2])
+@pytest.mark.parametrize("flux", FLUXES)
+def test_quad_p3_against_oracle(q, flux):
+    """Quadrilaterals at P3 (N = 16), straight (Q1) and curved (Q2) edges, on
+    the quasi-1D initial state of the smooth nozzle at refine 1 (280
+    elements): the rate, then two SSP-RK3 steps with the positivity limiter,
+    against the CPU oracle."""
+    from oracle.oracle import CpuOracle
+
+    mesh, op, cf, ini = (nz_module(m) for m in ("mesh", "operators", "config", "initial"))
+    geom = mesh.NozzleGeometry(contour="smooth", area_ratio=2.5)
+    topo, coords, _ = mesh.build_nozzle_mesh(geom, kind="quad", refine=1, geometry_order=q)
+    ops = op.build_operators(topo, coords, 3, dense=True)
+    flow = cf.FlowConditions().replace(flux=flux)
+    U0 = ini.quasi1d_initial(ops, flow, geom)
+    bk = backend(ops, flow)
+    orc = CpuOracle(ops, flow)
+    F, w = bk.rate(U0)
+    Fo, wo = orc.rate(U0)
+    print("quad P3 Q", q, flux, rel_err(F, Fo), rel_err(w, wo))
+    assert rel_err(F, Fo) <= 1e-12 and rel_err(w, wo) <= 1e-13
+    U, res = bk.run(U0, 2, "ssprk3")
+    Uo, reso = orc.run(U0, 2, "ssprk3", bk.dt_factor("ssprk3"))
+    print("  two steps", rel_err(U, Uo), res, reso, bk.limiter_activity())
+    assert rel_err(U, Uo) <= 1e-12
+    assert abs(res / reso - 1) <= 1e-12
