@@ -363,6 +363,19 @@ class HipBackend:
                   "nz_inverse_mass")
         return self._out(out, npy)
 
+    def project(self, values):
+        """L2 projection of point values ``(E, nqv, 4)`` onto the basis, on the
+        device (``project``, ``src/timestepping/initialize.py:41-46``)."""
+        v, npy = self._in(values)
+        rd = self.ops.ref
+        if tuple(v.shape) != (self.E, rd.n_qvol, 4):
+            raise ValueError(f"values must have shape {(self.E, rd.n_qvol, 4)}, got {tuple(v.shape)}")
+        out = self._torch.empty((self.E, rd.n_basis, 4), dtype=self._torch.float64,
+                                device=self.device)
+        self._chk(self.lib.nz_project(self._pp, self._p(v), self._p(out), self._stream()),
+                  "nz_project")
+        return self._out(out, npy)
+
     def limit(self, U):
         """Limited copy of ``U``; activity counters as ``numba_backend.py:265-280``."""
         u, npy = self._in(U)

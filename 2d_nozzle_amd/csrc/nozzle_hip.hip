@@ -1911,6 +1911,40 @@ __global__ __launch_bounds__(BS) void k_inverse_mass(const nz_problem P, const d
     for (int s = 0; s < 4; ++s) out[it * 4 + s] = r[s];
 }
 
+// L2 projection of point values onto the basis (src/timestepping/
+// initialize.py:41-46): U_e = M_e^-1 sum_q phi(q) (w det)_q v_q, one thread
+// per (element, component) with the element's right-hand side in registers.
+template <class C>
+__global__ __launch_bounds__(BS) void k_project(const nz_problem P, const double* __restrict__ T,
+                                                const double* __restrict__ V,
+                                                double* __restrict__ out) {
+    constexpr int N = C::N, NQV = C::NQV;
+    const int64_t it = static_cast<int64_t>(blockIdx.x) * BS + threadIdx.x;
+    if (it >= P.n_elem * 4) return;
+    const int64_t e = it >> 2;
+    const int s = static_cast<int>(it & 3);
+    double rhs[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) rhs[i] = 0.0;
+    for (int q = 0; q < NQV; ++q) {
+        const double wd = C::AFF ? T[P.off.w_vol + q] * P.geo[e * 5 + 4] : P.geo[(e * NQV + q) * 5 + 4];
+        const double v = wd * V[(e * NQV + q) * 4 + s];
+#pragma unroll
+        for (int i = 0; i < N; ++i) rhs[i] += T[P.off.phi_vol + i * NQV + q] * v;
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        double r = 0.0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            const double m = C::AFF ? T[P.off.minv_ref + i * N + j] / P.geo[e * 5 + 4]
+                                    : P.minv[e * sym_size(N) + sym_index(N, i, j)];
+            r += m * rhs[j];
+        }
+        out[(e * N + i) * 4 + s] = r;
+    }
+}
+
 }  // namespace nz
 
 // ---------------------------------------------------------------- generic kernels
@@ -2005,6 +2039,7 @@ struct Ops {
                double, int, double*, uint8_t*, unsigned long long*, int64_t, hipStream_t);
     void (*limit)(const nz_problem&, double*, uint8_t*, unsigned long long*, hipStream_t);
     void (*inverse_mass)(const nz_problem&, const double*, double*, hipStream_t);
+    void (*project)(const nz_problem&, const double*, double*, hipStream_t);
     void (*superbee)(const nz_problem&, double*, double, double, double*,
                      unsigned long long*, hipStream_t);
     void (*vjp)(const nz_problem&, const double*, const double*, double*, double*, double*,
@@ -2051,6 +2086,10 @@ struct Launch {
     static void inverse_mass(const nz_problem& P, const double* R, double* out, hipStream_t st) {
         const unsigned nb = static_cast<unsigned>((P.n_elem * C::N + nz::BS - 1) / nz::BS);
         hipLaunchKernelGGL((nz::k_inverse_mass<C>), dim3(nb), dim3(nz::BS), 0, st, P, P.tables, R, out);
+    }
+    static void project(const nz_problem& P, const double* V, double* out, hipStream_t st) {
+        const unsigned nb = static_cast<unsigned>((P.n_elem * 4 + nz::BS - 1) / nz::BS);
+        hipLaunchKernelGGL((nz::k_project<C>), dim3(nb), dim3(nz::BS), 0, st, P, P.tables, V, out);
     }
     static void superbee(const nz_problem& P, double* U, double tvb, double area_total,
                          double* work, unsigned long long* counters, hipStream_t st) {
@@ -2101,7 +2140,7 @@ struct Launch {
         else NZ_VJP(0);
 #undef NZ_VJP
     }
-    static Ops ops() { return Ops{&dg, &limit, &inverse_mass, &superbee, &vjp, C::TE}; }
+    static Ops ops() { return Ops{&dg, &limit, &inverse_mass, &project, &superbee, &vjp, C::TE}; }
 };
 
 // (N, nqv, nqf, nface, affine, straight) for p = 0..3 of each element family
@@ -2210,6 +2249,15 @@ int nz_inverse_mass(const nz_problem* p, const double* R, double* out, void* str
     if (!R || !out) return fail(NZ_E_ARG, "null array");
     o.inverse_mass(*p, R, out, static_cast<hipStream_t>(stream));
     return check_launch("nz_inverse_mass");
+}
+
+int nz_project(const nz_problem* p, const double* values, double* U, void* stream) {
+    Ops o;
+    int rc = resolve(p, &o);
+    if (rc) return rc;
+    if (!values || !U) return fail(NZ_E_ARG, "null array");
+    o.project(*p, values, U, static_cast<hipStream_t>(stream));
+    return check_launch("nz_project");
 }
 
 int nz_stage(const nz_problem* p, const double* Uin, const double* Un, double* Uout, double* dt,

@@ -283,8 +283,12 @@ def uniform_initial(ops, flow: FlowConditions, mach: float = 0.95) -> np.ndarray
     return project(values, ops)
 
 
-def quasi1d_initial(ops, flow: FlowConditions, geom, *, solution=None) -> np.ndarray:
-    """Projected quasi-1D state with a wall-slope transverse velocity (``initialize.py:68-102``)."""
+def quasi1d_initial(ops, flow: FlowConditions, geom, *, solution=None, backend=None) -> np.ndarray:
+    """Projected quasi-1D state with a wall-slope transverse velocity (``initialize.py:68-102``).
+
+    With ``backend`` (a ``HipBackend`` on these operators) the projection runs
+    on the device (``nz_project``); the point values are formed here either way.
+    """
     sol = solution if solution is not None else solve_quasi1d(geom, flow)
     xy = np.asarray(ops.xy_vol)
     x, y = xy[..., 0], xy[..., 1]
@@ -298,7 +302,10 @@ def quasi1d_initial(ops, flow: FlowConditions, geom, *, solution=None) -> np.nda
         h + np.minimum(x, h))
     vy = vx * np.clip(y / np.maximum(y_wall, 1e-30), 0.0, 1.0) * slope
     rhoE = p / (flow.gamma - 1.0) + 0.5 * rho * (vx * vx + vy * vy)
-    return project(np.stack([rho, rho * vx, rho * vy, rhoE], axis=-1), ops)
+    values = np.stack([rho, rho * vx, rho * vy, rhoE], axis=-1)
+    if backend is not None:
+        return backend.project(values)
+    return project(values, ops)
 
 
 def initial_state(ops, flow: FlowConditions, geom, kind: str = "quasi1d") -> np.ndarray:

@@ -116,8 +116,8 @@ def shocked_bench(refine: int, steps: int) -> dict:
     topo, coords, _ = mesh.build_nozzle_mesh(geom, kind="quad", refine=refine, geometry_order=2)
     ops = op.build_operators(topo, coords, 2, dense=False)
     flow = cf.FlowConditions(back_pressure_ratio=0.7, allow_shock_in_nozzle=True, flux="hllc")
-    U0 = ini.quasi1d_initial(ops, flow, geom)
     bk = be.HipBackend(ops, flow, cf.SolverOptions(limiter="positivity"))
+    U0 = ini.quasi1d_initial(ops, flow, geom, backend=bk)  # projected on the device
     E, N = ops.n_elem, ops.ref.n_basis
     print(f"shocked: {E} quads Q2 P2, host setup {time.perf_counter() - t0:.1f} s",
           file=sys.stderr, flush=True)

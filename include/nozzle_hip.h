@@ -130,6 +130,12 @@ int nz_rate(const nz_problem *p, const double *U, double *F, double *wave, void 
 /* out = M^-1 R per element (src/backends/_numba_kernels.py:762-777). */
 int nz_inverse_mass(const nz_problem *p, const double *R, double *out, void *stream);
 
+/* L2 projection onto the basis: U = M^-1 sum_q phi(q) (w det)_q values_q per
+ * element.  Replaces project (src/timestepping/initialize.py:41-46), the step
+ * that forms the initial state.  values: (E, nqv, 4) at the volume points;
+ * U: (E, N, 4). */
+int nz_project(const nz_problem *p, const double *values, double *U, void *stream);
+
 /* One fused Runge-Kutta stage: rate of Uin, stage combination, positivity
  * limiter, written to Uout (which must not alias Uin or Un).
  *   first != 0:  Uout = L(Uin + dt * F(Uin)); dt is computed from the wave

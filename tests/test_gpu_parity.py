@@ -225,3 +225,39 @@ def test_quad_p3_against_oracle(q, flux):
     print("  two steps", rel_err(U, Uo), res, reso, bk.limiter_activity())
     assert rel_err(U, Uo) <= 1e-12
     assert abs(res / reso - 1) <= 1e-12
+
+
+@pytest.mark.parametrize("name", ["cfg1", "quad_q2_p2", "tri_p0"])
+def test_device_projection_forms_the_reference_initial_state(name):
+    """``nz_project`` (``initialize.py:41-46``) forms the reference's own
+    quasi-1D initial state (fixture ``U0``) to 1e-13, and projects random
+    point values as the host projection does."""
+    from conftest import CASES
+
+    ini, mesh = nz_module("initial"), nz_module("mesh")
+    g = golden(name)
+    ops, flow = build_case(name, dense=False)
+    mk = CASES[name][0]
+    geom = mesh.NozzleGeometry(contour=mk["contour"], area_ratio=mk["area_ratio"])
+    bk = backend(ops, flow)
+    U0 = ini.quasi1d_initial(ops, flow, geom, backend=bk)
+    assert isinstance(U0, np.ndarray) and U0.shape == g["U0"].shape
+    print(name, rel_err(U0, g["U0"]))
+    assert rel_err(U0, g["U0"]) < 1e-13
+    vals = np.random.default_rng(2).standard_normal((ops.n_elem, ops.ref.n_qvol, 4))
+    assert rel_err(bk.project(vals), ini.project(vals, ops)) < 1e-13
+
+
+@pytest.mark.parametrize("kind,q,p", [("quad", 1, 3), ("quad", 2, 3), ("tri", 2, 2), ("tri", 1, 3)])
+def test_device_projection_is_exact_on_the_space(kind, q, p):
+    """Point values of a random element of the space project back onto it."""
+    mesh, op, cf = nz_module("mesh"), nz_module("operators"), nz_module("config")
+    geom = mesh.NozzleGeometry(contour="smooth", area_ratio=2.5)
+    topo, coords, _ = mesh.build_nozzle_mesh(geom, kind=kind, refine=1, geometry_order=q)
+    ops = op.build_operators(topo, coords, p, dense=False)
+    bk = backend(ops, cf.FlowConditions())
+    U = np.random.default_rng(4).standard_normal((ops.n_elem, ops.ref.n_basis, 4))
+    vals = np.einsum("iq,eis->eqs", np.asarray(ops.ref.phi_vol), U)
+    assert rel_err(bk.project(vals), U) < 1e-12
+    with pytest.raises(ValueError, match="shape"):
+        bk.project(vals[:, :-1])
