@@ -277,9 +277,11 @@ def uniform_state(flow: FlowConditions, mach: float = 0.95) -> np.ndarray:
     return np.array([rho, rho * vx, 0.0, p / (g - 1.0) + 0.5 * rho * vx * vx])
 
 
-def uniform_initial(ops, flow: FlowConditions, mach: float = 0.95) -> np.ndarray:
-    """``initialize.py:61-65``."""
+def uniform_initial(ops, flow: FlowConditions, mach: float = 0.95, *, backend=None) -> np.ndarray:
+    """``initialize.py:61-65``; with ``backend`` the projection runs on the device."""
     values = np.broadcast_to(uniform_state(flow, mach), (ops.n_elem, ops.ref.n_qvol, 4))
+    if backend is not None:
+        return backend.project(values)
     return project(values, ops)
 
 
@@ -317,12 +319,18 @@ def initial_state(ops, flow: FlowConditions, geom, kind: str = "quasi1d") -> np.
     raise ValueError(f"unknown initial_condition {kind!r}; use 'quasi1d' or 'uniform'")
 
 
-def change_order(U, ops_from, ops_to) -> np.ndarray:
-    """Re-project a solution onto another order on the same mesh (``initialize.py:121-147``)."""
+def change_order(U, ops_from, ops_to, *, backend=None) -> np.ndarray:
+    """Re-project a solution onto another order on the same mesh (``initialize.py:121-147``).
+
+    With ``backend`` (a ``HipBackend`` on ``ops_to``) the projection runs on the device.
+    """
     from .reference_element import shape_functions
 
     if ops_from.n_elem != ops_to.n_elem:
         raise ValueError(
             f"change_order needs the same mesh: {ops_from.n_elem} vs {ops_to.n_elem} elements")
     phi, _, _ = shape_functions(ops_from.ref.kind, ops_from.ref.order, ops_to.ref.points_vol)
-    return project(np.einsum("iq,eis->eqs", phi, np.asarray(U)), ops_to)
+    values = np.einsum("iq,eis->eqs", phi, np.asarray(U))
+    if backend is not None:
+        return backend.project(values)
+    return project(values, ops_to)

@@ -261,3 +261,22 @@ def test_device_projection_is_exact_on_the_space(kind, q, p):
     assert rel_err(bk.project(vals), U) < 1e-12
     with pytest.raises(ValueError, match="shape"):
         bk.project(vals[:, :-1])
+
+
+def test_device_projection_uniform_start_and_order_change():
+    """The uniform start and the order change (``initialize.py:61-65,
+    121-147``) through the device projection: the same states as on the host
+    (1e-13), and a P1 field raised to P2 and lowered again comes back."""
+    ini, mesh, op, cf = (nz_module(m) for m in ("initial", "mesh", "operators", "config"))
+    ops, flow = build_case("cfg1", dense=False)
+    bk = backend(ops, flow)
+    assert rel_err(ini.uniform_initial(ops, flow, backend=bk), ini.uniform_initial(ops, flow)) < 1e-13
+    geom = mesh.NozzleGeometry(contour="smooth", area_ratio=2.5)
+    topo, coords, _ = mesh.build_nozzle_mesh(geom, kind="quad", refine=1, geometry_order=2)
+    o1 = op.build_operators(topo, coords, 1, dense=False)
+    o2 = op.build_operators(topo, coords, 2, dense=False)
+    b1, b2 = backend(o1, cf.FlowConditions()), backend(o2, cf.FlowConditions())
+    U1 = np.random.default_rng(6).standard_normal((o1.n_elem, o1.ref.n_basis, 4))
+    U2 = ini.change_order(U1, o1, o2, backend=b2)
+    assert rel_err(U2, ini.change_order(U1, o1, o2)) < 1e-13
+    assert rel_err(ini.change_order(U2, o2, o1, backend=b1), U1) < 1e-12
