@@ -58,6 +58,7 @@ SIGNATURES = {
     "nz_rate": (C.c_int, [C.POINTER(Problem), P, P, P, P]),
     "nz_inverse_mass": (C.c_int, [C.POINTER(Problem), P, P, P]),
     "nz_project": (C.c_int, [C.POINTER(Problem), P, P, P]),
+    "nz_boundary_integrals": (C.c_int, [C.POINTER(Problem), P, P, P]),
     "nz_stage": (C.c_int, [C.POINTER(Problem), P, P, P, P, C.c_double, C.c_double, C.c_double,
                            C.c_int, P, P, P, P]),
     "nz_norm_finish": (C.c_int, [C.POINTER(Problem), P, P, P]),

@@ -376,6 +376,26 @@ class HipBackend:
                   "nz_project")
         return self._out(out, npy)
 
+    def performance(self, U):
+        """Thrust, mass flow and exit Mach of ``U`` (an ``observables.Performance``)
+        from the device's boundary integrals (``performance``,
+        ``src/post/postprocess.py:193-226``)."""
+        from .observables import Performance
+
+        u, _ = self._in(U)
+        out = self._torch.empty(8, dtype=self._torch.float64, device=self.device)
+        self._chk(self.lib.nz_boundary_integrals(self._pp, self._p(u), self._p(out), self._stream()),
+                  "nz_boundary_integrals")
+        mi, mo, wall, qi, qo, area, mach, p = out.cpu().tolist()
+        return Performance(
+            thrust=2.0 * (mo + mi),
+            thrust_wall=2.0 * wall,
+            mass_flow_in=2.0 * -qi,
+            mass_flow_out=2.0 * qo,
+            exit_mach_area_averaged=mach / area,
+            exit_pressure_ratio=p / area / self.flow.total_pressure,
+        )
+
     def limit(self, U):
         """Limited copy of ``U``; activity counters as ``numba_backend.py:265-280``."""
         u, npy = self._in(U)

@@ -1945,6 +1945,68 @@ __global__ __launch_bounds__(BS) void k_project(const nz_problem P, const double
     }
 }
 
+// Boundary integrals of the owner-side trace for the observables
+// (src/post/postprocess.py:56-75,155-171,193-226).  One workgroup scans every
+// boundary point in a fixed order and reduces in LDS in a fixed tree, so the
+// sums do not depend on the launch.  out[8]: axial momentum flux through the
+// inflow and the outflow, -p n_x ds over the walls, mass flux through the
+// inflow and the outflow, the outflow length, and the integrals of the Mach
+// number and of p over the outflow.
+template <class C>
+__global__ __launch_bounds__(BS) void k_boundary_integrals(const nz_problem P,
+                                                           const double* __restrict__ T,
+                                                           const double* __restrict__ U,
+                                                           double* __restrict__ out) {
+    constexpr int N = C::N, NQF = C::NQF;
+    constexpr int K = 8;
+    __shared__ double red[K][BS];
+    double acc[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) acc[j] = 0.0;
+    const int t = threadIdx.x;
+    const int64_t nb = P.n_edge - P.n_int;
+    for (int64_t it = t; it < nb * NQF; it += BS) {
+        const int64_t b = it / NQF;
+        const int q = static_cast<int>(it - b * NQF);
+        const int64_t e = P.bedge[2 * b];
+        const int ft = P.bedge[2 * b + 1];
+        const int f = ft >> 3, tag = ft & 7;
+        if (tag < 1 || tag > 3) continue;  // the symmetry axis carries none of them
+        double S[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int i = 0; i < N; ++i) {
+            const double w = T[P.off.phi_face + (f * N + i) * NQF + q];
+            for (int c = 0; c < 4; ++c) S[c] += w * U[(e * N + i) * 4 + c];
+        }
+        const int64_t k = P.n_int + b;
+        const double* eg = P.edge_geo + (C::STR ? k * 3 : (k * NQF + q) * 3);
+        const double nx = eg[0], ny = eg[1], ds = eg[2] * T[P.off.w_face + q];
+        const double r = fmax(S[0], kFloor), vx = S[1] / r, vy = S[2] / r;
+        const double p = fmax((P.gamma - 1.0) * (S[3] - 0.5 * r * (vx * vx + vy * vy)), kFloor);
+        const double vn = vx * nx + vy * ny;
+        if (tag == 3) {
+            acc[2] += -p * nx * ds;
+        } else {
+            const int o = tag == 2 ? 1 : 0;
+            acc[o] += (r * vx * vn + p * nx) * ds;
+            acc[3 + o] += r * vn * ds;
+            if (tag == 2) {
+                acc[5] += ds;
+                acc[6] += sqrt((vx * vx + vy * vy) * r / (P.gamma * p)) * ds;
+                acc[7] += p * ds;
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j) red[j][t] = acc[j];
+    __syncthreads();
+    for (int w = BS / 2; w > 0; w >>= 1) {
+        if (t < w)
+            for (int j = 0; j < K; ++j) red[j][t] += red[j][t + w];
+        __syncthreads();
+    }
+    if (t < K) out[t] = red[t][0];
+}
+
 }  // namespace nz
 
 // ---------------------------------------------------------------- generic kernels
@@ -2040,6 +2102,7 @@ struct Ops {
     void (*limit)(const nz_problem&, double*, uint8_t*, unsigned long long*, hipStream_t);
     void (*inverse_mass)(const nz_problem&, const double*, double*, hipStream_t);
     void (*project)(const nz_problem&, const double*, double*, hipStream_t);
+    void (*boundary_integrals)(const nz_problem&, const double*, double*, hipStream_t);
     void (*superbee)(const nz_problem&, double*, double, double, double*,
                      unsigned long long*, hipStream_t);
     void (*vjp)(const nz_problem&, const double*, const double*, double*, double*, double*,
@@ -2091,6 +2154,11 @@ struct Launch {
         const unsigned nb = static_cast<unsigned>((P.n_elem * 4 + nz::BS - 1) / nz::BS);
         hipLaunchKernelGGL((nz::k_project<C>), dim3(nb), dim3(nz::BS), 0, st, P, P.tables, V, out);
     }
+    static void boundary_integrals(const nz_problem& P, const double* U, double* out,
+                                   hipStream_t st) {
+        hipLaunchKernelGGL((nz::k_boundary_integrals<C>), dim3(1), dim3(nz::BS), 0, st, P, P.tables,
+                           U, out);
+    }
     static void superbee(const nz_problem& P, double* U, double tvb, double area_total,
                          double* work, unsigned long long* counters, hipStream_t st) {
         const int64_t nb = (P.n_elem + nz::BS - 1) / nz::BS;
@@ -2140,7 +2208,9 @@ struct Launch {
         else NZ_VJP(0);
 #undef NZ_VJP
     }
-    static Ops ops() { return Ops{&dg, &limit, &inverse_mass, &project, &superbee, &vjp, C::TE}; }
+    static Ops ops() {
+        return Ops{&dg, &limit, &inverse_mass, &project, &boundary_integrals, &superbee, &vjp, C::TE};
+    }
 };
 
 // (N, nqv, nqf, nface, affine, straight) for p = 0..3 of each element family
@@ -2258,6 +2328,15 @@ int nz_project(const nz_problem* p, const double* values, double* U, void* strea
     if (!values || !U) return fail(NZ_E_ARG, "null array");
     o.project(*p, values, U, static_cast<hipStream_t>(stream));
     return check_launch("nz_project");
+}
+
+int nz_boundary_integrals(const nz_problem* p, const double* U, double* out, void* stream) {
+    Ops o;
+    int rc = resolve(p, &o);
+    if (rc) return rc;
+    if (!U || !out) return fail(NZ_E_ARG, "null array");
+    o.boundary_integrals(*p, U, out, static_cast<hipStream_t>(stream));
+    return check_launch("nz_boundary_integrals");
 }
 
 int nz_stage(const nz_problem* p, const double* Uin, const double* Un, double* Uout, double* dt,

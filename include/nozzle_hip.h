@@ -136,6 +136,16 @@ int nz_inverse_mass(const nz_problem *p, const double *R, double *out, void *str
  * U: (E, N, 4). */
 int nz_project(const nz_problem *p, const double *values, double *U, void *stream);
 
+/* Boundary integrals of the owner-side trace behind the observables
+ * (thrust, mass flow, exit Mach).  Replaces the sums of performance,
+ * axial_momentum_flux and mass_flux (src/post/postprocess.py:56-75,
+ * 155-171, 193-226).  out (8 doubles): axial momentum flux through the
+ * inflow and the outflow, -p n_x ds over the walls, mass flux through the
+ * inflow and the outflow, the outflow length, and the integrals of the Mach
+ * number and of p over the outflow (one half-channel; the observables take
+ * twice the fluxes). */
+int nz_boundary_integrals(const nz_problem *p, const double *U, double *out, void *stream);
+
 /* One fused Runge-Kutta stage: rate of Uin, stage combination, positivity
  * limiter, written to Uout (which must not alias Uin or Un).
  *   first != 0:  Uout = L(Uin + dt * F(Uin)); dt is computed from the wave

@@ -280,3 +280,25 @@ def test_device_projection_uniform_start_and_order_change():
     U2 = ini.change_order(U1, o1, o2, backend=b2)
     assert rel_err(U2, ini.change_order(U1, o1, o2)) < 1e-13
     assert rel_err(ini.change_order(U2, o2, o1, backend=b1), U1) < 1e-12
+
+
+def test_device_observables_of_the_reference_state():
+    """``nz_boundary_integrals`` on the reference's converged state gives the
+    reference's thrust, wall thrust, mass flows, exit Mach and exit pressure
+    (``src/post/postprocess.py:193-226``) to 1e-13, as the host observables
+    do; and on a curved quadrilateral state the two agree."""
+    ob = nz_module("observables")
+    g = golden("cfg1_converged")
+    ops, flow = build_case("cfg1", dense=False)
+    perf = backend(ops, flow).performance(g["U"])
+    for k in ("thrust", "thrust_wall", "mass_flow_in", "mass_flow_out",
+              "exit_mach_area_averaged", "exit_pressure_ratio"):
+        print(k, getattr(perf, k), float(g[f"perf_{k}"]))
+        assert abs(getattr(perf, k) / float(g[f"perf_{k}"]) - 1) <= 1e-13, k
+    gq = golden("quad_q2_p2")
+    ops, flow = build_case("quad_q2_p2", dense=False)
+    a = backend(ops, flow).performance(gq["U0"])
+    b = ob.performance(gq["U0"], ops, flow)
+    for k in ("thrust", "thrust_wall", "mass_flow_in", "mass_flow_out",
+              "exit_mach_area_averaged", "exit_pressure_ratio"):
+        assert abs(getattr(a, k) - getattr(b, k)) <= 1e-13 * abs(getattr(b, k)), k
