@@ -1503,10 +1503,13 @@ struct VTile {
     // LDS allows them (measured 2 % faster than two waves at P2), two above
     static constexpr int LDS = 8 * (WORK + NF * N * NQF + EGN + GEN + TE) + 8 * TE * NF;
     static constexpr int WPE = N <= 6 && LDS * 12 <= 160 * 1024 ? 3 : 2;
+    // HLLC and AUSM+-up differentiate the flux with Dual<4>: capped at two
+    // waves per SIMD (256 registers) they run 10 % faster than uncapped (one
+    // wave, accumulation registers in use)
 };
 
 template <class C, int FLUX>
-__global__ __launch_bounds__(DG_BS) __attribute__((amdgpu_waves_per_eu(FLUX == 0 ? VTile<C>::WPE : 1)))
+__global__ __launch_bounds__(DG_BS) __attribute__((amdgpu_waves_per_eu(FLUX == 0 ? VTile<C>::WPE : 2)))
 void k_vjpw(const nz_problem P, const double* __restrict__ T,
                                               const double* __restrict__ U,
                                               const double* __restrict__ Lin,
