@@ -1480,10 +1480,10 @@ void k_vjp(const nz_problem P, const double* __restrict__ T,
 // and reading reference tables with scalar loads.  Per face the lanes form
 // the own and neighbour traces of U and lambda in canonical point order; one
 // (point, element) item per lane pulls jw (lambda_own - lambda_nbr) back
-// through the interface flux (Roe: reverse mode with roe_vjp; HLLC and
-// AUSM+-up: forward mode, the owner's four state tangents as one Dual<4>
-// pass, as k_vjp), and the element lifts the result with the side-0 face
-// table.  Boundary faces take the pre-pass
+// through the interface flux (Roe and AUSM+-up: reverse mode with roe_vjp
+// and ausm_vjp; HLLC: forward mode, the owner's four state tangents as one
+// Dual<4> pass, as k_vjp), and the element lifts the result with the side-0
+// face table.  Boundary faces take the pre-pass
 // values (k_bvjp).  Per volume round the lanes interpolate U and the
 // reference gradients of lambda to four points; one item per lane pulls the
 // contravariant cotangents back through the Euler fluxes (euler_flux_vjp) and
@@ -1503,13 +1503,13 @@ struct VTile {
     // LDS allows them (measured 2 % faster than two waves at P2), two above
     static constexpr int LDS = 8 * (WORK + NF * N * NQF + EGN + GEN + TE) + 8 * TE * NF;
     static constexpr int WPE = N <= 6 && LDS * 12 <= 160 * 1024 ? 3 : 2;
-    // HLLC and AUSM+-up differentiate the flux with Dual<4>: capped at two
-    // waves per SIMD (256 registers) they run 10 % faster than uncapped (one
-    // wave, accumulation registers in use)
+    // HLLC differentiates the flux with Dual<4>: capped at two waves per SIMD
+    // (256 registers) it runs 10 % faster than uncapped (one wave,
+    // accumulation registers in use)
 };
 
 template <class C, int FLUX>
-__global__ __launch_bounds__(DG_BS) __attribute__((amdgpu_waves_per_eu(FLUX == 0 ? VTile<C>::WPE : 2)))
+__global__ __launch_bounds__(DG_BS) __attribute__((amdgpu_waves_per_eu(FLUX == 1 ? 2 : VTile<C>::WPE)))
 void k_vjpw(const nz_problem P, const double* __restrict__ T,
                                               const double* __restrict__ U,
                                               const double* __restrict__ Lin,
@@ -1714,7 +1714,9 @@ void k_vjpw(const nz_problem P, const double* __restrict__ T,
                     }
                     if constexpr (FLUX == 0) {
                         roe_vjp(uL, uR, eg[0], eg[1], P.gamma, P.efix, sd == 0, ob, v);
-                    } else {  // HLLC, AUSM+-up: the owner's four tangents as duals
+                    } else if constexpr (FLUX == 2) {
+                        ausm_vjp(uL, uR, eg[0], eg[1], P.gamma, sd == 0, ob, v);
+                    } else {  // HLLC: the owner's four tangents as duals
                         using D = Dual<4>;
                         D dL[4], dR[4], fx[4];
 #pragma unroll

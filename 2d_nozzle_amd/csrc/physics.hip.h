@@ -534,6 +534,111 @@ __device__ __forceinline__ void roe_vjp(const double UL[4], const double UR[4], 
     for (int k = 0; k < 4; ++k) out[k] += own_left ? -db[k] : db[k];
 }
 
+// Reverse mode of flux_ausm with respect to one side's state (the left one
+// when own_left): returns ob^T dF/dU_own.  Every statement of flux_ausm is
+// adjointed for both sides (the branches on |M| and on the sign of the
+// interface velocity as evaluated; the max as the dual numbers take it, the
+// first argument on ties), then the own side's primitive cotangents are
+// pulled back to its conserved state.
+__device__ __forceinline__ void ausm_vjp(const double UL[4], const double UR[4], double nx,
+                                         double ny, double g, bool own_left, const double ob[4],
+                                         double out[4]) {
+    // ---- forward, keeping what the backward pass reads
+    const Prim<double> L = primitives(UL, g), R = primitives(UR, g);
+    const double vnL = L.u * nx + L.v * ny, vnR = R.u * nx + R.v * ny;
+    const double aL = root(g * L.p * L.ir), aR = root(g * R.p * R.ir);
+    const double ah = 0.5 * (aL + aR), rh = 0.5 * (L.r + R.r);
+    const double iah = recip(ah);
+    const double mL = vnL * iah, mR = vnR * iah;
+    const double mb2 = 0.5 * (mL * mL + mR * mR);
+    const bool supL = fabs(mL) >= 1.0, supR = fabs(mR) >= 1.0;
+    double m4p, m4m, pp, pm;
+    // subsonic pieces: m4p = E F, m4m = G H, pp = A B, pm = C D
+    const double E = 0.25 * (mL + 1.0) * (mL + 1.0), F = 1.0 + 0.5 * (mL - 1.0) * (mL - 1.0);
+    const double G = -0.25 * (mR - 1.0) * (mR - 1.0), H = 1.0 + 0.5 * (mR + 1.0) * (mR + 1.0);
+    const double B = (2.0 - mL) + 0.75 * mL * (mL - 1.0) * (mL - 1.0);
+    const double D = (-2.0 - mR) + 0.75 * mR * (mR + 1.0) * (mR + 1.0);
+    if (supL) m4p = 0.5 * (mL + fabs(mL));
+    else m4p = (0.25 * sq(mL + 1.0)) * (1.0 - 16.0 * 0.125 * (-0.25 * sq(mL - 1.0)));
+    if (supR) m4m = 0.5 * (mR - fabs(mR));
+    else m4m = (-0.25 * sq(mR - 1.0)) * (1.0 + 16.0 * 0.125 * (0.25 * sq(mR + 1.0)));
+    const double om = 1.0 - mb2;
+    const double f = om >= 0.0 ? om : 0.0;
+    const double irh = recip(rh * ah);
+    const double dp = L.p - R.p;
+    const double uh = ah * (m4p + m4m) + 0.4 * f * dp * irh;
+    if (supL) pp = mL > 0.0 ? 1.0 : 0.0;
+    else pp = (0.25 * sq(mL + 1.0)) * ((2.0 - mL) - 16.0 * 0.1875 * mL * (-0.25 * sq(mL - 1.0)));
+    if (supR) pm = mR > 0.0 ? 0.0 : 1.0;
+    else pm = (-0.25 * sq(mR - 1.0)) * ((-2.0 - mR) + 16.0 * 0.1875 * mR * (0.25 * sq(mR + 1.0)));
+    const double dv = vnL - vnR;
+    const double PL[4] = {L.r, L.r * L.u, L.r * L.v, UL[3] + L.p};
+    const double PR[4] = {R.r, R.r * R.u, R.r * R.v, UR[3] + R.p};
+    const double au = fabs(uh);
+    // ---- backward
+    double uhb = 0.0, aub = 0.0, PLb[4], PRb[4];
+    for (int k = 0; k < 4; ++k) {
+        uhb += 0.5 * ob[k] * (PL[k] + PR[k]);
+        aub -= 0.5 * ob[k] * (PR[k] - PL[k]);
+        PLb[k] = 0.5 * ob[k] * (uh + au);
+        PRb[k] = 0.5 * ob[k] * (uh - au);
+    }
+    const double phb = ob[1] * nx + ob[2] * ny;
+    uhb += uh < 0.0 ? -aub : aub;
+    // ph = pp pL + pm pR + 0.2 pp pm rh ah (vnL - vnR)
+    double Lpb = phb * pp, Rpb = phb * pm;
+    const double ppb = phb * (L.p + 0.2 * pm * rh * ah * dv);
+    const double pmb = phb * (R.p + 0.2 * pp * rh * ah * dv);
+    double rhb = phb * 0.2 * pp * pm * ah * dv;
+    double ahb = phb * 0.2 * pp * pm * rh * dv;
+    double vnLb = phb * 0.2 * pp * pm * rh * ah, vnRb = -vnLb;
+    // uh = ah (m4p + m4m) + 0.4 f (pL - pR) / (rh ah)
+    ahb += uhb * (m4p + m4m);
+    const double m4pb = uhb * ah, m4mb = uhb * ah;
+    const double mb2b = om >= 0.0 ? -uhb * 0.4 * dp * irh : 0.0;
+    Lpb += uhb * 0.4 * f * irh;
+    Rpb -= uhb * 0.4 * f * irh;
+    const double irhb = uhb * 0.4 * f * dp;
+    rhb -= irhb * irh * irh * ah;
+    ahb -= irhb * irh * irh * rh;
+    // the split Mach numbers and pressures
+    double mLb = 0.0, mRb = 0.0;
+    if (supL) {
+        mLb += m4pb * (mL > 0.0 ? 1.0 : 0.0);
+    } else {
+        mLb += m4pb * (0.5 * (mL + 1.0) * F + E * (mL - 1.0));
+        mLb += ppb * (0.5 * (mL + 1.0) * B +
+                      E * (-1.0 + 0.75 * (mL - 1.0) * (mL - 1.0) + 1.5 * mL * (mL - 1.0)));
+    }
+    if (supR) {
+        mRb += m4mb * (mR < 0.0 ? 1.0 : 0.0);
+    } else {
+        mRb += m4mb * (-0.5 * (mR - 1.0) * H + G * (mR + 1.0));
+        mRb += pmb * (-0.5 * (mR - 1.0) * D +
+                      G * (-1.0 + 0.75 * (mR + 1.0) * (mR + 1.0) + 1.5 * mR * (mR + 1.0)));
+    }
+    mLb += mb2b * mL;
+    mRb += mb2b * mR;
+    vnLb += mLb * iah;
+    vnRb += mRb * iah;
+    const double iahb = mLb * vnL + mRb * vnR;
+    ahb -= iahb * iah * iah;
+    const double aLb = 0.5 * ahb, aRb = 0.5 * ahb;
+    // the own side from here: a = sqrt(g p ir), vn = u nx + v ny, P
+    const Prim<double>& W = own_left ? L : R;
+    const double* Pb = own_left ? PLb : PRb;
+    const double aW = own_left ? aL : aR, aWb = own_left ? aLb : aRb;
+    const double vnWb = own_left ? vnLb : vnRb;
+    double pb = own_left ? Lpb : Rpb;
+    const double zb = aWb * 0.5 / aW;
+    pb += zb * g * W.ir + Pb[3];
+    const double irb = zb * g * W.p;
+    double rb = 0.5 * rhb - irb * W.ir * W.ir + Pb[0] + Pb[1] * W.u + Pb[2] * W.v;
+    const double ub = vnWb * nx + Pb[1] * W.r, vb = vnWb * ny + Pb[2] * W.r;
+    primitives_vjp(own_left ? UL : UR, g, rb, ub, vb, pb, 0.0, out);
+    out[3] += Pb[3];
+}
+
 // FLUX: 0 roe, 1 hllc, 2 ausmup (src/backends/numba_backend.py:59-67)
 template <int FLUX, class T, class N_>
 __device__ __forceinline__ T interface_flux(const nz_problem& P, const T UL[4], const T UR[4],
