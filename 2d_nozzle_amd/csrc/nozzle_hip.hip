@@ -1480,10 +1480,9 @@ void k_vjp(const nz_problem P, const double* __restrict__ T,
 // and reading reference tables with scalar loads.  Per face the lanes form
 // the own and neighbour traces of U and lambda in canonical point order; one
 // (point, element) item per lane pulls jw (lambda_own - lambda_nbr) back
-// through the interface flux (Roe and AUSM+-up: reverse mode with roe_vjp
-// and ausm_vjp; HLLC: forward mode, the owner's four state tangents as one
-// Dual<4> pass, as k_vjp), and the element lifts the result with the side-0
-// face table.  Boundary faces take the pre-pass
+// through the interface flux with a hand-written reverse-mode pass
+// (roe_vjp, hllc_vjp, ausm_vjp), and the element lifts the result with the
+// side-0 face table.  Boundary faces take the pre-pass
 // values (k_bvjp).  Per volume round the lanes interpolate U and the
 // reference gradients of lambda to four points; one item per lane pulls the
 // contravariant cotangents back through the Euler fluxes (euler_flux_vjp) and
@@ -1503,9 +1502,8 @@ struct VTile {
     // LDS allows them (measured 2 % faster than two waves at P2), two above
     static constexpr int LDS = 8 * (WORK + NF * N * NQF + EGN + GEN + TE) + 8 * TE * NF;
     static constexpr int WPE = N <= 6 && LDS * 12 <= 160 * 1024 ? 3 : 2;
-    // HLLC differentiates the flux with Dual<4>: capped at two waves per SIMD
-    // (256 registers) it runs 10 % faster than uncapped (one wave,
-    // accumulation registers in use)
+    // HLLC's reverse pass is capped at two waves per SIMD (256 registers): at
+    // three it spills and runs 60 % slower
 };
 
 template <class C, int FLUX>
@@ -1716,21 +1714,8 @@ void k_vjpw(const nz_problem P, const double* __restrict__ T,
                         roe_vjp(uL, uR, eg[0], eg[1], P.gamma, P.efix, sd == 0, ob, v);
                     } else if constexpr (FLUX == 2) {
                         ausm_vjp(uL, uR, eg[0], eg[1], P.gamma, sd == 0, ob, v);
-                    } else {  // HLLC: the owner's four tangents as duals
-                        using D = Dual<4>;
-                        D dL[4], dR[4], fx[4];
-#pragma unroll
-                        for (int c = 0; c < 4; ++c) {
-                            dL[c] = sd == 0 ? D::seed(uL[c], c) : D(uL[c]);
-                            dR[c] = sd == 0 ? D(uR[c]) : D::seed(uR[c], c);
-                        }
-                        interface_flux<FLUX>(P, dL, dR, eg[0], eg[1], fx);
-#pragma unroll
-                        for (int j = 0; j < 4; ++j) {
-                            double a = 0.0;
-                            for (int c = 0; c < 4; ++c) a += ob[c] * fx[c].d[j];
-                            v[j] = a;
-                        }
+                    } else {
+                        hllc_vjp(uL, uR, eg[0], eg[1], P.gamma, P.low_mach, sd == 0, ob, v);
                     }
                 } else {  // boundary faces: the pre-pass's values
                     const double* b = bv + ((k - P.n_int) * NQF + q) * 4;

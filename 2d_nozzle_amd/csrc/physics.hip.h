@@ -639,6 +639,245 @@ __device__ __forceinline__ void ausm_vjp(const double UL[4], const double UR[4],
     out[3] += Pb[3];
 }
 
+// Reverse mode of flux_hllc with respect to one side's state (the left one
+// when own_left): returns ob^T dF/dU_own.  The statements of flux_hllc are
+// adjointed for both sides with the branches as evaluated (the wave-speed
+// min/max and the low-Mach blend taking the selected argument, the first on
+// ties, as the dual numbers do; the floors and away_from_zero passing no
+// derivative when active), then the own side's primitive cotangents are
+// pulled back to its conserved state.
+__device__ __forceinline__ void hllc_vjp(const double UL[4], const double UR[4], double nx,
+                                         double ny, double g, double low_mach, bool own_left,
+                                         const double ob[4], double out[4]) {
+    // ---- forward, keeping what the backward pass reads
+    const Prim<double> L = primitives(UL, g), R = primitives(UR, g);
+    const double vnL = L.u * nx + L.v * ny, vnR = R.u * nx + R.v * ny;
+    const double zL = g * L.p * L.ir, zR = g * R.p * R.ir;
+    const bool zLok = zL >= kFloor, zRok = zR >= kFloor;
+    const double aL = root(zLok ? zL : kFloor), aR = root(zRok ? zR : kFloor);
+    const double sL = root(L.r), sR = root(R.r), iden = recip(sL + sR);
+    const double nvx = sL * L.u + sR * R.u, nvy = sL * L.v + sR * R.v, nH = sL * L.H + sR * R.H;
+    const double vx = nvx * iden, vy = nvy * iden, H = nH * iden;
+    const double vnt = vx * nx + vy * ny;
+    const double cq = (g - 1.0) * (H - 0.5 * (vx * vx + vy * vy));
+    const bool cok = cq >= kFloor;
+    const double at = root(cok ? cq : kFloor);
+    const double sl1 = vnL - aL, sl2 = vnt - at, sr1 = vnR + aR, sr2 = vnt + at;
+    const bool slA = sl1 <= sl2, srA = sr1 >= sr2;
+    const double SL0 = slA ? sl1 : sl2, SR0 = srA ? sr1 : sr2;
+    double SL = SL0, SR = SR0;
+    const bool lm = low_mach > 0.0;
+    double phi = 1.0, lo = 0.0, hi = 0.0, xL = 0.0, xR = 0.0;
+    bool phiA = true, machA = true, loA = true, hiA = true;
+    if (lm) {
+        xL = fabs(vnL) / aL;
+        xR = fabs(vnR) / aR;
+        machA = xL >= xR;
+        const double mach = machA ? xL : xR;
+        const double x = mach / low_mach;
+        phiA = 1.0 <= x;
+        phi = phiA ? 1.0 : x;
+        loA = vnL <= vnt;
+        hiA = vnR >= vnt;
+        lo = loA ? vnL : vnt;
+        hi = hiA ? vnR : vnt;
+        SL = lo - phi * (lo - SL0);
+        SR = hi + phi * (SR0 - hi);
+    }
+    const double mL = L.r * (SL - vnL), mR = R.r * (SR - vnR);
+    const double num = R.r * vnR * (SR - vnR) - L.r * vnL * (SL - vnL) + L.p - R.p;
+    const double dmm = mR - mL;
+    const bool dmok = fabs(dmm) > kFloor;
+    const double den = dmok ? dmm : kFloor;
+    const double SM = num / den;
+    const bool left = (SL >= 0.0) || (SR > 0.0 && SM >= 0.0);
+    const Prim<double>& W = left ? L : R;
+    const double* U = left ? UL : UR;
+    const double vnW = left ? vnL : vnR;
+    const bool star = !(SL >= 0.0 || SR <= 0.0);
+    // cotangents of both sides' primitives (r, u, v, p, H, ir), of the selected
+    // side's conserved state (direct), and of the shared intermediates
+    double Lb[6] = {0, 0, 0, 0, 0, 0}, Rb[6] = {0, 0, 0, 0, 0, 0}, Ub[4] = {0, 0, 0, 0};
+    double* Wb = left ? Lb : Rb;
+    // ---- backward: out = F(W) [+ S (X - U)]
+    {
+        const double* fb = ob;  // normal_flux(W)
+        Wb[0] += fb[0] * vnW + fb[1] * W.u * vnW + fb[2] * W.v * vnW + fb[3] * W.H * vnW;
+        Wb[1] += fb[1] * W.r * vnW;
+        Wb[2] += fb[2] * W.r * vnW;
+        Wb[4] += fb[3] * W.r * vnW;
+        Wb[3] += fb[1] * nx + fb[2] * ny;
+        const double vb = fb[0] * W.r + fb[1] * W.r * W.u + fb[2] * W.r * W.v + fb[3] * W.r * W.H;
+        Wb[1] += vb * nx;
+        Wb[2] += vb * ny;
+    }
+    double SLb = 0.0, SRb = 0.0, SMb = 0.0, vnLb = 0.0, vnRb = 0.0;
+    if (star) {
+        const double S = left ? SL : SR;
+        const double sv = S - vnW;
+        const double dsm = S - SM;
+        const bool dok = fabs(dsm) > kFloor;
+        const double den1 = dok ? dsm : kFloor;
+        const double fac = sv / den1;
+        const double rs = W.r * fac;
+        const double d = SM - vnW;
+        const double us = W.u + d * nx, vs = W.v + d * ny;
+        const double qv = W.r * sv;
+        const bool qok = fabs(qv) > kFloor;
+        const double q = qok ? qv : kFloor;
+        const double Y = SM + W.p / q;
+        const double Es = U[3] / W.r + d * Y;
+        const double X[4] = {rs, rs * us, rs * vs, rs * Es};
+        double Sb = 0.0, Xb[4];
+        for (int k = 0; k < 4; ++k) {
+            Sb += ob[k] * (X[k] - U[k]);
+            Xb[k] = ob[k] * S;
+            Ub[k] -= ob[k] * S;
+        }
+        const double rsb = Xb[0] + Xb[1] * us + Xb[2] * vs + Xb[3] * Es;
+        const double usb = Xb[1] * rs, vsb = Xb[2] * rs, Esb = Xb[3] * rs;
+        // Es = U3 / r + d Y,  Y = SM + p / q,  q = af(r (S - vn))
+        Ub[3] += Esb / W.r;
+        Wb[0] -= Esb * U[3] / (W.r * W.r);
+        double db = Esb * Y + usb * nx + vsb * ny;
+        const double Yb = Esb * d;
+        SMb += Yb;
+        Wb[3] += Yb / q;
+        const double qb = -Yb * W.p / (q * q);
+        double svb = 0.0;
+        if (qok) {
+            Wb[0] += qb * sv;
+            svb += qb * W.r;
+        }
+        Wb[1] += usb;
+        Wb[2] += vsb;
+        // rs = r fac,  fac = (S - vn) / af(S - SM)
+        Wb[0] += rsb * fac;
+        const double facb = rsb * W.r;
+        svb += facb / den1;
+        const double den1b = -facb * sv / (den1 * den1);
+        if (dok) {
+            Sb += den1b;
+            SMb -= den1b;
+        }
+        // d = SM - vn,  sv = S - vn
+        SMb += db;
+        Sb += svb;
+        const double vnb = -db - svb;
+        if (left) {
+            SLb += Sb;
+            vnLb += vnb;
+        } else {
+            SRb += Sb;
+            vnRb += vnb;
+        }
+        // SM = num / af(mR - mL)
+        const double numb = SMb / den;
+        const double denb = -SMb * num / (den * den);
+        double mLb = 0.0, mRb = 0.0;
+        if (dmok) {
+            mRb += denb;
+            mLb -= denb;
+        }
+        Rb[0] += numb * vnR * (SR - vnR);
+        vnRb += numb * R.r * (SR - 2.0 * vnR);
+        SRb += numb * R.r * vnR;
+        Lb[0] -= numb * vnL * (SL - vnL);
+        vnLb -= numb * L.r * (SL - 2.0 * vnL);
+        SLb -= numb * L.r * vnL;
+        Lb[3] += numb;
+        Rb[3] -= numb;
+        // mL = rL (SL - vnL), mR = rR (SR - vnR)
+        Lb[0] += mLb * (SL - vnL);
+        SLb += mLb * L.r;
+        vnLb -= mLb * L.r;
+        Rb[0] += mRb * (SR - vnR);
+        SRb += mRb * R.r;
+        vnRb -= mRb * R.r;
+    }
+    // wave speeds, the low-Mach blend first
+    double SL0b = SLb, SR0b = SRb, vntb = 0.0, aLb = 0.0, aRb = 0.0;
+    if (lm) {
+        SL0b = SLb * phi;
+        SR0b = SRb * phi;
+        const double lob = SLb * (1.0 - phi), hib = SRb * (1.0 - phi);
+        const double phib = -SLb * (lo - SL0) + SRb * (SR0 - hi);
+        if (loA) vnLb += lob;
+        else vntb += lob;
+        if (hiA) vnRb += hib;
+        else vntb += hib;
+        if (!phiA) {
+            const double machb = phib / low_mach;
+            if (machA) {
+                vnLb += machb * (vnL < 0.0 ? -1.0 : 1.0) / aL;
+                aLb -= machb * xL / aL;
+            } else {
+                vnRb += machb * (vnR < 0.0 ? -1.0 : 1.0) / aR;
+                aRb -= machb * xR / aR;
+            }
+        }
+    }
+    double atb = 0.0;
+    if (slA) {
+        vnLb += SL0b;
+        aLb -= SL0b;
+    } else {
+        vntb += SL0b;
+        atb -= SL0b;
+    }
+    if (srA) {
+        vnRb += SR0b;
+        aRb += SR0b;
+    } else {
+        vntb += SR0b;
+        atb += SR0b;
+    }
+    // at = sqrt(max((g - 1)(H - (vx^2 + vy^2) / 2), floor)), vnt = vx nx + vy ny
+    double vxb = vntb * nx, vyb = vntb * ny, Hb = 0.0;
+    if (cok) {
+        const double cqb = atb * 0.5 / at;
+        Hb += (g - 1.0) * cqb;
+        vxb -= (g - 1.0) * cqb * vx;
+        vyb -= (g - 1.0) * cqb * vy;
+    }
+    // Roe means, as in roe_vjp
+    const double nvxb = vxb * iden, nvyb = vyb * iden, nHb = Hb * iden;
+    const double idenb = vxb * nvx + vyb * nvy + Hb * nH;
+    const double sumb = -idenb * iden * iden;
+    const double sLb = sumb + nvxb * L.u + nvyb * L.v + nHb * L.H;
+    const double sRb = sumb + nvxb * R.u + nvyb * R.v + nHb * R.H;
+    Lb[0] += sLb * 0.5 / sL;
+    Rb[0] += sRb * 0.5 / sR;
+    Lb[1] += nvxb * sL;
+    Lb[2] += nvyb * sL;
+    Lb[4] += nHb * sL;
+    Rb[1] += nvxb * sR;
+    Rb[2] += nvyb * sR;
+    Rb[4] += nHb * sR;
+    // a = sqrt(max(g p ir, floor)), vn = u nx + v ny
+    if (zLok) {
+        const double zb = aLb * 0.5 / aL;
+        Lb[3] += zb * g * L.ir;
+        Lb[5] += zb * g * L.p;
+    }
+    if (zRok) {
+        const double zb = aRb * 0.5 / aR;
+        Rb[3] += zb * g * R.ir;
+        Rb[5] += zb * g * R.p;
+    }
+    Lb[1] += vnLb * nx;
+    Lb[2] += vnLb * ny;
+    Rb[1] += vnRb * nx;
+    Rb[2] += vnRb * ny;
+    // the own side's primitives back to its conserved state
+    const double* Ob = own_left ? Lb : Rb;
+    const Prim<double>& O = own_left ? L : R;
+    primitives_vjp(own_left ? UL : UR, g, Ob[0] - Ob[5] * O.ir * O.ir, Ob[1], Ob[2], Ob[3], Ob[4],
+                   out);
+    if (own_left == left)
+        for (int k = 0; k < 4; ++k) out[k] += Ub[k];
+}
+
 // FLUX: 0 roe, 1 hllc, 2 ausmup (src/backends/numba_backend.py:59-67)
 template <int FLUX, class T, class N_>
 __device__ __forceinline__ T interface_flux(const nz_problem& P, const T UL[4], const T UR[4],
