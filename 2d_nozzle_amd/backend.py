@@ -279,6 +279,11 @@ class HipBackend:
             nsb = int(self.lib.nz_superbee_work_doubles(self._pp))
             self._sb_work = torch.empty(nsb, **f64)
         self._flags = torch.zeros(E, dtype=torch.uint8, device=self.device)
+        # captured march chunks (see _advance_chunk); use_graphs=False runs
+        # every chunk eagerly
+        self.use_graphs = True
+        self._graphs: dict = {}
+        self._graph_seen: set = set()
         # launch order of the element kernel's tiles: spatially coherent, and
         # one contiguous run of it per XCD, so that the neighbour columns a
         # tile gathers were just read through the same L2 (norm slots keep the
@@ -293,13 +298,28 @@ class HipBackend:
     def _stream(self):
         return C.c_void_p(self._torch.cuda.current_stream(self.device).cuda_stream)
 
-    def _in(self, A):
-        """(device tensor, was_numpy)."""
+    def _in(self, A, shape=None, what: str = "array"):
+        """(device tensor, was_numpy).
+
+        ``shape``: the shape the kernels assume, ``"state"`` for ``(E, N, 4)``
+        or ``"elem"`` for ``(E,)``.  The C ABI takes no sizes, so this is the
+        guard that keeps an array of another mesh or order from being read or
+        written past its end on the device; it raises before any transfer.
+        """
+        if shape is not None:
+            self._check_shape(A, shape, what)
         torch = self._torch
         if isinstance(A, torch.Tensor):
             t = A.to(device=self.device, dtype=torch.float64)
             return t.contiguous(), False
         return torch.from_numpy(np.ascontiguousarray(A, dtype=np.float64)).to(self.device), True
+
+    def _check_shape(self, A, shape, what: str) -> None:
+        want = {"state": (self.E, self.N, 4), "elem": (self.E,)}.get(shape, shape)
+        got = tuple(np.shape(A))
+        if got != tuple(want):
+            raise ValueError(f"{what} must have shape {tuple(want)} for this mesh and order, "
+                             f"got {got}")
 
     def _out(self, t, as_numpy: bool):
         return t.cpu().numpy() if as_numpy else t
@@ -340,7 +360,7 @@ class HipBackend:
     # ------------------------------------------------------------ primitives
     def residual(self, U):
         """``(R, wave)`` with ``M dU/dt = -R`` (``assembly.py:153-157``)."""
-        u, npy = self._in(U)
+        u, npy = self._in(U, "state", "U")
         R = self._torch.empty_like(u)
         wave = self._torch.empty(self.E, dtype=u.dtype, device=self.device)
         self._chk(self.lib.nz_residual(self._pp, self._p(u), self._p(R), self._p(wave),
@@ -349,7 +369,7 @@ class HipBackend:
 
     def rate(self, U):
         """``-M^-1 R`` and the wave-speed sum, in one pass (``element_rate``)."""
-        u, npy = self._in(U)
+        u, npy = self._in(U, "state", "U")
         Fr = self._torch.empty_like(u)
         wave = self._torch.empty(self.E, dtype=u.dtype, device=self.device)
         self._chk(self.lib.nz_rate(self._pp, self._p(u), self._p(Fr), self._p(wave),
@@ -357,7 +377,7 @@ class HipBackend:
         return self._out(Fr, npy), self._out(wave, npy)
 
     def inverse_mass(self, R):
-        r, npy = self._in(R)
+        r, npy = self._in(R, "state", "R")
         out = self._torch.empty_like(r)
         self._chk(self.lib.nz_inverse_mass(self._pp, self._p(r), self._p(out), self._stream()),
                   "nz_inverse_mass")
@@ -366,10 +386,8 @@ class HipBackend:
     def project(self, values):
         """L2 projection of point values ``(E, nqv, 4)`` onto the basis, on the
         device (``project``, ``src/timestepping/initialize.py:41-46``)."""
-        v, npy = self._in(values)
         rd = self.ops.ref
-        if tuple(v.shape) != (self.E, rd.n_qvol, 4):
-            raise ValueError(f"values must have shape {(self.E, rd.n_qvol, 4)}, got {tuple(v.shape)}")
+        v, npy = self._in(values, (self.E, rd.n_qvol, 4), "values")
         out = self._torch.empty((self.E, rd.n_basis, 4), dtype=self._torch.float64,
                                 device=self.device)
         self._chk(self.lib.nz_project(self._pp, self._p(v), self._p(out), self._stream()),
@@ -382,7 +400,7 @@ class HipBackend:
         ``src/post/postprocess.py:193-226``)."""
         from .observables import Performance
 
-        u, _ = self._in(U)
+        u, _ = self._in(U, "state", "U")
         out = self._torch.empty(8, dtype=self._torch.float64, device=self.device)
         self._chk(self.lib.nz_boundary_integrals(self._pp, self._p(u), self._p(out), self._stream()),
                   "nz_boundary_integrals")
@@ -400,7 +418,7 @@ class HipBackend:
 
     def limit(self, U):
         """Limited copy of ``U``; activity counters as ``numba_backend.py:265-280``."""
-        u, npy = self._in(U)
+        u, npy = self._in(U, "state", "U")
         out = u.clone()
         if self.limiting:
             self._counters.zero_()
@@ -430,7 +448,7 @@ class HipBackend:
         ``flags`` bit 0: the cell mean was repaired; bit 1: a probe violated
         the density or pressure bound.  Counters are not accumulated here.
         """
-        u, npy = self._in(U)
+        u, npy = self._in(U, "state", "U")
         out = u.clone()
         cnt = self._torch.zeros(2, dtype=self._torch.int64, device=self.device)
         flags = self._torch.zeros(self.E, dtype=self._torch.uint8, device=self.device)
@@ -441,7 +459,9 @@ class HipBackend:
         return self._out(out, npy), self._out(flags, npy), (int(c[0]), int(c[1]))
 
     def time_step(self, wave_sum):
-        w, npy = self._in(wave_sum)
+        if tuple(np.shape(wave_sum)) == (self.E, 1, 1):
+            wave_sum = wave_sum.reshape(self.E)
+        w, npy = self._in(wave_sum, "elem", "wave_sum")
         dt = self._torch.empty_like(w)
         self._chk(self.lib.nz_time_step(self._pp, self._p(w), self.dt_factor(), self._p(dt),
                                         self._stream()), "nz_time_step")
@@ -449,7 +469,7 @@ class HipBackend:
         return self._out(dt, npy)
 
     def norm(self, A) -> float:
-        a, _ = self._in(A)
+        a, _ = self._in(A, "state", "A")
         self._chk(self.lib.nz_norm(self._pp, self._p(a), self._p(self._scratch),
                                    self._p(self._res), self._stream()), "nz_norm")
         return float(self._res.item())
@@ -500,17 +520,44 @@ class HipBackend:
                   "nz_norm_finish")
         return cur
 
+    def _advance_chunk(self, n_steps: int, scheme: str):
+        """``n_steps`` steps from ``_S[0]``; returns the buffer holding the result.
+
+        A chunk length seen before (same scheme, same problem descriptor and
+        buffers) is replayed from a HIP graph captured on its second call, so
+        a march's ``check_interval`` chunks cost one graph launch each instead
+        of one host call per kernel.  The first call of each chunk length runs
+        eagerly.  The graph records kernels only (no memset nodes).
+        """
+        cur, nxt = self._S[0], self._S[1]
+        if not self.use_graphs or n_steps < 2:
+            return self.advance(cur, nxt, n_steps, scheme)
+        key = (n_steps, scheme, bytes(self.P), self._scratch.data_ptr(),
+               self.dt_factor(scheme))
+        g = self._graphs.get(key)
+        if g is None:
+            if key not in self._graph_seen:
+                self._graph_seen.add(key)
+                return self.advance(cur, nxt, n_steps, scheme)
+            torch = self._torch
+            g = torch.cuda.CUDAGraph()
+            torch.cuda.current_stream(self.device).synchronize()
+            with torch.cuda.graph(g):
+                self.advance(cur, nxt, n_steps, scheme)
+            self._graphs[key] = g
+        g.replay()
+        return cur if n_steps % 2 == 0 else nxt
+
     def run(self, U, n_steps: int, scheme: str):
         """Advance ``n_steps`` steps; returns ``(U, residual_norm_of_last_step)``."""
         if scheme not in SCHEMES:
             raise ValueError(f"unknown scheme {scheme!r}; choose from {SCHEMES}")
-        u, npy = self._in(U)
+        u, npy = self._in(U, "state", "U")
         if n_steps <= 0:
             return self._out(u.clone(), npy), float("nan")
-        cur, nxt = self._S[0], self._S[1]
-        cur.copy_(u)
+        self._S[0].copy_(u)
         self._counters.zero_()
-        res_buf = self.advance(cur, nxt, n_steps, scheme)
+        res_buf = self._advance_chunk(n_steps, scheme)
         out = res_buf.clone()
         res = float(self._res.item())
         if self.limiting:
@@ -545,6 +592,8 @@ class HipBackend:
         ``(F, nqf, 3)`` and of the back pressure (a float).
         """
         torch = self._torch
+        self._check_shape(U, "state", "U")
+        self._check_shape(lam, "state", "lam")
         u, npy = self._in(U)
         l, _ = self._in(lam)
         a = self._adj_buffers()
@@ -576,6 +625,8 @@ class HipBackend:
             self.P.vjp_kernel = _VJP_KERNEL[self._vjp_key]
             return self.P.vjp_kernel
         torch = self._torch
+        self._check_shape(U, "state", "U")
+        self._check_shape(lam, "state", "lam")
         u, _ = self._in(U)
         lm, _ = self._in(lam)
         a = self._adj_buffers()
@@ -647,6 +698,8 @@ class HipBackend:
         if method not in ("rk4", "gmres"):
             raise ValueError(f"unknown adjoint method {method!r}; choose rk4 or gmres")
         torch = self._torch
+        self._check_shape(U, "state", "U")
+        self._check_shape(seed, "state", "seed")
         u, npy = self._in(U)
         s, _ = self._in(seed)
         self.tune_vjp(u, s)

@@ -16,9 +16,10 @@ marched at once:
   norms (``nz_norm_segments``) are summed in the same order as a lone solve's,
   so a design marched in the batch takes the same iterations and reaches the
   same state, bit for bit, as when it is marched alone.
-* The march control (``src/timestepping/march.py:125-230``): one step for the
-  initial residual, then chunks of ``check_interval`` steps with the
-  non-finite, tolerance, divergence and stall checks per design.  A design
+* The march control is ``march.StopRules`` per design (the reference's rules,
+  ``src/timestepping/march.py:125-230``): one step for the initial residual,
+  then chunks of ``check_interval`` steps with the non-finite, tolerance,
+  divergence and stall checks.  A design
   that stops is copied out at that step and its tiles leave the table, so it
   costs nothing afterwards (per-design convergence masking).
 * Each family runs on its own HIP stream, so small families fill the device
@@ -39,6 +40,7 @@ import numpy as np
 
 from .config import FlowConditions, SolverOptions, residual_scale
 from .initial import quasi1d_initial
+from .march import StopRules
 from .mesh import NozzleGeometry, build_nozzle_mesh
 from .operators import build_operators
 
@@ -299,7 +301,6 @@ class BatchedSteady:
 
         opts = self.opts
         scale = residual_scale(self.flow, 1.0)
-        tol = opts.tolerance * scale
         chunk = min(max(1, opts.check_interval), opts.max_iterations)
         results: list[DesignResult | None] = [None] * len(self.designs)
         state = {}  # global index -> dict(res0, window)
@@ -320,43 +321,16 @@ class BatchedSteady:
                 for j, d in enumerate(fam.active):
                     g, r = fam.idx[d], float(res[j])
                     if first:
-                        state[g] = {"res0": r, "window": []}
-                        if not np.isfinite(r):
-                            finish(fam, d, False, done, r, "initial residual is not finite")
-                        elif r <= tol:
-                            finish(fam, d, True, done, r, "")
-                        else:
-                            keep.append(d)
-                        continue
-                    st = state[g]
-                    if not np.isfinite(r):
-                        finish(fam, d, False, done, r,
-                               f"residual became non-finite after {done} iterations; reduce cfl "
-                               "(try 0.5) or enable limiter='superbee'")
-                    elif r <= tol:
-                        finish(fam, d, True, done, r, "")
-                    elif r > opts.divergence_factor * st["res0"]:
-                        finish(fam, d, False, done, r,
-                               f"residual grew by {r / st['res0']:.1e}x after {done} iterations, "
-                               f"which is past divergence_factor={opts.divergence_factor:g}; "
-                               "reduce cfl or start from initial_condition='quasi1d'")
+                        state[g] = {"res0": r, "rules": StopRules(opts, scale)}
+                        stop = state[g]["rules"].begin(r)
                     else:
-                        w = st["window"]
-                        w.append(r)
-                        if len(w) > opts.stall_window:
-                            w.pop(0)
-                        if (len(w) == opts.stall_window and max(w) > 0.0
-                                and min(w) / max(w) > opts.stall_ratio):
-                            finish(fam, d, False, done, r,
-                                   f"the residual has stalled at {r / scale:.3e} (scaled; target "
-                                   f"{opts.tolerance:g})")
-                        elif done >= opts.max_iterations:
-                            finish(fam, d, False, done, r,
-                                   f"hit max_iterations={opts.max_iterations} at scaled residual "
-                                   f"{r / scale:.2e} (target {opts.tolerance:g}); raise "
-                                   "max_iterations or relax tolerance")
-                        else:
-                            keep.append(d)
+                        stop = state[g]["rules"].check(done, r)
+                        if stop is None and done >= opts.max_iterations:
+                            stop = state[g]["rules"].budget(done)
+                    if stop is None:
+                        keep.append(d)
+                    else:
+                        finish(fam, d, stop.converged, done, r, stop.message)
                 if keep != fam.active:
                     fam.active = keep
                     fam._set_tiles()

@@ -1,30 +1,47 @@
-"""Residual-kernel microbenchmark of the DG path on one MI355X.
+"""Benchmark of the DG path on one MI355X: one JSON line.
 
-Workload: the synthetic converging-diverging channel (wall
-y = 1 + 1.5 (x - 0.3)^2 on x in [0, 1]) triangulated with 1024 x 512 cells =
-2^20 triangles at P3 (10 nodes per element), FP64, seeded smooth state
-(seed 0: density and pressure in [0.5, 2], local Mach in [0, 3]).  A "step" is
-one residual evaluation -M^-1 R(U) (``nz_rate``: boundary pre-pass + the fused
-element kernel) with the state resident in HBM.  It is timed for each of the
-Roe, HLLC and AUSM+-up fluxes; ``value`` is the Roe figure.
+Headline (``value``): the residual-kernel microbenchmark.  Workload: the
+synthetic converging-diverging channel (wall y = 1 + 1.5 (x - 0.3)^2 on
+x in [0, 1]) triangulated with 1024 x 512 cells = 2^20 triangles at P3
+(10 nodes per element), FP64, seeded smooth state (seed 0: density and
+pressure in [0.5, 2], local Mach in [0, 3]).  A "step" is one residual
+evaluation -M^-1 R(U) (``nz_rate``: boundary pre-pass + the element kernel)
+with the state resident in HBM, timed for each of the Roe, HLLC and AUSM+-up
+fluxes; ``value`` is the Roe figure in Gdof-stage/s = E N / t_eval / 1e9.
+
+The rest of the metric rides in the same line:
+
+* ``roofline``: algorithmic bytes of one evaluation
+  (``workloads.residual_bytes``) over its average duration from HIP events on
+  the launch stream, against the 8 TB/s HBM peak; ``traffic`` is the HBM
+  bytes per evaluation measured by a separate counter run (rocprofv3 --pmc
+  FETCH_SIZE, WRITE_SIZE), recorded in ``profiles/traffic.json`` and named by
+  ``traffic_source``; ``hbm_copy_gbs`` is a 1 GiB device copy timed the same
+  way.
+* ``march``: the fused SSP-RK3 stage loop on the same mesh and state.
+* ``config1``: time to solution of the smooth nozzle (P1 triangles, refine
+  1, Roe, tolerance 1e-10) on the device, with the stop iteration and the
+  relative error of thrust, mass flow and exit Mach against the reference's
+  converged figures (the committed fixture), and the same march on the CPU
+  oracle.
+* ``sweep``: designs/s and gradients/s of the 256-design batched sweep
+  against one-at-a-time solves.
+* ``adjoint``: gradients/s of the eight Bezier-wall design derivatives of
+  thrust on 200,704 P2 triangles.
+* ``cpu_baseline``: the CPU oracle (an OpenMP restatement of the reference's
+  compiled kernels), strict IEEE and -ffast-math builds, every host core and
+  one thread, median of 5 repeats of at least 50 evaluations on a bounded
+  sample of the channel.
+
+Each timed region follows ``--warmup`` untimed calls plus a steady-state
+warm-up (windows of calls until two consecutive windows agree within 2 % and
+at least ``--min-warm-s`` seconds have passed); the line reports it.  Legs
+after the headline are skipped, and say so, when the elapsed time passes
+``--budget-s``.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-march]
-                    [--shocked] [--adjoint] [--adjoint-stages K] [--sweep N]
-
-Prints one JSON line.  ``value`` is Gdof-stage/s = E N / t_eval / 1e9.
-``roofline`` prices the residual against HBM: the algorithmic bytes of one
-evaluation (``workloads.residual_bytes``) over its average duration from HIP
-events on the launch stream; ``traffic`` is the measured HBM bytes per
-evaluation when a counter profile of this workload is recorded in
-``profiles/traffic.json``.  ``hbm_copy_gbs`` is a device-to-device copy of
-1 GiB timed the same way (the attainable-bandwidth reference).  ``march``
-reports the fused SSP-RK3 stage loop on the same mesh and state (positivity
-limiter on).  ``cpu_baseline`` times the CPU oracle (an OpenMP restatement of
-the reference's compiled kernels) on a bounded sample of the same workload.
-The optional legs add the shocked quadrilateral march (``--shocked``), the
-Bezier-wall design gradients (``--adjoint``), the per-step cost of the
-adjoint march on that mesh (``--adjoint-stages``) and the batched design
-sweep with its gradients (``--sweep``), each under a key of its own.
+                    [--no-config1] [--no-sweep] [--no-adjoint] [--shocked]
+                    [--adjoint-stages K] [--adjoint-fd]
 """
 
 from __future__ import annotations
@@ -60,46 +77,195 @@ def _traffic(workload: str):
     return data.get(workload, {}).get("bytes_per_launch")
 
 
-def cpu_baseline(order: int, ncx: int, ncy: int, budget_s: float, flux: str = "roe") -> dict:
-    """The CPU oracle on a bounded sample: the same channel at ncx x ncy cells,
-    timing the same unit of work (one residual evaluation -M^-1 R)."""
-    from oracle.oracle import CpuOracle
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(order: int, budget_s: float, flux: str = "roe") -> dict:
+    """The CPU oracle on a bounded sample of the channel, timing the same unit
+    of work as the headline (one residual evaluation -M^-1 R).
+
+    Four variants: the strict-IEEE and the -ffast-math build (the reference's
+    compiled kernels run with fastmath), each on every host core this process
+    may use and on one thread.  Each variant is the median of 5 repeats of at
+    least 50 evaluations after two warm-up evaluations, on the largest
+    channel sample (256 x 128, 128 x 64 or 64 x 32 cells) whose 5 x 50
+    evaluations fit ``budget_s``.  ``value`` is the fast-math build on every
+    core, the configuration the reference's own CPU backend runs in.
+    """
+    from oracle.oracle import CpuOracle, set_threads
 
     wl, cf = _pkg("workloads"), _pkg("config")
-    ops, U = wl.channel_case(ncx, ncy, order, dense=True)
-    orc = CpuOracle(ops, cf.FlowConditions(flux=flux))
-    orc.rate(U)  # warm-up
-    evals, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < budget_s:
-        orc.rate(U)
-        evals += 1
-    dt = time.perf_counter() - t0
     cores = len(os.sched_getaffinity(0))
+    flow = cf.FlowConditions(flux=flux)
+    cases: dict = {}
+    variants = {}
+    for fast in (True, False):
+        for threads in (cores, 1):
+            set_threads(threads)
+            pick = None
+            for cells in ((256, 128), (128, 64), (64, 32)):
+                if cells not in cases:
+                    cases[cells] = wl.channel_case(cells[0], cells[1], order, dense=True)
+                ops, U = cases[cells]
+                orc = CpuOracle(ops, flow, fast=fast)
+                orc.rate(U)
+                orc.rate(U)
+                t0 = time.perf_counter()
+                orc.rate(U)
+                one = time.perf_counter() - t0
+                pick = (cells, ops, U, orc, one)
+                if 5 * 50 * one <= budget_s:
+                    break
+            cells, ops, U, orc, one = pick
+            n_eval = max(50, int(budget_s / 5 / max(one, 1e-9) * 0.5))
+            n_eval = min(n_eval, 2000)
+            reps = []
+            for _ in range(5):
+                t0 = time.perf_counter()
+                for _ in range(n_eval):
+                    orc.rate(U)
+                reps.append(time.perf_counter() - t0)
+            med = float(np.median(reps))
+            key = f"{'fast' if fast else 'strict'}_{'all' if threads == cores else '1'}thread"
+            variants[key] = {
+                "gdof_stage_s": round(n_eval * ops.n_elem * ops.ref.n_basis / med / 1e9, 5),
+                "threads": threads, "sample_cells": f"{cells[0]}x{cells[1]}",
+                "elements": ops.n_elem, "evaluations_per_repeat": n_eval,
+                "median_s": round(med, 4), "repeats_s": [round(r, 4) for r in reps],
+            }
+    set_threads(cores)
+    head = variants[f"fast_allthread"]
     return {
-        "value": evals * ops.n_elem * ops.ref.n_basis / dt / 1e9,
+        "value": head["gdof_stage_s"],
         "unit": "Gdof-stage/s",
-        "cores": int(os.environ.get("OMP_NUM_THREADS", cores)),
+        "cores": cores,
         "kind": "port",
-        "sample": f"{ncx}x{ncy}-cell channel, {ops.n_elem} P{order} triangles, {evals} residual "
-                  f"evaluations ({flux}) in {dt:.1f} s with the C/OpenMP restatement of the "
-                  "reference's CPU kernels (edge_pass + element_rate)",
+        "cpu_model": _cpu_model(),
+        "sample": f"{head['sample_cells']}-cell channel ({head['elements']} P{order} triangles), "
+                  f"median of 5 repeats of {head['evaluations_per_repeat']} residual evaluations "
+                  f"({flux}) with the -ffast-math C/OpenMP restatement of the reference's CPU "
+                  f"kernels (edge_pass + element_rate) on {cores} threads",
+        "variants": variants,
     }
 
 
-def _timed(fn, steps: int, warmup: int, stream) -> float:
-    """Seconds for ``steps`` calls of ``fn`` (HIP events on ``stream``)."""
+def _window(fn, n: int, stream) -> float:
+    import torch
+
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(n):
+        fn()
+    ev1.record(stream)
+    ev1.synchronize()
+    return ev0.elapsed_time(ev1) / 1e3
+
+
+def _timed(fn, steps: int, warmup: int, stream, min_warm_s: float = 1.0,
+           max_warm_s: float = 8.0, agree: float = 0.02):
+    """``(seconds for steps calls, warm-up record)``, from HIP events on ``stream``.
+
+    After ``warmup`` calls, windows of calls run until two consecutive windows
+    agree within ``agree`` and at least ``min_warm_s`` seconds have passed (at
+    most ``max_warm_s``), so the timed calls see the device's steady state
+    whatever ``warmup`` is; then exactly ``steps`` calls are timed between a
+    device synchronisation on each side.
+    """
     import torch
 
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ev0.record(stream)
-    for _ in range(steps):
-        fn()
-    ev1.record(stream)
+    n_win = max(steps, 10)
+    t0, calls, prev, settled = time.perf_counter(), warmup, None, False
+    while True:
+        w = _window(fn, n_win, stream) / n_win
+        calls += n_win
+        el = time.perf_counter() - t0
+        if prev is not None and abs(w - prev) <= agree * prev and el >= min_warm_s:
+            settled = True
+            break
+        if el >= max_warm_s:
+            break
+        prev = w
     torch.cuda.synchronize()
-    return ev0.elapsed_time(ev1) / 1e3
+    t = _window(fn, steps, stream)
+    torch.cuda.synchronize()
+    return t, {"calls": calls, "seconds": round(el, 2), "settled": settled}
+
+
+def config1_bench() -> dict:
+    """Time to solution of BASELINE config 1: the smooth nozzle (area ratio
+    2.5, P1 triangles, refine 1, Roe, positivity limiter, SSP-RK3) from the
+    quasi-1D state to a scaled residual of 1e-10, on the device and on the CPU
+    oracle, with each result's relative error against the reference's
+    converged figures."""
+    import torch
+
+    from oracle.oracle import CpuOracle
+
+    mesh, opm, cf, ini, mr, ob = (_pkg("mesh"), _pkg("operators"), _pkg("config"),
+                                  _pkg("initial"), _pkg("march"), _pkg("observables"))
+    geom = mesh.NozzleGeometry(contour="smooth", area_ratio=2.5)
+    topo, coords, _ = mesh.build_nozzle_mesh(geom, refine=1)
+    ops = opm.build_operators(topo, coords, 1, dense=True)
+    flow = cf.FlowConditions()
+    opts = cf.SolverOptions(tolerance=1e-10, print_interval=0)
+    U0 = ini.quasi1d_initial(ops, flow, geom)
+    with np.load(os.path.join(ROOT, "tests", "golden", "cfg1_converged.npz")) as g:
+        ref = {k: float(g[k]) for k in ("perf_thrust", "perf_mass_flow_in", "perf_mass_flow_out",
+                                         "perf_exit_mach_area_averaged", "iterations")}
+    names = {"thrust": "perf_thrust", "mass_flow_in": "perf_mass_flow_in",
+             "mass_flow_out": "perf_mass_flow_out",
+             "exit_mach": "perf_exit_mach_area_averaged"}
+
+    def errors(perf):
+        got = {"thrust": perf.thrust, "mass_flow_in": perf.mass_flow_in,
+               "mass_flow_out": perf.mass_flow_out, "exit_mach": perf.exit_mach_area_averaged}
+        return {k: float(abs(got[k] / ref[v] - 1)) for k, v in names.items()}
+
+    mr.solve_steady(ops, flow, cf.SolverOptions(max_iterations=101, print_interval=0), U0=U0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = mr.solve_steady(ops, flow, opts, U0=U0, backend="hip")
+    torch.cuda.synchronize()
+    t_hip = time.perf_counter() - t0
+
+    class _Oracle:
+        scheme = "ssprk3"
+
+        def __init__(self):
+            self.o = CpuOracle(ops, flow, fast=False)
+            self.fac = cf.step_coefficient(1, opts.cfl, "ssprk3")
+
+        def run(self, U, n, scheme):
+            return self.o.run(U, n, scheme, self.fac)
+
+        def limiter_activity(self):
+            return float("nan")
+
+    t0 = time.perf_counter()
+    Uo, _, conv_o, it_o, _, _, _ = mr.march(U0, _Oracle(), opts, cf.residual_scale(flow, 1.0))
+    t_orc = time.perf_counter() - t0
+    return {
+        "workload": "nozzle-smooth-ar2.5-tri-p1-refine1-roe-tol1e-10", "elements": ops.n_elem,
+        "converged": bool(res.converged), "iterations": int(res.iterations),
+        "reference_iterations": int(ref["iterations"]),
+        "hip_seconds": round(t_hip, 4),
+        "hip_rel_err": errors(ob.performance(res.U, ops, flow)),
+        "oracle_seconds": round(t_orc, 3), "oracle_iterations": int(it_o),
+        "oracle_threads": len(os.sched_getaffinity(0)),
+        "oracle_rel_err": errors(ob.performance(Uo, ops, flow)),
+        "speedup_vs_oracle": round(t_orc / t_hip, 2),
+    }
 
 
 def shocked_bench(refine: int, steps: int) -> dict:

@@ -20,6 +20,7 @@
  * (tests/golden/make_goldens.py).
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <string.h>
 
@@ -594,4 +595,12 @@ void or_superbee(i64 nelem, int nbf, int nqf, int nface, double *U, const double
                 *u = ub[e * 4 + s] + theta * (*u - ub[e * 4 + s]);
             }
     }
+}
+
+/* OpenMP threads of the parallel loops above (timing of the CPU baseline at
+ * one thread and at every core); returns the previous maximum. */
+int or_set_threads(int n) {
+    const int prev = omp_get_max_threads();
+    if (n > 0) omp_set_num_threads(n);
+    return prev;
 }

@@ -63,8 +63,16 @@ def load(fast: bool = False) -> C.CDLL:
                                         C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
                                         C.c_double, P, P]
     lib.or_superbee.argtypes = [i64, C.c_int, C.c_int, C.c_int, P, P, P, P, P, P, C.c_double, P]
+    lib.or_set_threads.argtypes = [C.c_int]
+    lib.or_set_threads.restype = C.c_int
     _LIBS[name] = lib
     return lib
+
+
+def set_threads(n: int, fast: bool = False) -> int:
+    """OpenMP threads of the oracle's loops (0: leave as is); returns the
+    previous maximum."""
+    return int(load(fast).or_set_threads(int(n)))
 
 
 def _p(a: np.ndarray) -> int:

@@ -43,3 +43,27 @@ def test_converged_smooth_nozzle():
         assert v <= 1e-10, (k, v)
     assert rel_err(res.U, g["U"]) <= 1e-9
     assert res.mean_repairs == int(g["mean_repairs"])
+
+
+@pytest.mark.parametrize("limiter", ["positivity", "superbee"])
+def test_graph_chunks_match_eager(limiter):
+    """``run`` replays a captured HIP graph for a chunk length it has seen
+    before.  Every replay must give the eager chunk's state and residual bit
+    for bit, with other eager work on the same backend between replays."""
+    cf = nz_module("config")
+    be = nz_module("backend")
+    ops, flow = build_case("cfg1", dense=False)
+    U0 = golden("cfg1")["U0"]
+    opts = cf.SolverOptions(limiter=limiter)
+    eager = be.HipBackend(ops, flow, opts)
+    eager.use_graphs = False
+    graph = be.HipBackend(ops, flow, opts)
+    Ue, Ug = U0, U0
+    for k in range(5):
+        Ue, re = eager.run(Ue, 50, "ssprk3")
+        Ug, rg = graph.run(Ug, 50, "ssprk3")
+        assert np.array_equal(Ue, Ug) and re == rg, k
+        assert len(graph._graphs) == (0 if k == 0 else 1)
+        graph.residual(Ug)  # eager work between replays
+        graph.run(Ug, 1, "ssprk3")
+    assert graph.n_limit_calls == eager.n_limit_calls + 5 * 3

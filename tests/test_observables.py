@@ -52,3 +52,75 @@ def test_march_logic_on_the_oracle():
 
     U, hist, conv, iters, res, res0, msg = mr.march(U0, Diverging(), opts, 1.0)
     assert not conv and ("non-finite" in msg or "grew" in msg)
+
+
+class _ScriptedBackend:
+    """A backend whose residuals follow a script: the march's stop rules alone."""
+
+    scheme = "ssprk3"
+
+    def __init__(self, residuals, activity=float("nan")):
+        self.r = list(residuals)
+        self.calls = []
+        self.act = activity
+
+    def run(self, U, n, scheme):
+        self.calls.append(n)
+        return U, self.r.pop(0)
+
+    def limiter_activity(self):
+        return self.act
+
+
+def _march(residuals, activity=float("nan"), **kw):
+    cf, mr = nz_module("config"), nz_module("march")
+    opts = cf.SolverOptions(print_interval=0, **kw)
+    bk = _ScriptedBackend(residuals, activity)
+    out = mr.march(0, bk, opts, 2.0)
+    return out, bk.calls
+
+
+def test_stop_rules_each_rule():
+    import math
+
+    # tolerance 1e-6 at scale 2: stops when res <= 2e-6
+    (U, h, conv, it, res, res0, msg), calls = _march([1.0, 0.5, 1e-6], tolerance=1e-6,
+                                                     check_interval=10)
+    assert conv and it == 21 and msg == "" and calls == [1, 10, 10]
+    assert h.iterations == [1, 11, 21] and res == 1e-6 and res0 == 1.0
+    # converged at the first step
+    (_, _, conv, it, _, _, msg), calls = _march([1e-7], tolerance=1e-6)
+    assert conv and it == 1 and calls == [1]
+    # non-finite first residual and later residual
+    (_, _, conv, it, _, _, msg), _ = _march([math.nan])
+    assert not conv and it == 1 and msg == "initial residual is not finite"
+    (_, _, conv, it, _, _, msg), _ = _march([1.0, math.inf], check_interval=5)
+    assert not conv and it == 6 and "non-finite after 6 iterations" in msg
+    # divergence: past divergence_factor times the first residual
+    (_, _, conv, it, _, _, msg), _ = _march([1.0, 2.0, 101.0], divergence_factor=100.0,
+                                            check_interval=5)
+    assert not conv and it == 11 and "grew by 1.0e+02x" in msg
+    # budget: the last chunk is cut to the budget
+    (_, _, conv, it, res, _, msg), calls = _march([1.0, 0.9, 0.5], max_iterations=15,
+                                                  check_interval=10, tolerance=1e-9)
+    assert not conv and it == 15 and calls == [1, 10, 4] and "max_iterations=15" in msg
+    assert res == 0.5
+
+
+def test_stop_rules_stall_and_precedence():
+    flat = [1.0] + [0.5 + 1e-4 * k for k in range(10)]
+    (_, _, conv, it, _, _, msg), _ = _march(flat, stall_window=4, stall_ratio=0.98,
+                                            check_interval=3, tolerance=1e-12)
+    assert not conv and it == 1 + 4 * 3 and "stalled" in msg and "smaller cfl" in msg
+    (_, _, conv, _, _, _, msg), _ = _march(flat, activity=2.5, stall_window=4, stall_ratio=0.98,
+                                           check_interval=3, tolerance=1e-12)
+    assert "limit cycle" in msg and "~2.5 element(s)" in msg and "Refine (refine=+1)" in msg
+    # a residual that meets the tolerance wins over a full stall window
+    (_, _, conv, it, _, _, _), _ = _march([1.0, 1e-9, 1e-9, 1e-9], stall_window=2,
+                                          tolerance=1e-9, check_interval=2)
+    assert conv and it == 3
+    # the window holds only the last stall_window checks
+    falling = [1.0, 0.9, 0.5, 0.499, 0.498, 0.497]
+    (_, _, conv, it, _, _, msg), _ = _march(falling, stall_window=3, stall_ratio=0.98,
+                                            check_interval=1, tolerance=1e-12)
+    assert "stalled" in msg and it == 5
