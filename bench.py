@@ -135,6 +135,8 @@ def cpu_baseline(order: int, budget_s: float, flux: str = "roe") -> dict:
                 reps.append(time.perf_counter() - t0)
             med = float(np.median(reps))
             key = f"{'fast' if fast else 'strict'}_{'all' if threads == cores else '1'}thread"
+            print(f"  cpu {key}: {cells[0]}x{cells[1]} cells, median {med:.3f} s for {n_eval} "
+                  "evaluations", file=sys.stderr, flush=True)
             variants[key] = {
                 "gdof_stage_s": round(n_eval * ops.n_elem * ops.ref.n_basis / med / 1e9, 5),
                 "threads": threads, "sample_cells": f"{cells[0]}x{cells[1]}",
@@ -142,7 +144,7 @@ def cpu_baseline(order: int, budget_s: float, flux: str = "roe") -> dict:
                 "median_s": round(med, 4), "repeats_s": [round(r, 4) for r in reps],
             }
     set_threads(cores)
-    head = variants[f"fast_allthread"]
+    head = variants["fast_allthread"]
     return {
         "value": head["gdof_stage_s"],
         "unit": "Gdof-stage/s",
@@ -170,14 +172,15 @@ def _window(fn, n: int, stream) -> float:
 
 
 def _timed(fn, steps: int, warmup: int, stream, min_warm_s: float = 1.0,
-           max_warm_s: float = 8.0, agree: float = 0.02):
+           max_warm_s: float = 8.0, agree: float = 0.02, reset=None):
     """``(seconds for steps calls, warm-up record)``, from HIP events on ``stream``.
 
     After ``warmup`` calls, windows of calls run until two consecutive windows
     agree within ``agree`` and at least ``min_warm_s`` seconds have passed (at
     most ``max_warm_s``), so the timed calls see the device's steady state
     whatever ``warmup`` is; then exactly ``steps`` calls are timed between a
-    device synchronisation on each side.
+    device synchronisation on each side.  ``reset`` (optional) runs untimed
+    just before the timed calls, e.g. to restore a marched state.
     """
     import torch
 
@@ -196,6 +199,8 @@ def _timed(fn, steps: int, warmup: int, stream, min_warm_s: float = 1.0,
         if el >= max_warm_s:
             break
         prev = w
+    if reset is not None:
+        reset()
     torch.cuda.synchronize()
     t = _window(fn, steps, stream)
     torch.cuda.synchronize()
@@ -287,10 +292,13 @@ def shocked_bench(refine: int, steps: int) -> dict:
     E, N = ops.n_elem, ops.ref.n_basis
     print(f"shocked: {E} quads Q2 P2, host setup {time.perf_counter() - t0:.1f} s",
           file=sys.stderr, flush=True)
-    cur, nxt = bk._S[0], bk._S[1]
-    cur.copy_(torch.from_numpy(U0).to(bk.device))
-    bk._counters.zero_()
-    state = {"cur": cur, "nxt": nxt}
+    u0 = torch.from_numpy(U0).to(bk.device)
+    state = {}
+
+    def restart():
+        state["cur"], state["nxt"] = bk._S[0], bk._S[1]
+        state["cur"].copy_(u0)
+        bk._counters.zero_()
 
     def step():
         c = bk.advance(state["cur"], state["nxt"], 1, "ssprk3")
@@ -298,7 +306,8 @@ def shocked_bench(refine: int, steps: int) -> dict:
         state["cur"] = c
 
     st = torch.cuda.current_stream()
-    t = _timed(step, steps, 1, st)
+    restart()
+    t, warm = _timed(step, steps, 1, st, min_warm_s=0.5, reset=restart)
     cnt = bk._counters.cpu().tolist()
     nb = wl.residual_bytes(ops)
     return {
@@ -307,7 +316,7 @@ def shocked_bench(refine: int, steps: int) -> dict:
         "gdof_stage_s": round(3 * steps * E * N / t / 1e9, 4),
         "ms_per_step": round(t / steps * 1e3, 4),
         "stage_hbm_frac": round(nb["step"] / (t / steps) / 1e9 / HBM_PEAK_GBS, 4),
-        "elements_scaled_per_stage": round(cnt[0] / (3 * (steps + 1)), 1),
+        "elements_scaled_per_stage": round(cnt[0] / (3 * steps), 1),
         "residual_norm_last": float(bk._res.item()),
     }
 
@@ -347,22 +356,25 @@ def adjoint_stage_bench(refine: int, steps: int) -> dict:
             "adjoint_gdof_stage_s": round(4 * E * N / t_a / 1e9, 3)}
 
 
-def adjoint_bench(refine: int) -> dict:
+def adjoint_bench(refine: int, fd_check: bool = False) -> dict:
     """Design gradients of thrust w.r.t. the eight interior control heights of
     the degree-9 Bezier wall (perturbed 5 %, seed 0): tri P2 on the 14 x 7
     base grid (refine 5: 200,704 elements), forward solve to 1e-11 (the
-    round-off plateau of this mesh is about 4e-12), steady
-    adjoint by the residual-VJP kernels to 1e-8 of the seed (the reference's
-    stop), one central difference (warm-started,
-    h = 1e-6 relative) as a check."""
+    round-off plateau of this mesh is about 4e-12), steady adjoint by the
+    residual-VJP kernels to 1e-8 of the seed (the reference's stop).
+    ``gradients_per_s`` counts the forward solve and the adjoint.  With
+    ``fd_check``, one component is also checked against a warm-started central
+    difference (h = 1e-6 relative)."""
     import torch
 
     sens, wl, cf, ini = _pkg("sensitivity"), _pkg("workloads"), _pkg("config"), _pkg("initial")
+    t_setup = time.perf_counter()
     geom = wl.bezier9_geometry(0)
     case = sens.DifferentiableCase(geometry=geom, order=2, refine=refine, kind="tri", nx=14,
                                    nr=7, options=cf.SolverOptions(tolerance=1e-11,
-                                                                  print_interval=5000))
+                                                                  print_interval=10000))
     U0 = ini.quasi1d_initial(case.operators(), case.flow, geom)
+    t_setup = time.perf_counter() - t_setup
     names = [f"bz{i}" for i in range(1, 9)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -373,24 +385,28 @@ def adjoint_bench(refine: int) -> dict:
     t = time.perf_counter() - t0
     print(f"adjoint: {fwd.operators.n_elem} elements, forward {fwd.iterations} it, adjoint "
           f"{info['adjoint_iterations']} it, {t:.1f} s", file=sys.stderr, flush=True)
-    k = "bz4"
-    base = case.default_params()
-    h = 1e-6 * max(1.0, abs(base[k]))
-    vals = []
-    for sgn in (1.0, -1.0):
-        with contextlib.redirect_stdout(sys.stderr):
-            f = case.solve_forward({k: base[k] + sgn * h}, U0=fwd.U)
-        vals.append(sens.objective_and_partials(f.U, f.operators, case.flow, "thrust")[0])
-    fd = (vals[0] - vals[1]) / (2 * h)
-    return {
+    rec = {
         "workload": f"bezier9-tri-p2-14x7-refine{refine}", "elements": fwd.operators.n_elem,
         "thrust": J, "forward_iterations": fwd.iterations,
+        "forward_seconds": round(fwd.history.wall_time, 3),
         "adjoint_iterations": info["adjoint_iterations"], "seconds": round(t, 3),
+        "host_setup_s": round(t_setup, 2),
         "gradients_per_s": round(len(names) / t, 4),
         "gradient": {n: grad[n] for n in names},
-        "fd_check": {"name": k, "adjoint": grad[k], "central_difference": fd,
-                     "rel_diff": abs(grad[k] - fd) / max(abs(fd), 1e-300)},
     }
+    if fd_check:
+        k = "bz4"
+        base = case.default_params()
+        h = 1e-6 * max(1.0, abs(base[k]))
+        vals = []
+        for sgn in (1.0, -1.0):
+            with contextlib.redirect_stdout(sys.stderr):
+                f = case.solve_forward({k: base[k] + sgn * h}, U0=fwd.U)
+            vals.append(sens.objective_and_partials(f.U, f.operators, case.flow, "thrust")[0])
+        fd = (vals[0] - vals[1]) / (2 * h)
+        rec["fd_check"] = {"name": k, "adjoint": grad[k], "central_difference": fd,
+                           "rel_diff": abs(grad[k] - fd) / max(abs(fd), 1e-300)}
+    return rec
 
 
 def _stops(res) -> dict:
@@ -515,10 +531,12 @@ def sweep_bench(n: int, tol: float, n_lone: int, max_it: int = 50000) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    # The first ~100 evaluations after start run about 20 % slower while the
-    # device ramps up; the defaults time the steady state.
     ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--min-warm-s", type=float, default=1.0,
+                    help="steady-state warm-up before each timed region (seconds, at least)")
+    ap.add_argument("--budget-s", type=float, default=420.0,
+                    help="legs after the headline start only while the elapsed time is below this")
     ap.add_argument("--ncx", type=int, default=1024)
     ap.add_argument("--ncy", type=int, default=512)
     ap.add_argument("--order", type=int, default=3)
@@ -526,20 +544,24 @@ def main() -> int:
     ap.add_argument("--limiter", default="positivity", choices=("positivity", "none"))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-march", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-config1", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="time budget of each of the four CPU-baseline variants")
     ap.add_argument("--shocked", action="store_true",
                     help="also march the shocked quadrilateral case (quads Q2 P2, pb 0.7, HLLC)")
     ap.add_argument("--shocked-refine", type=int, default=7)
-    ap.add_argument("--adjoint", action="store_true",
-                    help="also time the design-gradient case (degree-9 Bezier wall, tri P2)")
     ap.add_argument("--adjoint-refine", type=int, default=5)
+    ap.add_argument("--adjoint-fd", action="store_true",
+                    help="also check one adjoint component against a central difference")
     ap.add_argument("--adjoint-stages", type=int, default=0,
                     help="also time this many adjoint and forward steps on the Bezier-wall mesh")
-    ap.add_argument("--sweep", type=int, default=0,
-                    help="also march this many designs of the mixed sweep (seed 1) in one batch")
+    ap.add_argument("--sweep", type=int, default=256,
+                    help="designs of the mixed sweep (seed 1) marched in one batch")
     ap.add_argument("--sweep-tol", type=float, default=1e-10)
     ap.add_argument("--sweep-maxit", type=int, default=50000)
-    ap.add_argument("--sweep-lone", type=int, default=16,
+    ap.add_argument("--sweep-lone", type=int, default=8,
                     help="designs of the sweep also solved one at a time, for the comparison")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -551,19 +573,19 @@ def main() -> int:
 
     import torch
 
+    t_start = time.perf_counter()
     wl, cf, be = _pkg("workloads"), _pkg("config"), _pkg("backend")
-    t_setup = time.perf_counter()
     ops, U = wl.channel_case(args.ncx, args.ncy, args.order)
     E, N = ops.n_elem, ops.ref.n_basis
-    print(f"setup {time.perf_counter() - t_setup:.1f} s: {E} elements, {ops.n_edges_total} edges, "
-          f"N={N}", file=sys.stderr, flush=True)
+    print(f"setup {time.perf_counter() - t_start:.1f} s: {E} elements, {ops.n_edges_total} "
+          f"edges, N={N}", file=sys.stderr, flush=True)
     st = torch.cuda.current_stream()
     nb = wl.residual_bytes(ops)
 
     # ---- attainable bandwidth: a plain 1 GiB device copy, timed like the kernel
     ca = torch.full((2**27,), 1.0, dtype=torch.float64, device="cuda")
     cb = torch.empty_like(ca)
-    t_copy = _timed(lambda: cb.copy_(ca), 10, 2, st)
+    t_copy, _ = _timed(lambda: cb.copy_(ca), 10, 2, st, min_warm_s=0.3)
     copy_gbs = 2 * ca.numel() * 8 * 10 / t_copy / 1e9
     del ca, cb
 
@@ -581,15 +603,16 @@ def main() -> int:
             bk._chk(bk.lib.nz_rate(bk._pp, C.c_void_p(u0.data_ptr()), C.c_void_p(out.data_ptr()),
                                    C.c_void_p(wave.data_ptr()), bk._stream()), "nz_rate")
 
-        t = _timed(rate, args.steps, args.warmup, st)
-        per_flux[flux] = {"s_per_eval": t / args.steps, "bk": bk}
-        print(f"  residual {flux}: {t / args.steps * 1e6:.1f} us", file=sys.stderr, flush=True)
+        t, warm = _timed(rate, args.steps, args.warmup, st, min_warm_s=args.min_warm_s)
+        per_flux[flux] = {"s_per_eval": t / args.steps, "bk": bk, "warm": warm}
+        print(f"  residual {flux}: {t / args.steps * 1e6:.1f} us (warm-up {warm})",
+              file=sys.stderr, flush=True)
     t_eval = per_flux[args.flux]["s_per_eval"]
     gdof = E * N / t_eval / 1e9
     achieved = nb["residual"] / t_eval / 1e9
     workload = f"channel-tri-p{args.order}-{args.ncx}x{args.ncy}"
     out = {
-        "metric": "Gdof-stage/s FP64 + % HBM peak",
+        "metric": "Gdof-stage/s FP64 + % HBM peak; adjoint gradients/s and batched designs/s",
         "value": round(gdof, 4),
         "unit": "Gdof-stage/s",
         "n_gpus": 1,
@@ -611,48 +634,86 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": _traffic(workload + "-" + args.flux),
+            "traffic_source": "profiles/traffic.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE "
+                              "runs of this workload, scaled by a 1 GiB copy (not this run)",
             "hbm_copy_gbs": round(copy_gbs, 1),
         },
         "algorithmic_bytes_per_eval": nb["residual"],
-        "per_flux_gdof_stage_s": {f: round(E * N / v["s_per_eval"] / 1e9, 4) for f, v in per_flux.items()},
+        "warmup_steady_state": per_flux[args.flux]["warm"],
+        "per_flux_gdof_stage_s": {f: round(E * N / v["s_per_eval"] / 1e9, 4)
+                                  for f, v in per_flux.items()},
+        "per_flux_us_per_eval": {f: round(v["s_per_eval"] * 1e6, 1) for f, v in per_flux.items()},
     }
+    skipped = []
+
+    def leg(name: str, enabled: bool, fn):
+        if not enabled:
+            return
+        el = time.perf_counter() - t_start
+        if el > args.budget_s:
+            skipped.append(f"{name} (elapsed {el:.0f} s > budget {args.budget_s:.0f} s)")
+            return
+        t0 = time.perf_counter()
+        try:
+            out[name] = fn()
+        except Exception as exc:  # a failed optional leg is reported, not hidden
+            out[name] = {"error": f"{type(exc).__name__}: {exc}"}
+        print(f"leg {name}: {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
 
     # ---- the fused SSP-RK3 march on the same mesh and state
-    if not args.no_march:
+    def march_leg():
         bk = per_flux[args.flux]["bk"]
-        cur, nxt = bk._S[0], bk._S[1]
-        cur.copy_(u0)
-        bk._counters.zero_()
-        state = {"cur": cur, "nxt": nxt}
+        state = {"cur": bk._S[0], "nxt": bk._S[1]}
+
+        def restart():
+            # the timed steps start from the seeded state: a long warm-up
+            # march drives this synthetic (non-steady) state far from it
+            state["cur"], state["nxt"] = bk._S[0], bk._S[1]
+            state["cur"].copy_(u0)
+            bk._counters.zero_()
 
         def step():
             c = bk.advance(state["cur"], state["nxt"], 1, "ssprk3")
             state["nxt"] = state["cur"] if c is not state["cur"] else state["nxt"]
             state["cur"] = c
 
-        msteps, mwarm = max(2, args.steps // 5), max(1, args.warmup // 20)
-        t = _timed(step, msteps, mwarm, st)
+        msteps = max(2, args.steps // 5)
+        restart()
+        t, warm = _timed(step, msteps, 1, st, min_warm_s=args.min_warm_s / 2, reset=restart)
         cnt = bk._counters.cpu().tolist()
-        out["march"] = {
+        return {
             "gdof_stage_s": round(3 * msteps * E * N / t / 1e9, 4),
             "ms_per_step": round(t / msteps * 1e3, 4),
             "stage_kernel_hbm_frac": round(nb["step"] / (t / msteps) / 1e9 / HBM_PEAK_GBS, 4),
             "limiter": args.limiter,
-            "elements_scaled_per_stage": round(cnt[0] / (3 * (msteps + mwarm)), 1),
+            "elements_scaled_per_stage": round(cnt[0] / (3 * msteps), 1),
             "residual_norm_last": float(bk._res.item()),
+            "warmup_steady_state": warm,
         }
-    if args.shocked:
-        out["shocked"] = shocked_bench(args.shocked_refine, max(2, args.steps // 2))
-    if args.adjoint:
-        out["adjoint"] = adjoint_bench(args.adjoint_refine)
-    if args.adjoint_stages > 0:
-        out["adjoint_stages"] = adjoint_stage_bench(args.adjoint_refine, args.adjoint_stages)
-    if args.sweep > 0:
-        out["sweep"] = sweep_bench(args.sweep, args.sweep_tol, args.sweep_lone, args.sweep_maxit)
-    if not args.no_cpu:
-        sample_cells = (64, 32) if args.order >= 2 else (256, 128)
-        out["cpu_baseline"] = cpu_baseline(args.order, sample_cells[0], sample_cells[1],
-                                           args.cpu_seconds)
+
+    leg("march", not args.no_march, march_leg)
+    leg("cpu_baseline", not args.no_cpu, lambda: cpu_baseline(args.order, args.cpu_seconds))
+    leg("config1", not args.no_config1, config1_bench)
+    leg("sweep", not args.no_sweep and args.sweep > 0,
+        lambda: sweep_bench(args.sweep, args.sweep_tol, args.sweep_lone, args.sweep_maxit))
+    leg("adjoint", not args.no_adjoint,
+        lambda: adjoint_bench(args.adjoint_refine, fd_check=args.adjoint_fd))
+    leg("shocked", args.shocked, lambda: shocked_bench(args.shocked_refine, max(2, args.steps // 2)))
+    leg("adjoint_stages", args.adjoint_stages > 0,
+        lambda: adjoint_stage_bench(args.adjoint_refine, args.adjoint_stages))
+    summary = {"gdof_stage_s": out["value"], "hbm_frac": out["roofline"]["frac"]}
+    if isinstance(out.get("adjoint"), dict) and "gradients_per_s" in out["adjoint"]:
+        summary["adjoint_gradients_per_s"] = out["adjoint"]["gradients_per_s"]
+    if isinstance(out.get("sweep"), dict) and "designs_per_s" in out["sweep"]:
+        summary["sweep_designs_per_s"] = out["sweep"]["designs_per_s"]
+        summary["sweep_gradients_per_s"] = out["sweep"]["gradients_per_s"]
+        summary["sweep_gradient_speedup_vs_one_at_a_time"] = out["sweep"]["gradient_speedup"]
+    if isinstance(out.get("config1"), dict) and "hip_seconds" in out["config1"]:
+        summary["config1_time_to_solution_s"] = out["config1"]["hip_seconds"]
+    out["metric_parts"] = summary
+    if skipped:
+        out["legs_skipped"] = skipped
+    out["bench_seconds"] = round(time.perf_counter() - t_start, 1)
     print(json.dumps(out), flush=True)
     return 0
 
