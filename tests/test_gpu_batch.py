@@ -91,3 +91,84 @@ def test_batched_gradients_match_lone_adjoints(sweep):
         assert J == gr.objective
         assert gr.adjoint_iterations == info["adjoint_iterations"]
         assert gr.gradient["area_ratio"] == pytest.approx(grad["area_ratio"], rel=1e-10)
+
+
+# ------------------------------------------------------------ full sweep
+@pytest.fixture(scope="module")
+def sweep256():
+    """The benchmarked sweep: 256 designs from default_rng(1) (area ratio
+    U[1.5, 4], refine and order in {1, 2, 3}, triangles and quadrilaterals),
+    Roe, positivity limiter, tolerance 1e-10, at most 50,000 iterations."""
+    batch, cf = nz_module("batch"), nz_module("config")
+    designs = batch.design_sweep(256, 1)
+    flow = cf.FlowConditions()
+    opts = cf.SolverOptions(tolerance=1e-10, max_iterations=50000, print_interval=0)
+    run = batch.BatchedSteady(designs, flow, opts)
+    results = run.solve(use_graph=True)
+    print(f"256 designs, {run.n_elements} elements, batch {run.wall_s:.1f} s, "
+          f"{sum(r.converged for r in results)} converged", flush=True)
+    return run, results, flow, opts
+
+
+def _sample(results) -> list[int]:
+    """Eight designs covering the families the sweep holds, the hardest first:
+    tri p3, quad p2 and quad p3 at refine 3, then one of every other
+    (kind, order) and two more at refine 3."""
+    picks: list[int] = []
+
+    def first(pred):
+        for i, r in enumerate(results):
+            if i not in picks and pred(r.design):
+                picks.append(i)
+                return
+
+    for kind, order in (("tri", 3), ("quad", 2), ("quad", 3)):
+        first(lambda d, k=kind, o=order: d.kind == k and d.order == o and d.refine == 3)
+    for kind, order in (("tri", 1), ("tri", 2), ("quad", 1)):
+        first(lambda d, k=kind, o=order: d.kind == k and d.order == o)
+    first(lambda d: d.refine == 3)
+    first(lambda d: d.refine == 3)
+    return picks
+
+
+def test_sweep256_sampled_designs_bit_identical_to_lone(sweep256):
+    run, results, flow, opts = sweep256
+    march = nz_module("march")
+    picks = _sample(results)
+    assert len(picks) == 8
+    for g in picks:
+        res = results[g]
+        d = res.design
+        lone = march.solve_steady(run.ops_list[g], flow, opts, U0=run.U0_list[g])
+        print(f"design {g} ({d.kind} p{d.order} refine {d.refine}, AR {d.area_ratio:.3f}): "
+              f"batch {res.iterations} it conv={res.converged}, lone {lone.iterations} it",
+              flush=True)
+        assert (res.converged, res.iterations) == (lone.converged, lone.iterations)
+        assert res.residual == lone.residual or (np.isnan(res.residual) and np.isnan(lone.residual))
+        assert np.array_equal(res.U, lone.U, equal_nan=True)
+
+
+def test_sweep256_batched_gradients_equal_lone_adjoints(sweep256):
+    run, results, flow, opts = sweep256
+    batch, sens = nz_module("batch"), nz_module("sensitivity")
+    grads = batch.batched_gradients(results, flow, opts, tolerance=1e-8)
+    n_conv = sum(r.converged for r in results)
+    assert sum(1 for gr in grads if gr is not None and gr.converged) == n_conv
+    checked = 0
+    for g in _sample(results):
+        res, gr = results[g], grads[g]
+        if not res.converged:
+            assert gr is None
+            continue
+        d = res.design
+        case = sens.DifferentiableCase(geometry=d.geometry(), flow=flow, order=d.order,
+                                       refine=d.refine, kind=d.kind, options=opts)
+        J, grad, fwd, info = case.value_and_gradient("thrust", names=("area_ratio",),
+                                                     U0=run.U0_list[g], adjoint_tolerance=1e-8)
+        print(f"design {g}: batch dJ/dAR {gr.gradient['area_ratio']:.12e} "
+              f"({gr.adjoint_iterations} it), lone {grad['area_ratio']:.12e} "
+              f"({info['adjoint_iterations']} it)", flush=True)
+        assert J == gr.objective and gr.adjoint_iterations == info["adjoint_iterations"]
+        assert gr.gradient["area_ratio"] == grad["area_ratio"]
+        checked += 1
+    assert checked >= 4

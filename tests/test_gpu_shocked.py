@@ -2,11 +2,11 @@
 limiter activation flags on the device against the CPU oracle, bit for bit,
 on states of a real march.
 
-The case is the reference's shocked setup at a reduced refinement (refine 3,
-4,480 elements).  After a few SSP-RK3 steps from the quasi-1D initial state
-(which holds the normal shock), one stage is taken without the limiter; the
-same input then goes to the fused stage limiter, the standalone device
-limiter and the oracle's limiter.
+The case is the reference's shocked setup at refine 3 (4,480 elements) and
+at the benchmarked refine 7 (1,146,880 elements).  After 20 SSP-RK3 steps
+from the quasi-1D initial state (which holds the normal shock), one stage is
+taken without the limiter; the same input then goes to the fused stage
+limiter, the standalone device limiter and the oracle's limiter.
 """
 
 from __future__ import annotations
@@ -21,19 +21,20 @@ from conftest import nz_module, rel_err
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module")
-def shocked():
+@pytest.fixture(scope="module", params=[3, 7], ids=["refine3", "refine7"])
+def shocked(request):
     import torch
 
     mesh, op, cf = nz_module("mesh"), nz_module("operators"), nz_module("config")
     ini, be = nz_module("initial"), nz_module("backend")
     geom = mesh.NozzleGeometry(contour="bell", area_ratio=2.5)
-    topo, coords, _ = mesh.build_nozzle_mesh(geom, kind="quad", refine=3, geometry_order=2)
+    topo, coords, _ = mesh.build_nozzle_mesh(geom, kind="quad", refine=request.param,
+                                             geometry_order=2)
     ops = op.build_operators(topo, coords, 2, dense=False)
     flow = cf.FlowConditions(back_pressure_ratio=0.7, allow_shock_in_nozzle=True, flux="hllc")
-    U0 = ini.quasi1d_initial(ops, flow, geom)
     lim = be.HipBackend(ops, flow, cf.SolverOptions(limiter="positivity"))
     nolim = be.HipBackend(ops, flow, cf.SolverOptions(limiter="none"))
+    U0 = ini.quasi1d_initial(ops, flow, geom, backend=lim)
     U, _ = lim.run(U0, 20, "ssprk3")
     print(f"shocked quad Q2 P2: {ops.n_elem} elements, limited so far {lim.n_limited}", flush=True)
     return ops, flow, lim, nolim, torch.from_numpy(np.asarray(U)).to(lim.device)
