@@ -56,6 +56,8 @@ SIGNATURES = {
     "nz_scratch_doubles": (C.c_int64, [C.POINTER(Problem)]),
     "nz_residual": (C.c_int, [C.POINTER(Problem), P, P, P, P]),
     "nz_rate": (C.c_int, [C.POINTER(Problem), P, P, P, P]),
+    "nz_rate_noalloc": (C.c_int, [C.POINTER(Problem), P, P, P, P]),
+    "nz_rate_noalloc_max_boundary_points": (C.c_int64, []),
     "nz_inverse_mass": (C.c_int, [C.POINTER(Problem), P, P, P]),
     "nz_project": (C.c_int, [C.POINTER(Problem), P, P, P]),
     "nz_boundary_integrals": (C.c_int, [C.POINTER(Problem), P, P, P]),

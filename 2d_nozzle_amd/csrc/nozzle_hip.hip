@@ -38,6 +38,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 
 #include "nozzle_hip.h"
 #include "physics.hip.h"
@@ -2085,6 +2086,45 @@ int check_launch(const char* what) {
     return NZ_OK;
 }
 
+// ------------------------------------------------ static scratch (no allocation)
+// The boundary pre-pass output of nz_rate_noalloc lives in this file-scope
+// device array instead of a caller's buffer: up to kNoAllocDoubles doubles
+// (8 MiB), i.e. (boundary edges) x (face points) <= 209,715.  Each GPU holds
+// its own copy for the life of the process.  Host code never names the
+// array in a kernel argument or copy: its device address comes from
+// hipGetSymbolAddress, once per device, with that device current.
+namespace noalloc {
+constexpr int64_t kNoAllocDoubles = int64_t{1} << 20;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+constexpr int kMaxDevices = 64;
+std::mutex g_lock;
+double* g_ptr[kMaxDevices] = {};
+
+// device address of `scratch` on the current device, or nullptr with the
+// error recorded
+double* scratch_ptr() {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        fail(NZ_E_LAUNCH, "nz_rate_noalloc: no current HIP device");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> hold(g_lock);
+    if (!g_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t st = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (st != hipSuccess || !ptr) {
+            char msg[256];
+            std::snprintf(msg, sizeof msg, "nz_rate_noalloc: hipGetSymbolAddress: %s",
+                          hipGetErrorString(st));
+            fail(NZ_E_LAUNCH, msg);
+            return nullptr;
+        }
+        g_ptr[dev] = static_cast<double*>(ptr);
+    }
+    return g_ptr[dev];
+}
+}  // namespace noalloc
+
 // ------------------------------------------------------------- dispatch
 struct Ops {
     void (*dg)(const nz_problem&, const double*, const double*, double*, double*, double, double,
@@ -2300,6 +2340,34 @@ int nz_rate(const nz_problem* p, const double* U, double* F, double* wave, void*
     o.dg(*p, U, nullptr, F, wave, 0.0, 0.0, 0.0, nz::kRate, nullptr, nullptr, nullptr,
          n_blocks(p, o), static_cast<hipStream_t>(stream));
     return check_launch("nz_rate");
+}
+
+int64_t nz_rate_noalloc_max_boundary_points(void) { return noalloc::kNoAllocDoubles / 5; }
+
+int nz_rate_noalloc(const nz_problem* p, const double* U, double* F, double* wave, void* stream) {
+    if (!p) return fail(NZ_E_ARG, "null problem");
+    if (!U || !F || !wave) return fail(NZ_E_ARG, "null array");
+    const int64_t nbq = (p->n_edge - p->n_int) * static_cast<int64_t>(p->nqf);
+    if (p->n_edge < p->n_int || p->nqf <= 0) return fail(NZ_E_ARG, "n_int out of range");
+    if (nbq * 5 > noalloc::kNoAllocDoubles) {
+        char msg[256];
+        std::snprintf(msg, sizeof msg,
+                      "nz_rate_noalloc: %lld boundary face points need %lld doubles of scratch; "
+                      "the static scratch holds %lld (use nz_rate with a bflux workspace)",
+                      static_cast<long long>(nbq), static_cast<long long>(nbq * 5),
+                      static_cast<long long>(noalloc::kNoAllocDoubles));
+        return fail(NZ_E_ARG, msg);
+    }
+    double* ws = noalloc::scratch_ptr();
+    if (!ws) return NZ_E_LAUNCH;
+    nz_problem q = *p;
+    q.bflux = ws;  // every entry the element kernel reads is written first by k_bflux
+    Ops o;
+    int rc = resolve(&q, &o);
+    if (rc) return rc;
+    o.dg(q, U, nullptr, F, wave, 0.0, 0.0, 0.0, nz::kRate, nullptr, nullptr, nullptr,
+         n_blocks(&q, o), static_cast<hipStream_t>(stream));
+    return check_launch("nz_rate_noalloc");
 }
 
 int nz_inverse_mass(const nz_problem* p, const double* R, double* out, void* stream) {

@@ -127,6 +127,20 @@ int nz_residual(const nz_problem *p, const double *U, double *R, double *wave, v
  * Replaces edge_pass + element_rate (src/backends/_numba_kernels.py:586-664). */
 int nz_rate(const nz_problem *p, const double *U, double *F, double *wave, void *stream);
 
+/* nz_rate without a caller-provided workspace: no counterpart in the
+ * reference.  It is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The boundary pre-pass output, for which nz_rate uses
+ * p->bflux, goes to a static device array of 2^20 doubles (8 MiB) per GPU,
+ * held until the process ends; p->bflux is ignored.  Meshes with more than
+ * nz_rate_noalloc_max_boundary_points() boundary face points (boundary edges
+ * x nqf) are rejected with NZ_E_ARG before any launch.  Nothing here calls
+ * hipMalloc or hipFree.  All calls on one GPU share that array: calls that
+ * overlap on one GPU (different streams or threads) race; calls in stream
+ * order do not.  Results are those of nz_rate, bit for bit. */
+int nz_rate_noalloc(const nz_problem *p, const double *U, double *F, double *wave, void *stream);
+int64_t nz_rate_noalloc_max_boundary_points(void);
+
 /* out = M^-1 R per element (src/backends/_numba_kernels.py:762-777). */
 int nz_inverse_mass(const nz_problem *p, const double *R, double *out, void *stream);
 
