@@ -50,6 +50,53 @@ class Problem(C.Structure):
     ]
 
 
+class MeshDesc(C.Structure):
+    """Mirror of ``nz_mesh_desc``."""
+
+    _fields_ = [
+        ("n_elem", C.c_int64), ("n_nodes", C.c_int64),
+        ("nface", C.c_int32), ("nodes_per_elem", C.c_int32),
+        ("elem_nodes", C.c_void_p), ("face_ends", C.c_int32 * 8),
+        ("tag_coords", C.c_void_p), ("tag_length", C.c_double), ("tag_tol", C.c_double),
+    ]
+
+
+class EdgeMaps(C.Structure):
+    """Mirror of ``nz_edge_maps`` (device pointers)."""
+
+    _fields_ = [(n, C.c_void_p) for n in (
+        "iedge_elem", "iedge_face", "bedge_elem", "bedge_face", "bedge_tag", "face_edge",
+        "face_side", "face_neighbour", "edge_elem", "edge_face", "fe_nbr", "fe_info", "bedge",
+        "counts")]
+
+
+class GeometryOffsets(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in (
+        "dgeom_dxi_vol", "dgeom_deta_vol", "geom_vol", "w_vol", "phi_vol", "dgeom_dsigma_face",
+        "geom_face", "w_face")]
+
+
+class GeometryDesc(C.Structure):
+    """Mirror of ``nz_geometry_desc``."""
+
+    _fields_ = [
+        ("n_elem", C.c_int64), ("n_edge", C.c_int64),
+        ("nq", C.c_int32), ("nqv", C.c_int32), ("nqf", C.c_int32), ("nbf", C.c_int32),
+        ("nface", C.c_int32), ("affine", C.c_int32), ("straight", C.c_int32), ("pad0", C.c_int32),
+        ("elem_nodes", C.c_void_p), ("coords", C.c_void_p),
+        ("edge_elem", C.c_void_p), ("edge_face", C.c_void_p),
+        ("tables", C.c_void_p), ("off", GeometryOffsets),
+    ]
+
+
+class GeometryOut(C.Structure):
+    """Mirror of ``nz_geometry_out`` (device pointers, the reference-layout ones nullable)."""
+
+    _fields_ = [(n, C.c_void_p) for n in (
+        "geo", "area", "minv", "mean_w", "edge_geo", "det", "metrics", "wdet", "xy_vol",
+        "mean_w_full", "normal", "jac", "length", "xy_face", "bad_jacobian")]
+
+
 # name -> (restype, argtypes); every exported symbol of the header
 P = C.c_void_p
 SIGNATURES = {
@@ -76,6 +123,10 @@ SIGNATURES = {
     "nz_adjoint_defect": (C.c_int, [C.POINTER(Problem), P, P, P, P, P, P, P]),
     "nz_superbee": (C.c_int, [C.POINTER(Problem), P, C.c_double, C.c_double, P, P, P]),
     "nz_last_error": (C.c_char_p, []),
+    "nz_build_edges_workspace_bytes": (C.c_int64, [C.c_int64, C.c_int32]),
+    "nz_build_edges": (C.c_int, [C.POINTER(MeshDesc), C.POINTER(EdgeMaps), P, C.c_int64, P]),
+    "nz_build_geometry": (C.c_int, [C.POINTER(GeometryDesc), C.POINTER(GeometryOut), P]),
+    "nz_mesh_last_error": (C.c_char_p, []),
 }
 
 _LIB: C.CDLL | None = None

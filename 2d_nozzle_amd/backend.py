@@ -148,7 +148,7 @@ class HipBackend:
             raise ValueError(f"unknown limiter {opts.limiter!r}")
         self.device = torch.device("cuda", torch.cuda.current_device() if device is None else device)
         f64 = dict(dtype=torch.float64, device=self.device)
-        topo, ed = ops.topology, ops.topology.edges
+        topo = ops.topology
         kind, q_geo = topo.kind, topo.geometry_order
         E, N, nqv, nqf, nface = ops.n_elem, rd.n_basis, rd.n_qvol, rd.n_qface, topo.n_faces
         self.E, self.N = E, N
@@ -159,63 +159,16 @@ class HipBackend:
         if F >= 2**28 or E >= 2**31:
             raise ValueError("mesh too large for the 32-bit face maps")
 
-        # -- geometry
-        X = np.asarray(ops.node_coords, dtype=np.float64)
-        det, xix, xiy, etx, ety = element_metrics(rd, X[topo.elem_nodes])
-        if self.affine:
-            geo = np.stack([xix[:, 0], xiy[:, 0], etx[:, 0], ety[:, 0], det[:, 0]], axis=-1)
+        dev_arrays = getattr(ops, "device_arrays", None)
+        if dev_arrays is not None:
+            # operators built on the device (device_mesh.DeviceOperators):
+            # every array is already in the element kernel's layout
+            self._init_device_geometry(ops, dev_arrays, nqf)
         else:
-            geo = np.stack([xix, xiy, etx, ety, rd.w_vol[None, :] * det], axis=-1)
-        minv = mean_w = None
-        if not self.affine:
-            minv = getattr(ops, "inv_mass", None)
-            if minv is None:
-                wdet = rd.w_vol[None, :] * det
-                minv = np.linalg.inv(np.einsum("iq,jq,eq->eij", rd.phi_vol, rd.phi_vol, wdet))
-            # the device keeps the upper triangles, row-major (i <= j)
-            iu = np.triu_indices(rd.n_basis)
-            minv = np.asarray(minv)[:, iu[0], iu[1]]
-            mean_w = np.asarray(ops.mean_weights)
-        nrm = np.asarray(ops.edge_normal)
-        jac = np.asarray(ops.edge_jac)
-        if self.straight:
-            edge_geo = np.stack([nrm[:, 0, 0], nrm[:, 0, 1], jac[:, 0]], axis=-1)
-        else:
-            edge_geo = np.stack([nrm[..., 0], nrm[..., 1], jac], axis=-1)
-
-        # -- maps: neighbour (or -tag) and (edge << 3 | side << 2 | other face)
-        fe, fs = np.asarray(ed.face_edge), np.asarray(ed.face_side)
-        nbr = np.asarray(ed.face_neighbour).astype(np.int64).copy()
-        bnd = fe >= ni
-        nbr[bnd] = -np.asarray(ed.bedge_tag)[fe[bnd] - ni]
-        oface = np.zeros_like(fe)
-        inner = ~bnd
-        iface = np.asarray(ed.iedge_face)
-        oface[inner] = np.where(fs[inner] == 0, iface[fe[inner], 1], iface[fe[inner], 0])
-        info = (fe << 3) | (fs << 2) | oface
-        bedge = np.stack([np.asarray(ed.bedge_elem),
-                          (np.asarray(ed.bedge_face) << 3) | np.asarray(ed.bedge_tag)], axis=-1)
-
+            self._init_host_geometry(ops, rd, topo, topo.edges, E, N, nqv, nqf, nface, ni)
         tables, off = _tables(rd)
-
-        def dev(a, dtype=None):
-            if a is None:
-                return None
-            t = torch.from_numpy(np.ascontiguousarray(a))
-            return t.to(device=self.device, dtype=dtype or t.dtype)
-
-        self._keep = {
-            "geo": dev(geo, torch.float64),
-            "area": dev(np.asarray(ops.elem_area), torch.float64),
-            "minv": dev(minv, torch.float64),
-            "mean_w": dev(mean_w, torch.float64),
-            "edge_geo": dev(edge_geo, torch.float64),
-            "fe_nbr": dev(nbr.astype(np.int32)),
-            "fe_info": dev(info.astype(np.int32)),
-            "bedge": dev(bedge.astype(np.int32)),
-            "bflux": torch.empty(max(ops.n_boundary, 1) * nqf * 5, **f64),
-            "tables": dev(tables),
-        }
+        self._keep["tables"] = torch.from_numpy(np.ascontiguousarray(tables)).to(self.device)
+        self._keep["bflux"] = torch.empty(max(ops.n_boundary, 1) * nqf * 5, **f64)
 
         def ptr(k):
             t = self._keep[k]
@@ -241,7 +194,6 @@ class HipBackend:
         self.superbee = self.limiting and opts.limiter == "superbee"
         P.limiter = (2 if self.superbee else 1) if self.limiting else 0
         self.tvb = float(getattr(opts, "tvb_constant", 50.0))
-        self.area_total = float(np.sum(np.asarray(ops.elem_area)))
         P.rho_t_bf = float(flow.stagnation_density) if getattr(flow, "backflow", False) else 0.0
         P.p_t_bf, P.band_bf = flow.total_pressure, 0.05
         P.at2, P.at = at2, math.sqrt(at2)
@@ -293,6 +245,75 @@ class HipBackend:
             tab = locality_tiles(np.asarray(ops.node_coords), np.asarray(topo.elem_nodes), E)
             self._tiles = torch.from_numpy(tab.reshape(-1)).to(self.device)
             P.tiles, P.n_tiles = C.c_void_p(self._tiles.data_ptr()), tab.shape[0]
+
+    # ------------------------------------------------------------ set-up
+    def _init_host_geometry(self, ops, rd, topo, ed, E, N, nqv, nqf, nface, ni):
+        """The element kernel's compact geometry and maps from host operators."""
+        torch = self._torch
+        X = np.asarray(ops.node_coords, dtype=np.float64)
+        det, xix, xiy, etx, ety = element_metrics(rd, X[topo.elem_nodes])
+        if self.affine:
+            geo = np.stack([xix[:, 0], xiy[:, 0], etx[:, 0], ety[:, 0], det[:, 0]], axis=-1)
+        else:
+            geo = np.stack([xix, xiy, etx, ety, rd.w_vol[None, :] * det], axis=-1)
+        minv = mean_w = None
+        if not self.affine:
+            minv = getattr(ops, "inv_mass", None)
+            if minv is None:
+                wdet = rd.w_vol[None, :] * det
+                minv = np.linalg.inv(np.einsum("iq,jq,eq->eij", rd.phi_vol, rd.phi_vol, wdet))
+            # the device keeps the upper triangles, row-major (i <= j)
+            iu = np.triu_indices(rd.n_basis)
+            minv = np.asarray(minv)[:, iu[0], iu[1]]
+            mean_w = np.asarray(ops.mean_weights)
+        nrm = np.asarray(ops.edge_normal)
+        jac = np.asarray(ops.edge_jac)
+        if self.straight:
+            edge_geo = np.stack([nrm[:, 0, 0], nrm[:, 0, 1], jac[:, 0]], axis=-1)
+        else:
+            edge_geo = np.stack([nrm[..., 0], nrm[..., 1], jac], axis=-1)
+
+        # -- maps: neighbour (or -tag) and (edge << 3 | side << 2 | other face)
+        fe, fs = np.asarray(ed.face_edge), np.asarray(ed.face_side)
+        nbr = np.asarray(ed.face_neighbour).astype(np.int64).copy()
+        bnd = fe >= ni
+        nbr[bnd] = -np.asarray(ed.bedge_tag)[fe[bnd] - ni]
+        oface = np.zeros_like(fe)
+        inner = ~bnd
+        iface = np.asarray(ed.iedge_face)
+        oface[inner] = np.where(fs[inner] == 0, iface[fe[inner], 1], iface[fe[inner], 0])
+        info = (fe << 3) | (fs << 2) | oface
+        bedge = np.stack([np.asarray(ed.bedge_elem),
+                          (np.asarray(ed.bedge_face) << 3) | np.asarray(ed.bedge_tag)], axis=-1)
+
+        def dev(a, dtype=None):
+            if a is None:
+                return None
+            t = torch.from_numpy(np.ascontiguousarray(a))
+            return t.to(device=self.device, dtype=dtype or t.dtype)
+
+        self._keep = {
+            "geo": dev(geo, torch.float64),
+            "area": dev(np.asarray(ops.elem_area), torch.float64),
+            "minv": dev(minv, torch.float64),
+            "mean_w": dev(mean_w, torch.float64),
+            "edge_geo": dev(edge_geo, torch.float64),
+            "fe_nbr": dev(nbr.astype(np.int32)),
+            "fe_info": dev(info.astype(np.int32)),
+            "bedge": dev(bedge.astype(np.int32)),
+        }
+        self.area_total = float(np.sum(np.asarray(ops.elem_area)))
+
+    def _init_device_geometry(self, ops, arrays, nqf):
+        """Geometry and maps already on the device (``DeviceOperators``)."""
+        keep = {}
+        for k in ("geo", "area", "minv", "mean_w", "edge_geo", "fe_nbr", "fe_info", "bedge"):
+            t = arrays.get(k)
+            if t is not None and t.device != self.device:
+                raise ValueError(f"device operator array {k} is on {t.device}, not {self.device}")
+            keep[k] = t
+        self._keep = keep
+        self.area_total = float(ops.elem_area_total)
 
     # ------------------------------------------------------------ helpers
     def _stream(self):

@@ -17,7 +17,7 @@ import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SRC = os.path.join(HERE, "csrc", "nozzle_hip.hip")
+SOURCES = [os.path.join(HERE, "csrc", "nozzle_hip.hip"), os.path.join(HERE, "csrc", "mesh_build.hip")]
 OUT = os.path.join(HERE, "_lib", "libnozzle_hip.so")
 
 HIPCC_FLAGS = [
@@ -25,7 +25,6 @@ HIPCC_FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     # No packed-FP32 instructions in device code.  The device pass honours
     # this feature; the host (x86) pass does not know it and prints "not a
     # recognized feature ... (ignoring feature)", which is harmless.
@@ -45,16 +44,36 @@ def hipcc() -> str:
 
 
 def build(verbose: bool = False) -> str:
+    """Compile every source to an object (in parallel), link the shared
+    library, and check its device code for packed-FP32 instructions."""
     os.makedirs(os.path.dirname(OUT), exist_ok=True)
-    tmp = OUT + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), "-I",
-           os.path.join(HERE, "csrc"), "-o", tmp, SRC]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError(f"hipcc failed with status {res.returncode}")
+    inc = ["-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "csrc")]
+    with tempfile.TemporaryDirectory() as td:
+        objs, procs = [], []
+        for src in SOURCES:
+            obj = os.path.join(td, os.path.basename(src) + ".o")
+            cmd = [hipcc(), *HIPCC_FLAGS, *inc, "-c", "-o", obj, src]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE,
+                                                stderr=subprocess.STDOUT, text=True)))
+            objs.append(obj)
+        failed = []
+        for cmd, pr in procs:
+            out, _ = pr.communicate()
+            if pr.returncode != 0:
+                sys.stderr.write(out)
+                failed.append(cmd[-1])
+        if failed:
+            raise RuntimeError(f"hipcc failed on {failed}")
+        tmp = OUT + ".tmp"
+        cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError(f"link failed with status {res.returncode}")
     check_no_packed_fp32(tmp)
     os.replace(tmp, OUT)
     return OUT

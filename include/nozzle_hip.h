@@ -237,6 +237,99 @@ int nz_adjoint_defect(const nz_problem *p, const double *U, const double *lam, c
 int nz_time_step(const nz_problem *p, const double *wave, double factor, double *dt,
                  void *stream);
 
+/* ---- mesh connectivity and geometry on the device -------------------------
+ * (src/geometry/mesh.py:77-208 build_edges; src/spatial/operators.py:209-342
+ * build_operators).  Sizes that are only known after the build (interior and
+ * boundary edge counts) come back in `counts`; outputs are sized for the
+ * worst case by the caller.  Errors: nz_mesh_last_error(). */
+#define NZ_TAG_INFLOW 1
+#define NZ_TAG_OUTFLOW 2
+#define NZ_TAG_WALL 3
+#define NZ_TAG_AXIS 4
+
+typedef struct {
+    int64_t n_elem;
+    int64_t n_nodes;
+    int32_t nface;            /* 3 triangles, 4 quadrilaterals */
+    int32_t nodes_per_elem;   /* geometry nodes per element */
+    const int64_t *elem_nodes;  /* (E, nodes_per_elem) */
+    int32_t face_ends[8];     /* corners of face f: local nodes face_ends[2f], face_ends[2f+1] */
+    const double *tag_coords; /* (n_nodes, 2): the coordinates the boundary tagger classifies */
+    double tag_length, tag_tol;  /* nozzle_boundary_tagger (src/geometry/mesh.py:340-358) */
+} nz_mesh_desc;
+
+typedef struct {
+    /* the reference's EdgeConnectivity (src/geometry/mesh.py:54-74), int64;
+     * interior arrays sized (E*nface/2, 2), boundary arrays (E*nface,) */
+    int64_t *iedge_elem, *iedge_face;
+    int64_t *bedge_elem, *bedge_face, *bedge_tag;
+    int64_t *face_edge, *face_side, *face_neighbour;  /* (E, nface) */
+    /* owner element and local face per edge, interior then boundary (E*nface,);
+     * nullable together */
+    int64_t *edge_elem, *edge_face;
+    /* the element kernel's maps (nz_problem fe_nbr, fe_info, bedge); nullable together */
+    int32_t *fe_nbr, *fe_info, *bedge;
+    /* (8,): n_interior, n_boundary, edges with tag 1, 2, 3, 4, faces shared
+     * by more than two elements (non-zero: the mesh is not conforming) */
+    int64_t *counts;
+} nz_edge_maps;
+
+int64_t nz_build_edges_workspace_bytes(int64_t n_elem, int32_t nface);
+
+/* Edge lists and face maps of a conforming mesh, bit-identical to
+ * build_edges (src/geometry/mesh.py:77-208): faces matched on their corner
+ * pair by a stable sort of the keys, interior edges in key order owned by
+ * the lower element, boundary faces in key order grouped by tag. */
+int nz_build_edges(const nz_mesh_desc *m, const nz_edge_maps *out, void *work,
+                   int64_t work_bytes, void *stream);
+
+/* Offsets (doubles) of the reference geometry tables inside
+ * nz_geometry_desc.tables (src/spatial/operators.py:46-135). */
+typedef struct {
+    int32_t dgeom_dxi_vol;      /* (NQ, nqv) */
+    int32_t dgeom_deta_vol;     /* (NQ, nqv) */
+    int32_t geom_vol;           /* (NQ, nqv) */
+    int32_t w_vol;              /* (nqv,) */
+    int32_t phi_vol;            /* (N, nqv) */
+    int32_t dgeom_dsigma_face;  /* (nface, NQ, nqf) */
+    int32_t geom_face;          /* (nface, NQ, nqf) */
+    int32_t w_face;             /* (nqf,) */
+} nz_geometry_offsets;
+
+typedef struct {
+    int64_t n_elem, n_edge;
+    int32_t nq, nqv, nqf, nbf, nface;
+    int32_t affine, straight;   /* as in nz_problem */
+    int32_t pad0;
+    const int64_t *elem_nodes;  /* (E, nq) */
+    const double *coords;       /* (n_nodes, 2) */
+    const int64_t *edge_elem, *edge_face;  /* owner element and face per edge (nz_edge_maps) */
+    const double *tables;
+    nz_geometry_offsets off;
+} nz_geometry_desc;
+
+typedef struct {
+    /* the element kernel's layout (nz_problem): geo, area, minv and mean_w
+     * (non-affine only), edge_geo */
+    double *geo, *area, *minv, *mean_w, *edge_geo;
+    /* the reference's Operators layout, each nullable:
+     * det (E,nqv), metrics (E,nqv,4) = xi_x, xi_y, eta_x, eta_y, wdet (E,nqv),
+     * xy_vol (E,nqv,2), mean_w_full (E,N) (also for affine elements),
+     * normal (F,nqf,2), jac (F,nqf), length (F,), xy_face (F,nqf,2) */
+    double *det, *metrics, *wdet, *xy_vol, *mean_w_full;
+    double *normal, *jac, *length, *xy_face;
+    int64_t *bad_jacobian;      /* (1,): volume points with det <= 0 */
+} nz_geometry_out;
+
+/* Per-element and per-edge geometry from node coordinates alone: the
+ * build_operators quantities (src/spatial/operators.py:245-318) in the
+ * element kernel's layout, the inverse mass matrices of non-affine elements
+ * by Cholesky.  Reused topology: call again with new coordinates. */
+int nz_build_geometry(const nz_geometry_desc *g, const nz_geometry_out *out, void *stream);
+
+/* Last error of nz_build_edges / nz_build_geometry on this thread. */
+const char *nz_mesh_last_error(void);
+
 /* Last error message of this thread (static storage; never NULL). */
 const char *nz_last_error(void);
 

@@ -33,15 +33,20 @@ def test_library_exports_every_declared_symbol():
         assert n in lib.SIGNATURES, n
 
 
-def test_problem_struct_layout(tmp_path):
+@pytest.mark.parametrize("cname,pyname", [("nz_problem", "Problem"), ("nz_mesh_desc", "MeshDesc"),
+                                           ("nz_edge_maps", "EdgeMaps"),
+                                           ("nz_geometry_desc", "GeometryDesc"),
+                                           ("nz_geometry_out", "GeometryOut")])
+def test_struct_layout(tmp_path, cname, pyname):
     """Compile a probe against the header and compare offsets with ctypes."""
     lib = nz_module("_lib")
-    fields = [f for f, _ in lib.Problem._fields_]
+    mirror = getattr(lib, pyname)
+    fields = [f for f, _ in mirror._fields_]
     src = tmp_path / "probe.c"
-    body = "\n".join(f'printf("%s %zu\\n", "{f}", offsetof(nz_problem, {f}));' for f in fields)
+    body = "\n".join(f'printf("%s %zu\\n", "{f}", offsetof({cname}, {f}));' for f in fields)
     src.write_text(
         "#include <stdio.h>\n#include <stddef.h>\n#include \"nozzle_hip.h\"\n"
-        "int main(void){" + body + 'printf("size %zu\\n", sizeof(nz_problem)); return 0;}\n'
+        "int main(void){" + body + f'printf("size %zu\\n", sizeof({cname})); return 0;}}\n'
     )
     exe = tmp_path / "probe"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
@@ -49,8 +54,8 @@ def test_problem_struct_layout(tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
     got = dict(line.split() for line in out if line)
     for f in fields:
-        assert int(got[f]) == getattr(lib.Problem, f).offset, f
-    assert int(got["size"]) == C.sizeof(lib.Problem)
+        assert int(got[f]) == getattr(mirror, f).offset, f
+    assert int(got["size"]) == C.sizeof(mirror)
 
 
 def test_unsupported_configuration_is_an_error():
