@@ -97,6 +97,22 @@ class GeometryOut(C.Structure):
         "mean_w_full", "normal", "jac", "length", "xy_face", "bad_jacobian")]
 
 
+class Quasi1DDesc(C.Structure):
+    """Mirror of ``nz_quasi1d_desc``."""
+
+    _fields_ = [
+        ("contour", C.c_int32), ("n_points", C.c_int32),
+        ("x", C.c_void_p), ("density", C.c_void_p), ("velocity", C.c_void_p),
+        ("pressure", C.c_void_p),
+        ("length", C.c_double), ("throat_x", C.c_double), ("y_throat", C.c_double),
+        ("y_inlet", C.c_double), ("y_exit", C.c_double),
+        ("slope_throat", C.c_double), ("slope_exit", C.c_double),
+        ("bezier_c1", C.c_double), ("bezier_c2", C.c_double),
+        ("ctrl9", C.c_double * 10), ("binom9", C.c_double * 10),
+        ("pi", C.c_double), ("e2m1", C.c_double), ("gamma", C.c_double), ("h", C.c_double),
+    ]
+
+
 # name -> (restype, argtypes); every exported symbol of the header
 P = C.c_void_p
 SIGNATURES = {
@@ -127,6 +143,8 @@ SIGNATURES = {
     "nz_build_edges": (C.c_int, [C.POINTER(MeshDesc), C.POINTER(EdgeMaps), P, C.c_int64, P]),
     "nz_build_geometry": (C.c_int, [C.POINTER(GeometryDesc), C.POINTER(GeometryOut), P]),
     "nz_mesh_last_error": (C.c_char_p, []),
+    "nz_quasi1d_values": (C.c_int, [C.POINTER(Quasi1DDesc), P, C.c_int64, P, P]),
+    "nz_initial_last_error": (C.c_char_p, []),
 }
 
 _LIB: C.CDLL | None = None

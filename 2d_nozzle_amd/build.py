@@ -17,7 +17,7 @@ import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SOURCES = [os.path.join(HERE, "csrc", "nozzle_hip.hip"), os.path.join(HERE, "csrc", "mesh_build.hip")]
+SOURCES = [os.path.join(HERE, "csrc", n) for n in ("nozzle_hip.hip", "mesh_build.hip", "initial.hip")]
 OUT = os.path.join(HERE, "_lib", "libnozzle_hip.so")
 
 HIPCC_FLAGS = [

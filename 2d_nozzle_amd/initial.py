@@ -285,12 +285,82 @@ def uniform_initial(ops, flow: FlowConditions, mach: float = 0.95, *, backend=No
     return project(values, ops)
 
 
-def quasi1d_initial(ops, flow: FlowConditions, geom, *, solution=None, backend=None) -> np.ndarray:
+_CONTOUR_IDS = {"bell": 0, "smooth": 0, "moc": 0, "conical": 1, "bezier": 2, "bezier9": 3,
+                "analytic": 4}
+
+
+def quasi1d_values_device(ops, flow: FlowConditions, geom, backend, *, solution=None):
+    """Conserved point values ``(E, nqv, 4)`` of the quasi-1D initial state at
+    the volume points, formed on the device (``nz_quasi1d_values``); returns a
+    device tensor.  The quasi-1D solution itself (601 stations) is solved on
+    the host, as it is once per design and independent of the mesh."""
+    import ctypes as C
+    import math
+
+    import torch
+
+    from . import _lib
+
+    sol = solution if solution is not None else solve_quasi1d(geom, flow)
+    dev = backend.device
+    f64 = dict(dtype=torch.float64, device=dev)
+    fields = getattr(ops, "device_fields", None) or {}
+    if fields.get("xy_vol") is not None:
+        xy = fields["xy_vol"]
+    else:
+        xy = torch.from_numpy(np.ascontiguousarray(np.asarray(ops.xy_vol), dtype=np.float64)).to(dev)
+    st = [torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+          for a in (sol.x, sol.density, sol.velocity, sol.pressure)]
+    prm = geom.params()
+    d = _lib.Quasi1DDesc()
+    d.contour, d.n_points = _CONTOUR_IDS[geom.contour], len(sol.x)
+    d.x, d.density, d.velocity, d.pressure = (C.c_void_p(t.data_ptr()) for t in st)
+    d.length = prm.get("length", 1.0)
+    if geom.contour != "analytic":
+        y_th = prm["throat_half_height"]
+        y_ex = y_th * prm["area_ratio"]
+        d.throat_x, d.y_throat, d.y_inlet, d.y_exit = prm["throat_x"], y_th, prm["inlet_half_height"], y_ex
+        dx_ds = (1.0 - prm["throat_x"]) * d.length
+        d.slope_throat = math.tan(prm["theta_initial"]) * dx_ds
+        d.slope_exit = math.tan(prm["theta_exit"]) * dx_ds
+        d.bezier_c1 = y_th + prm["bezier_w1"] * (y_ex - y_th)
+        d.bezier_c2 = y_th + prm["bezier_w2"] * (y_ex - y_th)
+        if geom.contour == "bezier9":
+            ctrl = [y_th] + [prm[f"bz{i}"] for i in range(1, 9)] + [y_ex]
+            for k in range(10):
+                d.ctrl9[k] = ctrl[k]
+                d.binom9[k] = float(math.comb(9, k))
+    d.pi, d.e2m1 = math.pi, math.exp(2.0) - 1.0
+    d.gamma, d.h = flow.gamma, 1e-6 * geom.length
+    E, nqv = ops.n_elem, ops.ref.n_qvol
+    out = torch.empty((E, nqv, 4), **f64)
+    lib = _lib.load()
+    rc = lib.nz_quasi1d_values(C.byref(d), C.c_void_p(xy.data_ptr()), E * nqv,
+                               C.c_void_p(out.data_ptr()), backend._stream())
+    if rc != 0:
+        raise _lib.HipError(f"nz_quasi1d_values failed ({rc}): "
+                            f"{lib.nz_initial_last_error().decode(errors='replace')}")
+    return out
+
+
+def quasi1d_initial(ops, flow: FlowConditions, geom, *, solution=None, backend=None,
+                    as_tensor: bool = False):
     """Projected quasi-1D state with a wall-slope transverse velocity (``initialize.py:68-102``).
 
-    With ``backend`` (a ``HipBackend`` on these operators) the projection runs
-    on the device (``nz_project``); the point values are formed here either way.
+    With ``backend`` (a ``HipBackend`` on these operators) the point values
+    are formed on the device (``nz_quasi1d_values``) and projected there
+    (``nz_project``); ``as_tensor`` then returns the device tensor instead of
+    a NumPy copy.  Without a backend both steps run here.
     """
+    if backend is not None:
+        u = backend.project(quasi1d_values_device(ops, flow, geom, backend, solution=solution))
+        return u if as_tensor else u.cpu().numpy()
+    return project(quasi1d_values(ops, flow, geom, solution=solution), ops)
+
+
+def quasi1d_values(ops, flow: FlowConditions, geom, *, solution=None) -> np.ndarray:
+    """Conserved point values ``(E, nqv, 4)`` of the quasi-1D initial state at
+    the volume points, on the host (``initialize.py:80-100``)."""
     sol = solution if solution is not None else solve_quasi1d(geom, flow)
     xy = np.asarray(ops.xy_vol)
     x, y = xy[..., 0], xy[..., 1]
@@ -304,10 +374,7 @@ def quasi1d_initial(ops, flow: FlowConditions, geom, *, solution=None, backend=N
         h + np.minimum(x, h))
     vy = vx * np.clip(y / np.maximum(y_wall, 1e-30), 0.0, 1.0) * slope
     rhoE = p / (flow.gamma - 1.0) + 0.5 * rho * (vx * vx + vy * vy)
-    values = np.stack([rho, rho * vx, rho * vy, rhoE], axis=-1)
-    if backend is not None:
-        return backend.project(values)
-    return project(values, ops)
+    return np.stack([rho, rho * vx, rho * vy, rhoE], axis=-1)
 
 
 def initial_state(ops, flow: FlowConditions, geom, kind: str = "quasi1d") -> np.ndarray:

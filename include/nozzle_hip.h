@@ -330,6 +330,37 @@ int nz_build_geometry(const nz_geometry_desc *g, const nz_geometry_out *out, voi
 /* Last error of nz_build_edges / nz_build_geometry on this thread. */
 const char *nz_mesh_last_error(void);
 
+/* ---- quasi-1D initial state on the device -------------------------------- */
+#define NZ_CONTOUR_HERMITE 0   /* bell, smooth, moc */
+#define NZ_CONTOUR_CONICAL 1
+#define NZ_CONTOUR_BEZIER 2    /* cubic with two weights */
+#define NZ_CONTOUR_BEZIER9 3   /* degree-9 Bernstein diverging wall */
+#define NZ_CONTOUR_ANALYTIC 4
+
+typedef struct {
+    int32_t contour;            /* NZ_CONTOUR_* */
+    int32_t n_points;           /* quasi-1D stations */
+    const double *x, *density, *velocity, *pressure;  /* (n_points,), x increasing */
+    /* wall (src/geometry/contour.py:264-311), parameters as the host computes them */
+    double length, throat_x, y_throat, y_inlet, y_exit;
+    double slope_throat, slope_exit;  /* Hermite end slopes: tan(theta) (1 - throat_x) length */
+    double bezier_c1, bezier_c2;      /* cubic Bezier interior control heights */
+    double ctrl9[10];                 /* degree-9 control heights, throat to exit */
+    double binom9[10];                /* C(9, k) */
+    double pi, e2m1;                  /* pi and exp(2) - 1 for the analytic wall */
+    double gamma;
+    double h;                         /* central-difference step of the wall slope, 1e-6 L */
+} nz_quasi1d_desc;
+
+/* Conserved point values (n, 4) of the quasi-1D initial state at the points
+ * xy (n, 2): the step before nz_project in quasi1d_initial
+ * (src/timestepping/initialize.py:68-102).  Hermite and conical walls are
+ * evaluated in the host's operation order without FP contraction
+ * (bit-identical values); Bezier walls round integer powers correctly. */
+int nz_quasi1d_values(const nz_quasi1d_desc *d, const double *xy, int64_t n, double *values,
+                      void *stream);
+const char *nz_initial_last_error(void);
+
 /* Last error message of this thread (static storage; never NULL). */
 const char *nz_last_error(void);
 

@@ -36,7 +36,8 @@ def test_library_exports_every_declared_symbol():
 @pytest.mark.parametrize("cname,pyname", [("nz_problem", "Problem"), ("nz_mesh_desc", "MeshDesc"),
                                            ("nz_edge_maps", "EdgeMaps"),
                                            ("nz_geometry_desc", "GeometryDesc"),
-                                           ("nz_geometry_out", "GeometryOut")])
+                                           ("nz_geometry_out", "GeometryOut"),
+                                           ("nz_quasi1d_desc", "Quasi1DDesc")])
 def test_struct_layout(tmp_path, cname, pyname):
     """Compile a probe against the header and compare offsets with ctypes."""
     lib = nz_module("_lib")
