@@ -39,7 +39,8 @@ from types import SimpleNamespace
 import numpy as np
 
 from .config import FlowConditions, SolverOptions, residual_scale
-from .initial import quasi1d_initial
+from .device_mesh import DeviceFamily
+from .initial import quasi1d_initial, quasi1d_values_device
 from .march import StopRules
 from .mesh import NozzleGeometry, build_nozzle_mesh
 from .operators import build_operators
@@ -161,14 +162,15 @@ class DesignResult:
 class _Family:
     """One (kind, order, geometry order) family marched by one backend."""
 
-    def __init__(self, idx, ops_list, U0_list, flow, opts, device):
+    def __init__(self, idx, ops_list, U0_list, flow, opts, device, geoms=None):
         import torch
 
         from .backend import HipBackend
 
         self.torch = torch
         self.idx = list(idx)
-        self.cat = concat_operators(ops_list)
+        on_device = getattr(ops_list[0], "device_arrays", None) is not None
+        self.cat = DeviceFamily(ops_list) if on_device else concat_operators(ops_list)
         self.bk = HipBackend(self.cat, flow, opts, device=device)
         if self.bk.superbee:
             raise NotImplementedError("the batched march supports limiter 'positivity' or 'none'")
@@ -177,7 +179,15 @@ class _Family:
         self.stream = torch.cuda.Stream(device=self.bk.device)
         f64 = dict(dtype=torch.float64, device=self.bk.device)
         self.cur, self.nxt = self.bk._S[0], self.bk._S[1]
-        self.cur.copy_(torch.from_numpy(np.concatenate(U0_list)).to(self.bk.device))
+        if U0_list is None:
+            # quasi-1D point values of every design on the device, projected
+            # for the whole family in one launch
+            vals = torch.cat([quasi1d_values_device(o, flow, g, self.bk)
+                              for o, g in zip(ops_list, geoms)])
+            self.cur.copy_(self.bk.project(vals))
+        else:
+            self.cur.copy_(torch.from_numpy(np.concatenate(U0_list)).to(self.bk.device))
+        self.U0 = self.cur.clone()
         self.res_dev = torch.empty(len(self.idx), **f64)
         self.active = list(range(len(self.idx)))  # local design indices
         self.graph = None
@@ -265,12 +275,33 @@ class BatchedSteady:
 
     def __init__(self, designs, flow: FlowConditions | None = None,
                  opts: SolverOptions | None = None, *, device=None, ops_list=None,
-                 U0_list=None):
+                 U0_list=None, device_setup: bool = False):
+        """``device_setup``: build each grid's connectivity once on the device
+        and share it among the designs on that grid (the reference rebuilds
+        every design's mesh, ``src/api.py:213-244``), each design's operators
+        from its node coordinates on the device, and the quasi-1D initial
+        states on the device; otherwise meshes, operators and initial states
+        are built on the host per design."""
         self.designs = list(designs)
         self.flow = flow or FlowConditions()
         self.opts = opts or SolverOptions()
         t0 = time.perf_counter()
-        if ops_list is None:
+        geoms = None
+        if ops_list is None and device_setup:
+            from .device_mesh import build_nozzle_operators
+
+            ops_list, geoms, topos = [], [], {}
+            for dsg in self.designs:
+                geom = dsg.geometry()
+                key = (dsg.kind, dsg.nx, dsg.nr, dsg.refine, dsg.geometry_order)
+                ops, topos[key], _ = build_nozzle_operators(
+                    geom, kind=dsg.kind, nx=dsg.nx, nr=dsg.nr, refine=dsg.refine,
+                    geometry_order=dsg.geometry_order, order=dsg.order, topology=topos.get(key),
+                    device=device)
+                ops_list.append(ops)
+                geoms.append(geom)
+            self.n_topologies = len(topos)
+        elif ops_list is None:
             ops_list, U0_list = [], []
             for dsg in self.designs:
                 geom = dsg.geometry()
@@ -280,15 +311,25 @@ class BatchedSteady:
                 ops = build_operators(topo, coords, dsg.order, dense=False)
                 ops_list.append(ops)
                 U0_list.append(quasi1d_initial(ops, self.flow, geom))
-        self.ops_list, self.U0_list = ops_list, U0_list
+        self.ops_list = ops_list
         self.host_setup_s = time.perf_counter() - t0
         fams: dict = {}
         for i, o in enumerate(ops_list):
             key = (o.topology.kind, o.topology.geometry_order, o.ref.order)
             fams.setdefault(key, []).append(i)
         t0 = time.perf_counter()
-        self.families = [_Family(idx, [ops_list[i] for i in idx], [U0_list[i] for i in idx],
-                                 self.flow, self.opts, device) for idx in fams.values()]
+        self.families = [
+            _Family(idx, [ops_list[i] for i in idx],
+                    None if U0_list is None else [U0_list[i] for i in idx], self.flow, self.opts,
+                    device, geoms=None if geoms is None else [geoms[i] for i in idx])
+            for idx in fams.values()]
+        if U0_list is None:  # formed on the device: host copies for the results and callers
+            U0_list = [None] * len(ops_list)
+            for f in self.families:
+                u0 = f.U0.cpu().numpy()
+                for j, g in enumerate(f.idx):
+                    U0_list[g] = u0[int(f.eoff[j]):int(f.eoff[j + 1])]
+        self.U0_list = U0_list
         self.device_setup_s = time.perf_counter() - t0
 
     @property
@@ -371,15 +412,19 @@ class DesignGradient:
 
 
 def _design_coords(dsg: Design, area_ratio: float) -> np.ndarray:
-    """Node coordinates of a design with the logical grid frozen at the design."""
-    from .mesh import nozzle_node_coords
+    """Node coordinates of a design with the logical grid frozen at the design
+    (the node distributions of ``build_nozzle_mesh``, no connectivity)."""
+    from .mesh import nozzle_node_coords, nozzle_x_distribution
 
     geom = dsg.geometry()
-    _, _, grids = build_nozzle_mesh(geom, kind=dsg.kind, nx=dsg.nx, nr=dsg.nr, refine=dsg.refine,
-                                    geometry_order=dsg.geometry_order)
+    q, sub = dsg.geometry_order, dsg.geometry_order * 2**dsg.refine
+    x_frac = nozzle_x_distribution(q * dsg.nx * 2**dsg.refine + 1, nx_base=dsg.nx, n_sub=sub,
+                                   spacing="throat", x_throat=geom.throat_location() / geom.length,
+                                   cluster_strength=3.0, cluster_width=0.12)
+    r_frac = np.linspace(0.0, 1.0, q * dsg.nr * 2**dsg.refine + 1)
     p = dict(geom.params())
     p["area_ratio"] = area_ratio
-    return nozzle_node_coords(grids["x_frac"], grids["r_frac"], p, geom.contour)
+    return nozzle_node_coords(x_frac, r_frac, p, geom.contour)
 
 
 def batched_gradients(results: list, flow: FlowConditions, opts: SolverOptions, *,
@@ -444,7 +489,9 @@ def batched_gradients(results: list, flow: FlowConditions, opts: SolverOptions, 
     fams = []
     for idx in parts:
         f = SimpleNamespace(idx=idx)
-        f.cat = concat_operators([results[g].operators for g in idx])
+        ops_g = [results[g].operators for g in idx]
+        f.cat = (DeviceFamily(ops_g) if getattr(ops_g[0], "device_arrays", None) is not None
+                 else concat_operators(ops_g))
         f.bk = bk = HipBackend(f.cat, flow, opts, device=device)
         f.stream = torch.cuda.Stream(device=bk.device)
         dev = bk.device

@@ -411,3 +411,86 @@ def build_channel_operators(ncx: int, ncy: int, order: int, *, kind: str = "tri"
     X = np.broadcast_to(x[None, :], (r.size, x.size)).reshape(-1)
     Y = (r[:, None] * yw[None, :]).reshape(-1)
     return DeviceOperators(topology, np.stack([X, Y], axis=-1), order)
+
+
+class _FamilyTopology:
+    """The host-side topology fields a family backend reads."""
+
+    def __init__(self, kind, geometry_order, n_faces, elem_nodes):
+        self.kind, self.geometry_order, self.n_faces = kind, geometry_order, n_faces
+        self.elem_nodes = elem_nodes
+
+
+class DeviceFamily:
+    """Several designs' ``DeviceOperators`` of one (element kind, order,
+    geometry order) as one block-diagonal problem on the device, in the
+    layout ``batch.concat_operators`` gives host operators: elements and
+    nodes design after design, then the interior edges of every design, then
+    the boundary edges of every design.  Only device arrays are concatenated
+    (offsets applied on the device); nothing is copied back."""
+
+    def __init__(self, ops_list):
+        torch = _torch()
+        o0 = ops_list[0]
+        key = (o0.topology.kind, o0.topology.geometry_order, o0.ref.order)
+        for o in ops_list:
+            if (o.topology.kind, o.topology.geometry_order, o.ref.order) != key:
+                raise ValueError("a family needs one element kind, order and geometry order")
+        self.ref = o0.ref
+        dev = o0.device
+        ne = np.array([o.n_elem for o in ops_list])
+        ni = np.array([o.n_interior for o in ops_list])
+        nb = np.array([o.n_boundary for o in ops_list])
+        nn = np.array([o.topology.n_nodes for o in ops_list])
+        self.elem_offset = eoff = np.concatenate([[0], np.cumsum(ne)])
+        self.interior_offset = ioff = np.concatenate([[0], np.cumsum(ni)])
+        self.boundary_offset = boff = np.concatenate([[0], np.cumsum(nb)])
+        noff = np.concatenate([[0], np.cumsum(nn)])
+        NI = int(ioff[-1])
+        self.n_elem, self.n_interior, self.n_boundary = int(eoff[-1]), NI, int(boff[-1])
+        nf = o0.topology.n_faces
+        straight = o0.straight
+        per_edge = 3 if straight else 3 * o0.ref.n_qface
+        fe_nbr, fe_info, bedge, eg_int, eg_bnd = [], [], [], [], []
+        for d, o in enumerate(ops_list):
+            a = o.device_arrays
+            E, nid, nbd = int(ne[d]), int(ni[d]), int(nb[d])
+            nbr = a["fe_nbr"][: E * nf].to(torch.int64)
+            fe_nbr.append(torch.where(nbr >= 0, nbr + int(eoff[d]), nbr))
+            info = a["fe_info"][: E * nf].to(torch.int64)
+            edge = info >> 3
+            edge = torch.where(edge < nid, edge + int(ioff[d]), edge - nid + NI + int(boff[d]))
+            fe_info.append((edge << 3) | (info & 7))
+            b = a["bedge"][: 2 * nbd].to(torch.int64).view(-1, 2)
+            bedge.append(torch.stack([b[:, 0] + int(eoff[d]), b[:, 1]], dim=1).reshape(-1))
+            eg = a["edge_geo"][: (nid + nbd) * per_edge]
+            eg_int.append(eg[: nid * per_edge])
+            eg_bnd.append(eg[nid * per_edge:])
+        cat = lambda xs: torch.cat(xs) if xs else None  # noqa: E731
+        self.device_arrays = {
+            "geo": cat([o.device_arrays["geo"] for o in ops_list]),
+            "area": cat([o.device_arrays["area"] for o in ops_list]),
+            "minv": None if o0.device_arrays["minv"] is None
+            else cat([o.device_arrays["minv"] for o in ops_list]),
+            "mean_w": None if o0.device_arrays["mean_w"] is None
+            else cat([o.device_arrays["mean_w"] for o in ops_list]),
+            "edge_geo": cat(eg_int + eg_bnd),
+            "fe_nbr": cat(fe_nbr).to(torch.int32),
+            "fe_info": cat(fe_info).to(torch.int32),
+            "bedge": (cat(bedge) if self.n_boundary else torch.zeros(2, dtype=torch.int64,
+                                                                      device=dev)).to(torch.int32),
+        }
+        self.device_fields = {}
+        elem_nodes = np.concatenate([o.topology.elem_nodes + noff[d] for d, o in enumerate(ops_list)])
+        self.topology = _FamilyTopology(*key[:2], nf, elem_nodes)
+        self.node_coords = np.concatenate([np.asarray(o.node_coords) for o in ops_list])
+        self.straight = straight
+        self.device = dev
+
+    @property
+    def n_edges_total(self) -> int:
+        return self.n_interior + self.n_boundary
+
+    @property
+    def elem_area_total(self) -> float:
+        return float(self.device_arrays["area"].sum().item())

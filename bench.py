@@ -40,7 +40,7 @@ after the headline are skipped, and say so, when the elapsed time passes
 ``--budget-s``.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-march]
-                    [--no-config1] [--no-sweep] [--no-adjoint] [--shocked]
+                    [--no-config1] [--no-shocked] [--no-sweep] [--no-adjoint]
                     [--adjoint-stages K] [--adjoint-fd]
 """
 
@@ -282,17 +282,27 @@ def shocked_bench(refine: int, steps: int) -> dict:
 
     mesh, op, cf, ini, be, wl = (_pkg("mesh"), _pkg("operators"), _pkg("config"),
                                  _pkg("initial"), _pkg("backend"), _pkg("workloads"))
-    t0 = time.perf_counter()
+    dm = _pkg("device_mesh")
     geom = mesh.NozzleGeometry(contour="bell", area_ratio=2.5)
-    topo, coords, _ = mesh.build_nozzle_mesh(geom, kind="quad", refine=refine, geometry_order=2)
-    ops = op.build_operators(topo, coords, 2, dense=False)
     flow = cf.FlowConditions(back_pressure_ratio=0.7, allow_shock_in_nozzle=True, flux="hllc")
+    t0 = time.perf_counter()
+    topo, coords, _ = mesh.build_nozzle_mesh(geom, kind="quad", refine=refine, geometry_order=2)
+    hops = op.build_operators(topo, coords, 2, dense=False)
+    ini.quasi1d_values(hops, flow, geom)
+    host_s = time.perf_counter() - t0
+    del hops, topo, coords
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ops, _, _ = dm.build_nozzle_operators(geom, kind="quad", refine=refine, geometry_order=2,
+                                          order=2)
     bk = be.HipBackend(ops, flow, cf.SolverOptions(limiter="positivity"))
-    U0 = ini.quasi1d_initial(ops, flow, geom, backend=bk)  # projected on the device
+    U0 = ini.quasi1d_initial(ops, flow, geom, backend=bk, as_tensor=True)
+    torch.cuda.synchronize()
+    dev_s = time.perf_counter() - t0
     E, N = ops.n_elem, ops.ref.n_basis
-    print(f"shocked: {E} quads Q2 P2, host setup {time.perf_counter() - t0:.1f} s",
-          file=sys.stderr, flush=True)
-    u0 = torch.from_numpy(U0).to(bk.device)
+    print(f"shocked: {E} quads Q2 P2, setup on the device {dev_s:.2f} s (host build "
+          f"{host_s:.1f} s)", file=sys.stderr, flush=True)
+    u0 = U0
     state = {}
 
     def restart():
@@ -318,6 +328,8 @@ def shocked_bench(refine: int, steps: int) -> dict:
         "stage_hbm_frac": round(nb["step"] / (t / steps) / 1e9 / HBM_PEAK_GBS, 4),
         "elements_scaled_per_stage": round(cnt[0] / (3 * steps), 1),
         "residual_norm_last": float(bk._res.item()),
+        "setup_s": round(dev_s, 2), "host_build_s": round(host_s, 2),
+        "setup": "mesh connectivity, operators and quasi-1D initial state on the device",
     }
 
 
@@ -432,12 +444,27 @@ def sweep_bench(n: int, tol: float, n_lone: int, max_it: int = 50000) -> dict:
     import torch
 
     batch, cf, mr = _pkg("batch"), _pkg("config"), _pkg("march")
+    mesh, opm, ini = _pkg("mesh"), _pkg("operators"), _pkg("initial")
     designs = batch.design_sweep(n, 1)
     flow = cf.FlowConditions()
     opts = cf.SolverOptions(tolerance=tol, max_iterations=max_it)
-    run = batch.BatchedSteady(designs, flow, opts)
+    # the host build of every design (mesh, operators, initial state), timed
+    # for comparison with the device build the batch uses
+    t0 = time.perf_counter()
+    for dsg in designs:
+        geom = dsg.geometry()
+        topo, coords, _ = mesh.build_nozzle_mesh(geom, kind=dsg.kind, nx=dsg.nx, nr=dsg.nr,
+                                                 refine=dsg.refine)
+        ini.quasi1d_initial(opm.build_operators(topo, coords, dsg.order, dense=False), flow, geom)
+    host_build_s = time.perf_counter() - t0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run = batch.BatchedSteady(designs, flow, opts, device_setup=True)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
     print(f"sweep: {n} designs, {run.n_elements} elements, {len(run.families)} families, "
-          f"host setup {run.host_setup_s:.1f} s", file=sys.stderr, flush=True)
+          f"{run.n_topologies} grids; setup on the device {setup_s:.2f} s (host build "
+          f"{host_build_s:.1f} s)", file=sys.stderr, flush=True)
     last = [time.perf_counter()]
 
     def progress(done, active):
@@ -513,7 +540,11 @@ def sweep_bench(n: int, tol: float, n_lone: int, max_it: int = 50000) -> dict:
         "lone_designs_per_s": round(lone_rate, 3),
         "speedup": round((n / t_batch) / lone_rate, 2),
         "lone_bit_identical": f"{lone_match}/{len(pick)}",
-        "host_setup_s": round(run.host_setup_s, 1),
+        "setup_s": round(setup_s, 2), "setup": "device: one connectivity build per grid "
+                                                  "shared by its designs, operators and "
+                                                  "quasi-1D initial states on the device",
+        "grids": run.n_topologies,
+        "host_build_s": round(host_build_s, 2),
         "gradients": n_grad, "adjoint_batch_s": round(t_adj, 3),
         "gradients_per_s": round(grad_rate, 4),
         "lone_sweep_gradients_per_s": round(lone_sweep_rate, 4),
@@ -549,8 +580,8 @@ def main() -> int:
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="time budget of each of the four CPU-baseline variants")
-    ap.add_argument("--shocked", action="store_true",
-                    help="also march the shocked quadrilateral case (quads Q2 P2, pb 0.7, HLLC)")
+    ap.add_argument("--no-shocked", action="store_true",
+                    help="skip the shocked quadrilateral march (quads Q2 P2, pb 0.7, HLLC)")
     ap.add_argument("--shocked-refine", type=int, default=7)
     ap.add_argument("--adjoint-refine", type=int, default=5)
     ap.add_argument("--adjoint-fd", action="store_true",
@@ -694,11 +725,12 @@ def main() -> int:
     leg("march", not args.no_march, march_leg)
     leg("cpu_baseline", not args.no_cpu, lambda: cpu_baseline(args.order, args.cpu_seconds))
     leg("config1", not args.no_config1, config1_bench)
+    leg("shocked", not args.no_shocked, lambda: shocked_bench(args.shocked_refine, 20))
     leg("sweep", not args.no_sweep and args.sweep > 0,
         lambda: sweep_bench(args.sweep, args.sweep_tol, args.sweep_lone, args.sweep_maxit))
     leg("adjoint", not args.no_adjoint,
         lambda: adjoint_bench(args.adjoint_refine, fd_check=args.adjoint_fd))
-    leg("shocked", args.shocked, lambda: shocked_bench(args.shocked_refine, max(2, args.steps // 2)))
+
     leg("adjoint_stages", args.adjoint_stages > 0,
         lambda: adjoint_stage_bench(args.adjoint_refine, args.adjoint_stages))
     summary = {"gdof_stage_s": out["value"], "hbm_frac": out["roofline"]["frac"]}

@@ -172,3 +172,25 @@ def test_sweep256_batched_gradients_equal_lone_adjoints(sweep256):
         assert gr.gradient["area_ratio"] == grad["area_ratio"]
         checked += 1
     assert checked >= 4
+
+
+def test_device_setup_batch(sweep):
+    """The same designs with the grids' connectivity built once on the device
+    and shared, operators and initial states built on the device: each design
+    matches its lone solve on the same operators bit for bit, and the
+    host-built batch to round-off (same stop iterations, observables within
+    1e-10)."""
+    run_h, results_h, flow, opts = sweep
+    batch, march, obs = nz_module("batch"), nz_module("march"), nz_module("observables")
+    run = batch.BatchedSteady(run_h.designs, flow, opts, device_setup=True)
+    assert run.n_topologies == 3  # tri r0, quad r0, quad r1 grids
+    results = run.solve(use_graph=True)
+    for g, (res, rh) in enumerate(zip(results, results_h)):
+        lone = march.solve_steady(run.ops_list[g], flow, opts, U0=run.U0_list[g])
+        assert (res.iterations, res.converged) == (lone.iterations, lone.converged)
+        assert np.array_equal(res.U, lone.U)
+        assert (res.iterations, res.converged) == (rh.iterations, rh.converged)
+        p, ph = obs.performance(res.U, res.operators, flow), obs.performance(rh.U, rh.operators, flow)
+        err = abs(p.thrust / ph.thrust - 1)
+        print(f"design {g}: device setup {res.iterations} it, thrust rel diff {err:.1e}", flush=True)
+        assert err <= 1e-10
