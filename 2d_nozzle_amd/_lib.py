@@ -145,6 +145,10 @@ SIGNATURES = {
     "nz_mesh_last_error": (C.c_char_p, []),
     "nz_quasi1d_values": (C.c_int, [C.POINTER(Quasi1DDesc), P, C.c_int64, P, P]),
     "nz_initial_last_error": (C.c_char_p, []),
+    "nz_observables_work_doubles": (C.c_int64, [C.POINTER(Problem)]),
+    "nz_performance_integrals": (C.c_int, [C.POINTER(Problem), P, C.c_double, P, P, P]),
+    "nz_objective": (C.c_int, [C.POINTER(Problem), P, C.c_int32, C.c_double, P, P, P, P, P]),
+    "nz_observables_last_error": (C.c_char_p, []),
 }
 
 _LIB: C.CDLL | None = None

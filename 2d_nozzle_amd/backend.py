@@ -415,27 +415,70 @@ class HipBackend:
                   "nz_project")
         return self._out(out, npy)
 
-    def performance(self, U):
-        """Thrust, mass flow and exit Mach of ``U`` (an ``observables.Performance``)
-        from the device's boundary integrals (``performance``,
-        ``src/post/postprocess.py:193-226``)."""
-        from .observables import Performance
+    def performance(self, U, geom=None, quasi1d=None):
+        """Every performance field of ``U`` (an ``observables.Performance``,
+        ``src/post/postprocess.py:193-255``) from integrals on the device:
+        the boundary integrals, the entropy integral over the domain and the
+        exit temperature integral.  ``geom`` (the ``NozzleGeometry``) gives
+        the throat-based coefficients and the quasi-1D ideal thrust."""
+        from .observables import assemble
 
+        torch = self._torch
         u, _ = self._in(U, "state", "U")
-        out = self._torch.empty(8, dtype=self._torch.float64, device=self.device)
-        self._chk(self.lib.nz_boundary_integrals(self._pp, self._p(u), self._p(out), self._stream()),
+        f64 = dict(dtype=torch.float64, device=self.device)
+        bnd = torch.empty(8, **f64)
+        vol = torch.empty(2, **f64)
+        work = torch.empty(max(int(self.lib.nz_observables_work_doubles(self._pp)), 1), **f64)
+        st = self._stream()
+        flow = self.flow
+        g = flow.gamma
+        s_t = flow.total_pressure ** (1.0 - g) * (flow.Rgas * flow.total_temperature) ** g
+        self._chk(self.lib.nz_boundary_integrals(self._pp, self._p(u), self._p(bnd), st),
                   "nz_boundary_integrals")
-        mi, mo, wall, qi, qo, area, mach, p = out.cpu().tolist()
-        if area == 0.0:  # no outflow boundary: the averages are undefined, as on the host
-            area = float("nan")
-        return Performance(
-            thrust=2.0 * (mo + mi),
-            thrust_wall=2.0 * wall,
-            mass_flow_in=2.0 * -qi,
-            mass_flow_out=2.0 * qo,
-            exit_mach_area_averaged=mach / area,
-            exit_pressure_ratio=p / area / self.flow.total_pressure,
-        )
+        rc = self.lib.nz_performance_integrals(self._pp, self._p(u), s_t, self._p(work),
+                                               self._p(vol), st)
+        if rc != 0:
+            raise _lib.HipError(f"nz_performance_integrals failed ({rc}): "
+                                f"{self.lib.nz_observables_last_error().decode(errors='replace')}")
+        mi, mo, wall, qi, qo, area_e, mach, p = bnd.cpu().tolist()
+        ent, pr = vol.cpu().tolist()
+        return assemble(flow, thrust=2.0 * (mo + mi), thrust_wall=2.0 * wall, mdot_in=2.0 * -qi,
+                        mdot_out=2.0 * qo, mach_int=mach, p_int=p, t_int=pr / flow.Rgas,
+                        area_e=area_e, entropy_int=ent, area=self.area_total, geom=geom,
+                        quasi1d=quasi1d)
+
+    def objective(self, U, name: str = "thrust", *, geom=None):
+        """``(J, dJ/dU, dJ/d(edge geometry))`` of a named objective on the
+        device (``nz_objective``; ``objective_value``,
+        ``src/sensitivity.py:96-130``): ``J`` a float, ``dJ/dU`` a device
+        tensor ``(E, N, 4)``, the edge cotangent ``(F, nqf, 3)`` a device
+        tensor.  ``thrust_coefficient`` needs ``geom`` (the throat height)."""
+        from .sensitivity import OBJECTIVE_IDS, OBJECTIVES
+
+        if name not in OBJECTIVES:
+            raise ValueError(f"unknown objective {name!r}; choose from {OBJECTIVES}")
+        torch = self._torch
+        u, _ = self._in(U, "state", "U")
+        oid = OBJECTIVE_IDS[name]
+        scale = 1.0
+        if name == "thrust_coefficient":
+            if geom is None:
+                raise ValueError("the thrust coefficient needs the geometry (throat height)")
+            scale = 1.0 / (self.flow.total_pressure * 2.0 * geom.throat_height())
+        elif name == "exit_pressure":
+            scale = 1.0 / self.flow.total_pressure
+        f64 = dict(dtype=torch.float64, device=self.device)
+        rd = self.ops.ref
+        J = torch.empty(1, **f64)
+        dU = torch.empty_like(u)
+        dG = torch.empty((self.ops.n_edges_total, rd.n_qface, 3), **f64)
+        work = torch.empty(max(int(self.lib.nz_observables_work_doubles(self._pp)), 1), **f64)
+        rc = self.lib.nz_objective(self._pp, self._p(u), oid, scale, self._p(work), self._p(J),
+                                   self._p(dU), self._p(dG), self._stream())
+        if rc != 0:
+            raise _lib.HipError(f"nz_objective failed ({rc}): "
+                                f"{self.lib.nz_observables_last_error().decode(errors='replace')}")
+        return float(J.item()), dU, dG
 
     def limit(self, U):
         """Limited copy of ``U``; activity counters as ``numba_backend.py:265-280``."""

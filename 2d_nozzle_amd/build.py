@@ -17,7 +17,8 @@ import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SOURCES = [os.path.join(HERE, "csrc", n) for n in ("nozzle_hip.hip", "mesh_build.hip", "initial.hip")]
+SOURCES = [os.path.join(HERE, "csrc", n) for n in ("nozzle_hip.hip", "mesh_build.hip", "initial.hip",
+                                                        "observables.hip")]
 OUT = os.path.join(HERE, "_lib", "libnozzle_hip.so")
 
 HIPCC_FLAGS = [

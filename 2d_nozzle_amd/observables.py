@@ -54,15 +54,84 @@ def mass_flux(tr: BoundaryTrace, gamma: float) -> float:
 
 @dataclass(frozen=True)
 class Performance:
+    """The reference's integral metrics of a solution (field names of
+    ``src/post/postprocess.py:89-150``); fields that need the geometry (the
+    throat and the quasi-1D ideal thrust) are NaN without it."""
+
     thrust: float
     thrust_wall: float
     mass_flow_in: float
     mass_flow_out: float
     exit_mach_area_averaged: float
     exit_pressure_ratio: float
+    thrust_imbalance: float = float("nan")
+    thrust_coefficient: float = float("nan")
+    mass_imbalance: float = float("nan")
+    entropy_error: float = float("nan")
+    exit_temperature_ratio: float = float("nan")
+    specific_thrust: float = float("nan")
+    specific_thrust_ratio: float = float("nan")
+    discharge_coefficient: float = float("nan")
+    ideal_thrust: float = float("nan")
+    thrust_efficiency: float = float("nan")
 
 
-def performance(U, ops, flow) -> Performance:
+def entropy_integral(U, ops, flow) -> float:
+    """Integral over the domain of ``(s / s_t - 1)^2``, ``s = p / rho^gamma``
+    (``entropy_error``, ``src/post/postprocess.py:174-190``, before the
+    division by the area and the square root)."""
+    g = flow.gamma
+    s_t = flow.total_pressure ** (1.0 - g) * (flow.Rgas * flow.total_temperature) ** g
+    uq = np.einsum("iq,eis->eqs", ops.ref.phi_vol, np.asarray(U))
+    rho, _, _, p = _prim(uq, g)
+    ratio = (p / rho**g) / s_t
+    return float(((ratio - 1.0) ** 2 * np.asarray(ops.weighted_det)).sum())
+
+
+def ideal_thrust(q1d, flow) -> float:
+    """Quasi-1D axial force of the same geometry, momentum form, full nozzle
+    (``_ideal_thrust``, ``src/post/postprocess.py:258-263``)."""
+    rho, vx, p, a = q1d.density, q1d.velocity, q1d.pressure, q1d.area
+    return float((rho[-1] * vx[-1] ** 2 + p[-1]) * a[-1] - (rho[0] * vx[0] ** 2 + p[0]) * a[0])
+
+
+def assemble(flow, *, thrust, thrust_wall, mdot_in, mdot_out, mach_int, p_int, t_int, area_e,
+             entropy_int, area, geom=None, quasi1d=None) -> Performance:
+    """The reference's performance fields (``postprocess.py:193-255``) from
+    the integrals, shared by the host and the device paths."""
+    denom = abs(thrust) if abs(thrust) > 0.0 else 1.0
+    nan = float("nan")
+    a_t = cf = cd = ideal = nan
+    if geom is not None:
+        a_t = 2.0 * geom.throat_height()
+        cf = thrust / (flow.total_pressure * a_t)
+        cd = mdot_in / (flow.stagnation_density * flow.stagnation_sound_speed * a_t)
+        if quasi1d is None:
+            from .initial import solve_quasi1d
+
+            quasi1d = solve_quasi1d(geom, flow)
+        ideal = ideal_thrust(quasi1d, flow)
+    area_e = area_e if area_e != 0.0 else nan
+    return Performance(
+        thrust=thrust, thrust_wall=thrust_wall, mass_flow_in=mdot_in, mass_flow_out=mdot_out,
+        exit_mach_area_averaged=mach_int / area_e,
+        exit_pressure_ratio=p_int / area_e / flow.total_pressure,
+        thrust_imbalance=abs(thrust - thrust_wall) / denom,
+        thrust_coefficient=cf,
+        mass_imbalance=abs(mdot_out - mdot_in) / (abs(mdot_in) if mdot_in else 1.0),
+        entropy_error=float(np.sqrt(entropy_int / area)),
+        exit_temperature_ratio=t_int / area_e / flow.total_temperature,
+        specific_thrust=thrust / mdot_in if mdot_in else nan,
+        specific_thrust_ratio=(thrust / mdot_in / flow.stagnation_sound_speed
+                               if mdot_in else nan),
+        discharge_coefficient=cd,
+        ideal_thrust=ideal,
+        thrust_efficiency=thrust / ideal if ideal == ideal and ideal else nan,
+    )
+
+
+def performance(U, ops, flow, geom=None, quasi1d=None) -> Performance:
+    """All performance fields of ``U`` on the host (``postprocess.py:193-255``)."""
     g = flow.gamma
     inlet = boundary_trace(U, ops, BoundaryTag.INFLOW)
     outlet = boundary_trace(U, ops, BoundaryTag.OUTFLOW)
@@ -71,13 +140,11 @@ def performance(U, ops, flow) -> Performance:
     _, _, _, pw = _prim(wall.state, g)
     thrust_wall = 2.0 * float(-(pw * wall.normal[..., 0] * wall.weight).sum())
     rho, vx, vy, p = _prim(outlet.state, g)
-    area = float(outlet.weight.sum())
     mach = np.sqrt((vx * vx + vy * vy) * rho / (g * p))
-    return Performance(
-        thrust=thrust,
-        thrust_wall=thrust_wall,
-        mass_flow_in=2.0 * -mass_flux(inlet, g),
-        mass_flow_out=2.0 * mass_flux(outlet, g),
-        exit_mach_area_averaged=float((mach * outlet.weight).sum() / area),
-        exit_pressure_ratio=float((p * outlet.weight).sum() / area / flow.total_pressure),
-    )
+    t_e = p / (flow.Rgas * rho)
+    return assemble(
+        flow, thrust=thrust, thrust_wall=thrust_wall, mdot_in=2.0 * -mass_flux(inlet, g),
+        mdot_out=2.0 * mass_flux(outlet, g), mach_int=float((mach * outlet.weight).sum()),
+        p_int=float((p * outlet.weight).sum()), t_int=float((t_e * outlet.weight).sum()),
+        area_e=float(outlet.weight.sum()), entropy_int=entropy_integral(U, ops, flow),
+        area=float(np.asarray(ops.elem_area).sum()), geom=geom, quasi1d=quasi1d)

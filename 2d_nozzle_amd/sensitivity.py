@@ -31,7 +31,10 @@ from .operators import build_operators
 
 DESIGN_PARAMETERS = ("area_ratio", "throat_x", "throat_half_height", "inlet_half_height",
                      "theta_initial", "theta_exit", "bezier_w1", "bezier_w2", "length")
-OBJECTIVES = ("thrust", "mass_flow")
+# the reference's objectives (src/sensitivity.py:90) and their ids in nz_objective
+OBJECTIVES = ("thrust", "thrust_coefficient", "exit_mach", "mass_flow", "exit_pressure")
+OBJECTIVE_IDS = {"thrust": 0, "thrust_coefficient": 1, "mass_flow": 2, "exit_mach": 3,
+                 "exit_pressure": 4}
 
 
 # ------------------------------------------------------------------ geometry chain
@@ -177,11 +180,15 @@ def _trace(U, ops, tag):
     return S, phi, ed.bedge_elem[sl], g
 
 
-def objective_and_partials(U, ops, flow, name: str = "thrust"):
-    """``J``, ``dJ/dU`` ``(E, N, 4)`` and ``dJ/d(edge geometry)`` ``(F, nqf, 3)``.
-
-    ``thrust = 2 sum_{out, in} int (rho v_x v_n + p n_x) ds``;
-    ``mass_flow = -2 int_in rho v_n ds`` (``src/sensitivity.py:96-130``).
+def objective_and_partials(U, ops, flow, name: str = "thrust", geom=None):
+    """``J``, ``dJ/dU`` ``(E, N, 4)`` and ``dJ/d(edge geometry)`` ``(F, nqf, 3)``
+    on the host (``objective_value``, ``src/sensitivity.py:96-130``):
+    ``thrust = 2 sum_{out, in} int (rho v_x v_n + p n_x) ds``, the thrust
+    coefficient ``thrust / (p_t 2 y_throat)`` (needs ``geom``),
+    ``mass_flow = -2 int_in rho v_n ds``, and the outflow averages
+    ``int M ds / int ds`` (exit Mach) and ``int p ds / int ds / p_t`` (exit
+    pressure).  Floors contribute no derivative when active.  The device
+    computes the same in ``HipBackend.objective``.
     """
     if name not in OBJECTIVES:
         raise ValueError(f"unknown objective {name!r}; choose from {OBJECTIVES}")
@@ -190,8 +197,19 @@ def objective_and_partials(U, ops, flow, name: str = "thrust"):
     J = 0.0
     dU = np.zeros(np.asarray(U).shape)
     dG = np.zeros((ops.n_edges_total, ops.ref.n_qface, 3))
-    tags = (BoundaryTag.OUTFLOW, BoundaryTag.INFLOW) if name == "thrust" else (BoundaryTag.INFLOW,)
-    fac = 2.0 if name == "thrust" else -2.0
+    if name in ("thrust", "thrust_coefficient"):
+        tags, fac = (BoundaryTag.OUTFLOW, BoundaryTag.INFLOW), 2.0
+        if name == "thrust_coefficient":
+            if geom is None:
+                raise ValueError("the thrust coefficient needs the geometry (throat height)")
+            fac /= flow.total_pressure * 2.0 * geom.throat_height()
+    elif name == "mass_flow":
+        tags, fac = (BoundaryTag.INFLOW,), -2.0
+    else:
+        tags, fac = (BoundaryTag.OUTFLOW,), 1.0
+    averaged = name in ("exit_mach", "exit_pressure")
+    num = area = 0.0
+    keep = []
     for tag in tags:
         S, phi, belem, gidx = _trace(U, ops, tag)
         if S.size == 0:
@@ -212,7 +230,23 @@ def objective_and_partials(U, ops, flow, name: str = "thrust"):
         # d p / d S, with the density floor
         dp = np.stack([(g - 1.0) * ke / rho * drho, -(g - 1.0) * m1 / rho, -(g - 1.0) * m2 / rho,
                        np.full_like(rho, g - 1.0)], axis=-1) * dp_on[..., None]
-        if name == "thrust":
+        if averaged:
+            if name == "exit_mach":
+                q2 = m1 * m1 + m2 * m2
+                Q = np.sqrt(((m1 / rho) ** 2 + (m2 / rho) ** 2) * rho / (g * p))
+                safe = np.where(q2 > 0.0, q2, 1.0)
+                h = 0.5 * Q
+                dQ = np.stack([h * (-drho / rho - dp[..., 0] / p), h * (2 * m1 / safe - dp[..., 1] / p),
+                               h * (2 * m2 / safe - dp[..., 2] / p), h * (-dp[..., 3] / p)], axis=-1)
+                dQ = np.where((q2 > 0.0)[..., None], dQ, 0.0)
+            else:
+                Q = p / flow.total_pressure
+                dQ = dp / flow.total_pressure
+            num += float((Q * ds).sum())
+            area += float(ds.sum())
+            keep.append((phi, belem, gidx, Q, dQ, ds))
+            continue
+        if name in ("thrust", "thrust_coefficient"):
             I = m1 * vn_num / rho + p * nx
             dI = np.stack([-m1 * vn_num / rho**2 * drho, (vn_num + m1 * nx) / rho, m1 * ny / rho,
                            np.zeros_like(rho)], axis=-1) + dp * nx[..., None]
@@ -227,7 +261,22 @@ def objective_and_partials(U, ops, flow, name: str = "thrust"):
         dG[gidx, :, 2] += fac * I * w[None, :]
         gS = fac * dI * ds[..., None]  # (n, nqf, 4)
         np.add.at(dU, belem, np.einsum("kiq,kqs->kis", phi, gS))
+    if averaged:
+        J = num / area
+        for phi, belem, gidx, Q, dQ, ds in keep:
+            dG[gidx, :, 2] += (Q - J) * w[None, :] / area
+            np.add.at(dU, belem, np.einsum("kiq,kqs->kis", phi, dQ * (ds / area)[..., None]))
     return J, dU, dG
+
+
+class _Throat:
+    """The throat height of a design, as the objectives read it."""
+
+    def __init__(self, y):
+        self.y = float(y)
+
+    def throat_height(self) -> float:
+        return self.y
 
 
 # ------------------------------------------------------------------ the case
@@ -308,8 +357,11 @@ class DifferentiableCase:
                 f"the forward solve did not converge, so no gradient can be taken: {fwd.message}")
         ops = fwd.operators
         flow = self.flow_at(params)
-        J, J_U, J_edge = objective_and_partials(fwd.U, ops, flow, objective)
         bk = HipBackend(ops, flow, self.options)
+        # objective and partials on the device (nz_objective); the geometry
+        # cotangent comes back for the host chain to node coordinates
+        J, J_U, J_edge = bk.objective(fwd.U, objective, geom=_Throat(base["throat_half_height"]))
+        J_edge = J_edge.cpu().numpy()
         lam, rel, its, ok = bk.adjoint_solve(fwd.U, J_U, tolerance=adjoint_tolerance,
                                              verbose=verbose, method=adjoint_method)
         if not ok:
@@ -325,5 +377,7 @@ class DifferentiableCase:
             h = fd_step * max(1.0, abs(base[k]))
             dX = (self.coords({**base, k: base[k] + h}) - self.coords({**base, k: base[k] - h})) / (2 * h)
             grad[k] = float((gX * dX).sum())
+            if k == "throat_half_height" and objective == "thrust_coefficient":
+                grad[k] -= J / base[k]  # the coefficient's scale 1 / (p_t 2 y_throat)
         info = {"adjoint_iterations": its, "adjoint_relative_residual": rel}
         return J, grad, fwd, info

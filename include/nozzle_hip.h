@@ -330,6 +330,34 @@ int nz_build_geometry(const nz_geometry_desc *g, const nz_geometry_out *out, voi
 /* Last error of nz_build_edges / nz_build_geometry on this thread. */
 const char *nz_mesh_last_error(void);
 
+/* ---- observables (src/post/postprocess.py:174-255, src/sensitivity.py:96-130) */
+#define NZ_OBJ_THRUST 0
+#define NZ_OBJ_THRUST_COEFFICIENT 1
+#define NZ_OBJ_MASS_FLOW 2
+#define NZ_OBJ_EXIT_MACH 3
+#define NZ_OBJ_EXIT_PRESSURE 4
+
+/* Workspace (doubles) of nz_performance_integrals and nz_objective. */
+int64_t nz_observables_work_doubles(const nz_problem *p);
+
+/* out[0] = integral over the domain of (p / (rho^gamma s_t) - 1)^2 (the
+ * entropy error times the area, entropy_error, postprocess.py:174-190);
+ * out[1] = integral over the outflow of p / rho ds (the exit temperature
+ * ratio's numerator times R).  Summed in a fixed order. */
+int nz_performance_integrals(const nz_problem *p, const double *U, double s_t, double *work,
+                             double *out, void *stream);
+
+/* J (one double), dJ/dU (E, N, 4) and dJ/d(n_x, n_y, |dX/dsigma|) (F, nqf, 3)
+ * of an objective (objective_value, src/sensitivity.py:96-130; ids
+ * NZ_OBJ_*): thrust 2 sum_{out,in} int (rho v_x v_n + p n_x) ds, its
+ * coefficient (scale = 1 / (p_t A_throat)), mass flow -2 int_in rho v_n ds,
+ * and the outflow averages of the Mach number and of p (scale = 1 / p_t).
+ * The density and pressure floors contribute no derivative when active.
+ * Interior edges get zero geometry derivatives. */
+int nz_objective(const nz_problem *p, const double *U, int32_t objective, double scale,
+                 double *work, double *J, double *dU, double *dG, void *stream);
+const char *nz_observables_last_error(void);
+
 /* ---- quasi-1D initial state on the device -------------------------------- */
 #define NZ_CONTOUR_HERMITE 0   /* bell, smooth, moc */
 #define NZ_CONTOUR_CONICAL 1

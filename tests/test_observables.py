@@ -16,6 +16,41 @@ def test_performance_of_reference_state():
     assert abs(perf.exit_pressure_ratio / float(g["perf_exit_pressure_ratio"]) - 1) <= 1e-13
 
 
+def test_all_performance_fields_of_reference_state():
+    """The fields the reference's converged-state fixture holds beyond the
+    boundary integrals (postprocess.py:193-255), with the geometry given."""
+    ob, mesh = nz_module("observables"), nz_module("mesh")
+    g = golden("cfg1_converged")
+    ops, flow = build_case("cfg1", dense=False)
+    geom = mesh.NozzleGeometry(contour="smooth", area_ratio=2.5)
+    perf = ob.performance(g["U"], ops, flow, geom=geom)
+    for k in ("thrust_coefficient", "discharge_coefficient", "exit_temperature_ratio"):
+        assert abs(getattr(perf, k) / float(g[f"perf_{k}"]) - 1) <= 1e-13, k
+    # identities between the fields
+    assert perf.thrust_imbalance == abs(perf.thrust - perf.thrust_wall) / abs(perf.thrust)
+    assert perf.specific_thrust == perf.thrust / perf.mass_flow_in
+    assert perf.thrust_efficiency == perf.thrust / perf.ideal_thrust
+    assert 0.0 < perf.entropy_error < 0.1 and 0.9 < perf.thrust_efficiency < 1.1
+
+
+def test_entropy_error_closed_form():
+    """A uniform state with p = c s_t rho^gamma has s / s_t = c at every
+    point, so the entropy error is |c - 1| exactly (up to rounding)."""
+    import numpy as np
+
+    ob = nz_module("observables")
+    ops, flow = build_case("quad_q2_p2", dense=False)
+    g = flow.gamma
+    s_t = flow.total_pressure ** (1.0 - g) * (flow.Rgas * flow.total_temperature) ** g
+    rho, vx, vy, c = 0.7, 0.3, -0.1, 1.0375
+    p = c * s_t * rho**g
+    state = np.array([rho, rho * vx, rho * vy, p / (g - 1) + 0.5 * rho * (vx * vx + vy * vy)])
+    U = np.broadcast_to(state, (ops.n_elem, ops.ref.n_basis, 4)).copy()
+    area = float(np.asarray(ops.elem_area).sum())
+    err = np.sqrt(ob.entropy_integral(U, ops, flow) / area)
+    assert abs(err - (c - 1.0)) <= 1e-12
+
+
 def test_march_logic_on_the_oracle():
     """The host march's stop rules, driven by the CPU oracle (no GPU)."""
     import numpy as np
