@@ -157,6 +157,7 @@ class DesignResult:
     message: str = ""
     operators: object = None
     flow: FlowConditions | None = None
+    wall_time: float = float("nan")  # seconds from the batch's start to this design's stop
 
 
 class _Family:
@@ -275,7 +276,7 @@ class BatchedSteady:
 
     def __init__(self, designs, flow: FlowConditions | None = None,
                  opts: SolverOptions | None = None, *, device=None, ops_list=None,
-                 U0_list=None, device_setup: bool = False):
+                 U0_list=None, device_setup: bool = False, geoms=None):
         """``device_setup``: build each grid's connectivity once on the device
         and share it among the designs on that grid (the reference rebuilds
         every design's mesh, ``src/api.py:213-244``), each design's operators
@@ -286,7 +287,12 @@ class BatchedSteady:
         self.flow = flow or FlowConditions()
         self.opts = opts or SolverOptions()
         t0 = time.perf_counter()
-        geoms = None
+        if ops_list is not None and U0_list is None:
+            # device operators given: the initial states are formed on the device
+            if geoms is None:
+                geoms = [d.geometry() for d in self.designs]
+        else:
+            geoms = None
         if ops_list is None and device_setup:
             from .device_mesh import build_nozzle_operators
 
@@ -351,7 +357,8 @@ class BatchedSteady:
             results[g] = DesignResult(
                 design=self.designs[g], U=fam.take(d), converged=conv, iterations=done,
                 residual=float(res), residual_initial=float(state[g]["res0"]), message=msg,
-                operators=self.ops_list[g], flow=self.flow)
+                operators=self.ops_list[g], flow=self.flow,
+                wall_time=time.perf_counter() - t0)
 
         def check(first: bool, done: int):
             for fam in self.families:
@@ -622,3 +629,137 @@ def batched_gradients(results: list, flow: FlowConditions, opts: SolverOptions, 
                                     adjoint_iterations=f.its[j],
                                     adjoint_relative_residual=f.rels[j], converged=conv)
     return out
+
+
+# ------------------------------------------------------------- the sweep
+def build_device_designs(designs, device=None):
+    """Device operators of every design, one connectivity build per grid."""
+    from .device_mesh import build_nozzle_operators
+
+    ops_list, topos = [], {}
+    for dsg in designs:
+        key = (dsg.kind, dsg.nx, dsg.nr, dsg.refine, dsg.geometry_order)
+        ops, topos[key], _ = build_nozzle_operators(
+            dsg.geometry(), kind=dsg.kind, nx=dsg.nx, nr=dsg.nr, refine=dsg.refine,
+            geometry_order=dsg.geometry_order, order=dsg.order, topology=topos.get(key),
+            device=device)
+        ops_list.append(ops)
+    return ops_list
+
+
+def _serpentine(shape: tuple[int, ...]) -> list[tuple[int, ...]]:
+    """Grid indices walked so that consecutive points are neighbours: the
+    last axis runs forward and backward alternately."""
+    import itertools
+
+    out = list(itertools.product(*[range(n) for n in shape]))
+    if len(shape) <= 1:
+        return out
+    n_last, walked = shape[-1], []
+    for start in range(0, len(out), n_last):
+        block = out[start:start + n_last]
+        walked += block[::-1] if sum(block[0][:-1]) % 2 == 1 else block
+    return walked
+
+
+SWEEP_AXES = ("area_ratio", "order", "refine")
+
+
+def sweep(*, flow: FlowConditions | None = None, options: SolverOptions | None = None,
+          kind: str = "tri", contour: str = "smooth", order: int = 1, refine: int = 1,
+          nx: int = 14, nr: int = 5, warm_start: bool = False, device=None, **grids):
+    """A parameter sweep on the batched device march, tabulated as the
+    reference's ``sweep`` does (``src/sweep.py:300-446``).
+
+    Iterable keywords among ``area_ratio``, ``order`` and ``refine`` are the
+    axes of a Cartesian grid; the other settings are fixed.  Every point is
+    marched in one batch (grids' connectivity built once on the device,
+    operators and initial states on the device), and each converged point's
+    metrics come from the device's performance integrals.  ``warm_start``:
+    the reference starts each point from the previous one's converged field
+    (serpentine order, ``src/sweep.py:379-414``); batched, the even points of
+    that order march first from the quasi-1D state and each odd point then
+    starts from its converged predecessor on the same grid and order.
+    Returns an ``outputs.SweepResult``.
+    """
+    from .backend import HipBackend
+    from .initial import operating_regime, solve_quasi1d
+    from .march import probe_minima
+    from .outputs import METRICS, SweepResult, sweep_metrics
+
+    flow = flow or FlowConditions()
+    options = options or SolverOptions(print_interval=0)
+    names = tuple(k for k in grids)
+    for k in names:
+        if k not in SWEEP_AXES:
+            raise NotImplementedError(f"the batched sweep varies {SWEEP_AXES}; {k!r} is not one")
+    if not names:
+        raise ValueError("sweep needs at least one iterable parameter grid, e.g. "
+                         "sweep(area_ratio=np.linspace(2, 4, 5))")
+    axes = [np.atleast_1d(np.asarray(list(grids[n]), dtype=float)) for n in names]
+    shape = tuple(len(a) for a in axes)
+    total = int(np.prod(shape))
+    points, designs = [None] * total, [None] * total
+    for idx in np.ndindex(*shape):
+        fi = int(np.ravel_multi_index(idx, shape))
+        pt = {n: float(axes[i][j]) for i, (n, j) in enumerate(zip(names, idx))}
+        points[fi] = pt
+        designs[fi] = Design(area_ratio=pt.get("area_ratio", 2.5), kind=kind,
+                             order=int(pt.get("order", order)), refine=int(pt.get("refine", refine)),
+                             contour=contour, nx=nx, nr=nr)
+    ops_list = build_device_designs(designs, device)
+    results: list = [None] * total
+    if not warm_start:
+        run = BatchedSteady(designs, flow, options, device=device, ops_list=ops_list)
+        results = run.solve()
+    else:
+        walk = [int(np.ravel_multi_index(i, shape)) for i in _serpentine(shape)]
+        first, second = walk[0::2], walk[1::2]
+        run1 = BatchedSteady([designs[g] for g in first], flow, options, device=device,
+                             ops_list=[ops_list[g] for g in first])
+        for g, r in zip(first, run1.solve()):
+            results[g] = r
+        U0 = []
+        for g in second:
+            prev = walk[walk.index(g) - 1]
+            pr = results[prev]
+            same = (designs[prev].kind, designs[prev].order, designs[prev].refine) == (
+                designs[g].kind, designs[g].order, designs[g].refine)
+            U0.append(pr.U if pr.converged and same else None)
+        if second:
+            cold = [u is None for u in U0]
+            if all(cold):
+                run2 = BatchedSteady([designs[g] for g in second], flow, options, device=device,
+                                     ops_list=[ops_list[g] for g in second])
+            else:
+                # initial states: the predecessor's field, or the quasi-1D
+                # state (device-projected) where there is none
+                from .initial import quasi1d_initial
+
+                for j, g in enumerate(second):
+                    if U0[j] is None:
+                        bk = HipBackend(ops_list[g], flow, options, device=device)
+                        U0[j] = quasi1d_initial(ops_list[g], flow, designs[g].geometry(),
+                                                backend=bk)
+                run2 = BatchedSteady([designs[g] for g in second], flow, options, device=device,
+                                     ops_list=[ops_list[g] for g in second], U0_list=U0)
+            for g, r in zip(second, run2.solve()):
+                results[g] = r
+    values = {m: np.full(total, np.nan) for m in METRICS}
+    converged = np.zeros(total, dtype=bool)
+    regimes, messages = [""] * total, [""] * total
+    for g, r in enumerate(results):
+        converged[g], messages[g] = r.converged, r.message
+        geom = designs[g].geometry()
+        q1d = solve_quasi1d(geom, flow)
+        regimes[g] = operating_regime(geom.area_ratio, flow.back_pressure_ratio, flow.gamma)[0].value
+        if not r.converged:
+            continue
+        rmin, pmin = probe_minima(r.U, ops_list[g], flow.gamma)
+        r.min_density, r.min_pressure = rmin, pmin
+        perf = HipBackend(ops_list[g], flow, options, device=device).performance(
+            r.U, geom=geom, quasi1d=q1d)
+        for m, v in sweep_metrics(perf, r, geom, flow, q1d).items():
+            values[m][g] = v
+    return SweepResult(parameters=names, points=points, values=values, converged=converged,
+                       regimes=regimes, messages=messages, shape=shape)
