@@ -47,6 +47,7 @@ class Problem(C.Structure):
         ("tiles", C.c_void_p), ("n_tiles", C.c_int64),
         ("vjp_tiles", C.c_void_p), ("n_vjp_tiles", C.c_int64),
         ("vjp_kernel", C.c_int32), ("pad1", C.c_int32),
+        ("patches", C.c_void_p), ("n_patches", C.c_int64),
     ]
 
 

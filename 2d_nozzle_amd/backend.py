@@ -81,6 +81,10 @@ LOCALITY_MIN_ELEMENTS = 1 << 14
 # autotuner measured fastest with the Roe flux in this process (0 one-wave
 # tiles of 16 elements, 1 four-wave tiles); see HipBackend.tune_vjp
 _VJP_KERNEL: dict = {}
+# (element kind, order, flux, structured grid) -> True when the element
+# kernel on patch tiles measured faster than on tiles (affine meshes of at
+# least LOCALITY_MIN_ELEMENTS elements); see HipBackend.tune_element_kernel
+_DG_PATCHES: dict = {}
 
 
 def _morton2(ix: np.ndarray, iy: np.ndarray) -> np.ndarray:
@@ -120,6 +124,56 @@ def locality_tiles(coords: np.ndarray, elem_nodes: np.ndarray, E: int,
     launch[~ok] = order[~taken]
     e0 = launch * TILE
     return np.stack([e0, np.minimum(e0 + TILE, E), launch], axis=1).astype(np.int64)
+
+
+PATCH_INTS = 18  # per patch: 16 element ids (-1 padded), count, norm slot
+
+
+def patch_order(topo, coords: np.ndarray, E: int) -> np.ndarray:
+    """Element order whose consecutive runs of 16 form compact patches.
+
+    On the structured triangle grids (``topo.logical`` names ``nx_cells`` by
+    ``nr_cells`` cells, elements ``c`` and ``c + ncx ncy`` being the lower and
+    upper triangle of cell ``c = i ncy + j``) the cells follow a Morton curve
+    of ``(i, j)`` with both triangles of a cell together, so aligned runs of 16
+    are blocks of 4 x 2 cells; otherwise the Morton curve of the element
+    centroids.  The order only decides which elements share a workgroup.
+    """
+    lg = getattr(topo, "logical", None) or {}
+    ncx, ncy = int(lg.get("nx_cells", 0)), int(lg.get("nr_cells", 0))
+    e = np.arange(E, dtype=np.int64)
+    if topo.kind == "tri" and ncx > 0 and ncy > 0 and E == 2 * ncx * ncy and max(ncx, ncy) <= 65536:
+        c = e % (ncx * ncy)
+        key = (_morton2(c // ncy, c % ncy) << np.uint64(1)) | (e // (ncx * ncy)).astype(np.uint64)
+    else:
+        cent = coords[topo.elem_nodes].mean(axis=1)
+        lo, span = cent.min(axis=0), np.ptp(cent, axis=0) + 1e-300
+        qc = np.minimum((cent - lo) / span * 65536.0, 65535.0).astype(np.int64)
+        key = _morton2(qc[:, 0], qc[:, 1])
+    return np.argsort(key, kind="stable")
+
+
+def patch_table(order: np.ndarray, n_domains: int = XCDS) -> np.ndarray:
+    """Patch table ``(P, 18)`` int32 for the element kernel: the elements of
+    ``order`` in runs of 16 (ids, -1 padding, count, norm slot = run index),
+    dealt across the L2 domains as :func:`locality_tiles` deals tiles."""
+    E = order.shape[0]
+    n = (E + TILE - 1) // TILE
+    ids = np.full(n * TILE, -1, dtype=np.int64)
+    ids[:E] = order
+    ids = ids.reshape(n, TILE)
+    cnt = np.full(n, TILE, dtype=np.int64)
+    cnt[-1] = E - TILE * (n - 1)
+    share = -(-n // n_domains)
+    b = np.arange(n)
+    pos = (b % n_domains) * share + b // n_domains
+    ok = pos < n
+    launch = np.empty(n, dtype=np.int64)
+    launch[ok] = pos[ok]
+    taken = np.zeros(n, dtype=bool)
+    taken[pos[ok]] = True
+    launch[~ok] = np.flatnonzero(~taken)
+    return np.concatenate([ids[launch], cnt[launch, None], launch[:, None]], axis=1).astype(np.int32)
 
 
 class HipBackend:
@@ -241,10 +295,76 @@ class HipBackend:
         # tile gathers were just read through the same L2 (norm slots keep the
         # element order, so results do not depend on the launch order)
         self._tiles = None
+        self._patches = None
         if E >= LOCALITY_MIN_ELEMENTS:
             tab = locality_tiles(np.asarray(ops.node_coords), np.asarray(topo.elem_nodes), E)
             self._tiles = torch.from_numpy(tab.reshape(-1)).to(self.device)
             P.tiles, P.n_tiles = C.c_void_p(self._tiles.data_ptr()), tab.shape[0]
+            if self.affine:
+                self.set_patches(_DG_PATCHES.get(self._dg_key, False))
+
+    def set_patches(self, on: bool) -> None:
+        """Run the element kernel on patch tiles (affine meshes: each shared
+        edge's flux solved once) or on consecutive / locality-ordered tiles.
+        The two give bit-identical states; the residual norm's partial sums are
+        grouped by workgroup, so it agrees to rounding.  Drops captured march
+        chunks, whose launches hold the old problem."""
+        if on and not self.affine:
+            raise ValueError("patch tiles need an affine (straight-sided triangle) mesh")
+        if on and self._patches is None:
+            ops = self.ops
+            order = patch_order(ops.topology, np.asarray(ops.node_coords), self.E)
+            self._patches = self._torch.from_numpy(patch_table(order).reshape(-1)).to(self.device)
+        self.P.patches = C.c_void_p(self._patches.data_ptr() if on else 0)
+        self.P.n_patches = self._patches.numel() // PATCH_INTS if on else 0
+        nscr = int(self.lib.nz_scratch_doubles(self._pp))  # one norm partial per patch
+        if nscr < 0:
+            _lib.check(-1, "nz_scratch_doubles")
+        if self._scratch.numel() < nscr:
+            self._scratch = self._torch.empty(nscr, dtype=self._torch.float64, device=self.device)
+        self._graphs.clear()
+        self._graph_seen.clear()
+
+    @property
+    def patches(self) -> bool:
+        return bool(self.P.patches)
+
+    @property
+    def _dg_key(self):
+        lg = getattr(self.ops.topology, "logical", None) or {}
+        return (self.ops.topology.kind, int(self.ops.ref.order), int(self.P.flux_id),
+                bool(lg.get("nx_cells")))
+
+    def tune_element_kernel(self, U) -> bool:
+        """Autotuner for the element kernel's work shape on affine meshes of
+        at least LOCALITY_MIN_ELEMENTS elements: times three fused SSP-RK3
+        steps from ``U`` on locality-ordered tiles and on patch tiles, keeps
+        the faster for this (element kind, order, flux, structured or not) for
+        the rest of the process, and returns True for patches.  The two give
+        the same states bit for bit.  Smaller or curved meshes keep tiles."""
+        if not self.affine or self._tiles is None:
+            return False
+        key = self._dg_key
+        if key not in _DG_PATCHES:
+            torch = self._torch
+            self._check_shape(U, "state", "U")
+            u, _ = self._in(U)
+            st = torch.cuda.current_stream(self.device)
+            cost = []
+            for on in (False, True):
+                self.set_patches(on)
+                self._S[0].copy_(u)
+                self.advance(self._S[0], self._S[1], 1, "ssprk3")
+                self._S[0].copy_(u)
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(st)
+                self.advance(self._S[0], self._S[1], 3, "ssprk3")
+                e1.record(st)
+                e1.synchronize()
+                cost.append(e0.elapsed_time(e1))
+            _DG_PATCHES[key] = cost[1] < cost[0]
+        self.set_patches(_DG_PATCHES[key])
+        return _DG_PATCHES[key]
 
     # ------------------------------------------------------------ set-up
     def _init_host_geometry(self, ops, rd, topo, ed, E, N, nqv, nqf, nface, ni):

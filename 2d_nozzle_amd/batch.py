@@ -209,6 +209,8 @@ class _Family:
         self.seg = torch.tensor(np.asarray(seg, dtype=np.int64), device=dev)
         self.counts = torch.tensor([4.0 * (self.eoff[d + 1] - self.eoff[d]) * self.N
                                     for d in self.active], dtype=torch.float64, device=dev)
+        if self.bk.patches:  # the family's table decides which designs march
+            self.bk.set_patches(False)
         P = self.bk.P
         P.tiles = C.c_void_p(self.tiles.data_ptr()) if tiles else C.c_void_p(0)
         P.n_tiles = len(tiles)
@@ -500,6 +502,8 @@ def batched_gradients(results: list, flow: FlowConditions, opts: SolverOptions, 
         f.cat = (DeviceFamily(ops_g) if getattr(ops_g[0], "device_arrays", None) is not None
                  else concat_operators(ops_g))
         f.bk = bk = HipBackend(f.cat, flow, opts, device=device)
+        if bk.patches:  # patches would mix the designs' elements
+            bk.set_patches(False)
         f.stream = torch.cuda.Stream(device=bk.device)
         dev = bk.device
         f.parts = [objective_and_partials(results[g].U, results[g].operators, flow, objective)

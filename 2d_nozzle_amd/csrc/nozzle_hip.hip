@@ -275,7 +275,7 @@ struct Tile {
     // (128 VGPRs) so that the LDS and the registers allow the same
     // occupancy: the fourth wave hides more of the latency than the few
     // spilled registers cost.
-    static constexpr int LDS = 8 * (WORK + NF * N * NQF + EGN + GEN) + 8 * TE * NF;
+    static constexpr int LDS = 8 * (WORK + NF * N * NQF + EGN + GEN + TE) + 8 * TE * NF;
     static constexpr int WPE = LDS * 16 <= 160 * 1024 ? 4 : 1;
 };
 
@@ -493,6 +493,181 @@ __device__ __forceinline__ void store_tile(double* __restrict__ out, const doubl
     }
 }
 
+// The volume terms of an element tile (shared by k_dg and k_dgp): the state
+// interpolated to the volume points (four points per round, one per lane
+// group), the Euler fluxes at one (point, element) item per lane, their
+// contravariant components through LDS, and the projection onto the basis
+// accumulated into acc.  eid(pe) is the element held by tile slot pe.
+template <class C, class L, class EID>
+__device__ __forceinline__ void dg_volume(const nz_problem& P, const double* __restrict__ T,
+                                          double* sW, const double* sGe,
+                                          const double (&u)[C::N], double (&acc)[C::N], int t,
+                                          int cs, int ne, bool fold, EID eid) {
+    constexpr int N = C::N, NQV = C::NQV;
+    double* sUq = sW;                   // point states, then xi-flux
+    double* sFy = sW + L::QR * QS;      // eta-flux
+    // point-major volume tables: phi, dxi, deta, axi, aeta (N each) per point
+    const double* vt = T + P.off.vol_t;
+    const int vproj = fold ? 3 * N : N;
+#pragma unroll 1
+    for (int q0 = 0; q0 < NQV; q0 += L::QR) {
+#pragma unroll 2  // two points' table rows in flight
+        for (int qq = 0; qq < L::QR; ++qq) {
+            if (q0 + qq < NQV) {
+                double a = 0.0;
+                const double* ph = vt + (q0 + qq) * 5 * N;
+#pragma unroll
+                for (int i = 0; i < N; ++i) a += ph[i] * u[i];
+                sUq[qq * QS + cs] = a;
+            }
+        }
+        wave_sync();
+        {
+            const int qq = t / TE, pe = t % TE, q = q0 + qq;
+            const int64_t ep = eid(pe);
+            if (q < NQV && pe < ne) {
+                double uq[4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) uq[c] = sUq[at(qq, c, pe)];
+                const Prim<double> w = primitives(uq, P.gamma);
+                const double F[4] = {w.r * w.u, w.r * w.u * w.u + w.p, w.r * w.u * w.v, w.r * w.u * w.H};
+                const double G[4] = {w.r * w.v, w.r * w.u * w.v, w.r * w.v * w.v + w.p, w.r * w.v * w.H};
+                double gx, gy, hx, hy, wd;
+                if (C::AFF) {
+                    const double* ge = sGe + pe * 5;
+                    gx = ge[0];
+                    gy = ge[1];
+                    hx = ge[2];
+                    hy = ge[3];
+                    wd = T[P.off.w_vol + q] * ge[4];
+                } else {
+                    const double* gq = P.geo + (ep * NQV + q) * 5;
+                    gx = gq[0];
+                    gy = gq[1];
+                    hx = gq[2];
+                    hy = gq[3];
+                    wd = gq[4];
+                }
+                const double sc = fold ? 1.0 : wd;  // exact when folded
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    sUq[at(qq, c, pe)] = (gx * F[c] + gy * G[c]) * sc;
+                    sFy[at(qq, c, pe)] = (hx * F[c] + hy * G[c]) * sc;
+                }
+            }
+        }
+        wave_sync();
+#pragma unroll 1
+        for (int qq = 0; qq < L::QR; ++qq) {
+            const int q = q0 + qq;
+            if (q < NQV) {
+                const double a = sUq[qq * QS + cs], b = sFy[qq * QS + cs];
+                const double* pr = vt + q * 5 * N + vproj;
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    acc[i] = fma(-pr[i], a, acc[i]);
+                    acc[i] = fma(-pr[N + i], b, acc[i]);
+                }
+            }
+        }
+        // the next round rewrites these rows after the reads above
+    }
+
+}
+
+// The epilogue of an element tile (shared by k_dg and k_dgp): the inverse
+// mass of non-affine elements, then either the residual / rate rows, or the
+// fused Runge-Kutta stage (time step, combination, norm partial, positivity
+// limiter); the tile's rows staged in LDS go out through store().
+template <class C, class L, bool STAGE, class STORE>
+__device__ __forceinline__ void dg_epilogue(
+    const nz_problem& P, const double* __restrict__ T, double* sW, const int* sIn,
+    double (&u)[C::N], double (&acc)[C::N], double wave, int t, int el, int s, int64_t e, bool live,
+    int64_t slot, const double* __restrict__ Un, double* __restrict__ dt, double dt_factor,
+    double ca, double cb, int mode, double* __restrict__ partial, uint8_t* __restrict__ flags,
+    unsigned long long* __restrict__ counters, STORE store) {
+    constexpr int N = C::N, NF = C::NF;
+    // acc holds R (unfolded) or -rate (affine, folded) for component s
+    if ((STAGE || mode != kRes) && !C::AFF && live) {
+        double r[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const double* me = P.minv + e * sym_size(N);
+            double a = 0.0;
+#pragma unroll
+            for (int j = 0; j < N; ++j) a += me[sym_index(N, i, j)] * acc[j];
+            r[i] = a;
+        }
+#pragma unroll
+        for (int i = 0; i < N; ++i) acc[i] = r[i];
+    }
+    wave_sync();  // last reads of the volume rows are done; reuse sW for staging
+    if (!STAGE) {
+        const double osign = mode == kRes ? 1.0 : -1.0;  // rate = -M^-1 R
+#pragma unroll
+        for (int i = 0; i < N; ++i) sW[el * L::SIO + i * 4 + s] = osign * acc[i];
+        if (s == 0 && live) dt[e] = wave;
+        wave_sync();
+        store();
+        return;
+    }
+
+    // fused stage: time step, combination, norm partials, limiter, store
+    double h = 0.0;
+    if (live) {
+        if (mode == kStageFirst) {
+            const double d = wave < kFloor ? kFloor : wave;
+            h = dt_factor * 2.0 * P.area[e] / d;
+            if (s == 0) dt[e] = h;
+        } else {
+            h = dt[e];
+        }
+    }
+    double nacc = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const double F = -acc[i];
+        if (mode == kStageFirst) {
+            u[i] = u[i] + h * F;
+            const double v = F * P.wnorm[s];
+            nacc += live ? v * v : 0.0;
+        } else {
+            u[i] = live ? ca * Un[(e * N + i) * 4 + s] + cb * (u[i] + h * F) : 0.0;
+        }
+    }
+    if (mode == kStageFirst) {
+        double tot = nacc;
+        for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o, 64);
+        if (t == 0) partial[slot] = tot;
+    }
+    if (P.limiter == 1 && N > 1) {  // superbee runs as separate passes
+        uint8_t fl = 0;
+        int scaled = 0;
+        if (live) {
+            int sidebits = 0;
+#pragma unroll
+            for (int f = 0; f < NF; ++f) sidebits |= ((sIn[el * NF + f] >> 2) & 1) << f;
+            double mw[N];
+#pragma unroll
+            for (int i = 0; i < N; ++i) mw[i] = C::AFF ? T[P.off.mean_w + i] : P.mean_w[e * N + i];
+            fl = limit_column<C>(u, s, mw, T, P, sidebits, &scaled);
+        }
+        if (counters) {
+            wave_count(&counters[0], live && s == 0 && scaled);
+            wave_count(&counters[1], live && s == 0 && (fl & 1));
+        }
+        if (flags && live && s == 0) flags[e] = fl;
+    } else if (flags && live && s == 0) {
+        flags[e] = 0;
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) sW[el * L::SIO + i * 4 + s] = u[i];
+    wave_sync();
+    store();
+
+
+}
+
 template <class C, int FLUX, bool STAGE>
 __global__ __launch_bounds__(DG_BS) __attribute__((amdgpu_waves_per_eu(Tile<C>::WPE)))
 void k_dg(const nz_problem P, const double* __restrict__ T,
@@ -510,6 +685,7 @@ void k_dg(const nz_problem P, const double* __restrict__ T,
     __shared__ double sPF[NPF];
     __shared__ double sEG[L::EGN];
     __shared__ double sGe[L::GEN];
+    __shared__ double sRd[C::AFF ? TE : 1];  // 1 / det J of the affine elements
     __shared__ int sNb[TE * NF], sIn[TE * NF];
 
     const int t = threadIdx.x;
@@ -605,6 +781,8 @@ void k_dg(const nz_problem P, const double* __restrict__ T,
         acc[i] = 0.0;
     }
     __builtin_amdgcn_s_setprio(0);
+    // folded lift signs: (+-1) / det J, as +-(1 / det J) (the same double)
+    if (C::AFF && t < TE) sRd[t] = 1.0 / sGe[t * 5 + 4];
     double wave = 0.0;
     wave_sync();  // the staging rows are reused below
 
@@ -698,7 +876,7 @@ void k_dg(const nz_problem P, const double* __restrict__ T,
                     sp = bf[4];
                 }
                 double sg = sd ? -1.0 : 1.0;
-                if (fold) sg = sg / sGe[pe * 5 + 4];
+                if (fold) sg = sd ? -sRd[pe] : sRd[pe];
 #pragma unroll
                 for (int c = 0; c < 4; ++c) sTo[at(qo, c, pe)] = sg * fx[c];
                 sSp[q * TE + pe] = sp;
@@ -732,154 +910,334 @@ void k_dg(const nz_problem P, const double* __restrict__ T,
     }
     wave_sync();
 
-    // --------------------------------------------------------------- volume
-    double* sUq = sW;                   // point states, then xi-flux
-    double* sFy = sW + L::QR * QS;      // eta-flux
-    // point-major volume tables: phi, dxi, deta, axi, aeta (N each) per point
-    const double* vt = T + P.off.vol_t;
-    const int vproj = fold ? 3 * N : N;
-#pragma unroll 1
-    for (int q0 = 0; q0 < NQV; q0 += L::QR) {
-#pragma unroll 2  // two points' table rows in flight
-        for (int qq = 0; qq < L::QR; ++qq) {
-            if (q0 + qq < NQV) {
-                double a = 0.0;
-                const double* ph = vt + (q0 + qq) * 5 * N;
-#pragma unroll
-                for (int i = 0; i < N; ++i) a += ph[i] * u[i];
-                sUq[qq * QS + cs] = a;
-            }
-        }
-        wave_sync();
-        {
-            const int qq = t / TE, pe = t % TE, q = q0 + qq;
-            const int64_t ep = e0 + pe;
-            if (q < NQV && pe < ne) {
-                double uq[4];
-#pragma unroll
-                for (int c = 0; c < 4; ++c) uq[c] = sUq[at(qq, c, pe)];
-                const Prim<double> w = primitives(uq, P.gamma);
-                const double F[4] = {w.r * w.u, w.r * w.u * w.u + w.p, w.r * w.u * w.v, w.r * w.u * w.H};
-                const double G[4] = {w.r * w.v, w.r * w.u * w.v, w.r * w.v * w.v + w.p, w.r * w.v * w.H};
-                double gx, gy, hx, hy, wd;
-                if (C::AFF) {
-                    const double* ge = sGe + pe * 5;
-                    gx = ge[0];
-                    gy = ge[1];
-                    hx = ge[2];
-                    hy = ge[3];
-                    wd = T[P.off.w_vol + q] * ge[4];
-                } else {
-                    const double* gq = P.geo + (ep * NQV + q) * 5;
-                    gx = gq[0];
-                    gy = gq[1];
-                    hx = gq[2];
-                    hy = gq[3];
-                    wd = gq[4];
-                }
-                const double sc = fold ? 1.0 : wd;  // exact when folded
-#pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    sUq[at(qq, c, pe)] = (gx * F[c] + gy * G[c]) * sc;
-                    sFy[at(qq, c, pe)] = (hx * F[c] + hy * G[c]) * sc;
-                }
-            }
-        }
-        wave_sync();
-#pragma unroll 1
-        for (int qq = 0; qq < L::QR; ++qq) {
-            const int q = q0 + qq;
-            if (q < NQV) {
-                const double a = sUq[qq * QS + cs], b = sFy[qq * QS + cs];
-                const double* pr = vt + q * 5 * N + vproj;
-#pragma unroll
-                for (int i = 0; i < N; ++i) {
-                    acc[i] = fma(-pr[i], a, acc[i]);
-                    acc[i] = fma(-pr[N + i], b, acc[i]);
-                }
-            }
-        }
-        // the next round rewrites these rows after the reads above
-    }
+    dg_volume<C, L>(P, T, sW, sGe, u, acc, t, cs, ne, fold,
+                    [&](int pe) -> int64_t { return e0 + pe; });
+    dg_epilogue<C, L, STAGE>(P, T, sW, sIn, u, acc, wave, t, el, s, e, live, slot, Un, dt,
+                             dt_factor, ca, cb, mode, partial, flags, counters,
+                             [&]() { store_tile<L>(Uout + e0 * NS, sW, t, ne); });
+}
 
-    // ------------------------------------------------------------ epilogue
-    // acc holds R (unfolded) or -rate (affine, folded) for component s
-    if ((STAGE || mode != kRes) && !C::AFF && live) {
-        double r[N];
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-            const double* me = P.minv + e * sym_size(N);
-            double a = 0.0;
-#pragma unroll
-            for (int j = 0; j < N; ++j) a += me[sym_index(N, i, j)] * acc[j];
-            r[i] = a;
-        }
-#pragma unroll
-        for (int i = 0; i < N; ++i) acc[i] = r[i];
-    }
-    wave_sync();  // last reads of the volume rows are done; reuse sW for staging
-    if (!STAGE) {
-        const double osign = mode == kRes ? 1.0 : -1.0;  // rate = -M^-1 R
-#pragma unroll
-        for (int i = 0; i < N; ++i) sW[el * L::SIO + i * 4 + s] = osign * acc[i];
-        if (s == 0 && live) dt[e] = wave;
-        wave_sync();
-        store_tile<L>(Uout + e0 * NS, sW, t, ne);
-        return;
-    }
+// ------------------------------------------------------ patch tiles
+// k_dgp: k_dg on patch tiles, for affine triangles.  A patch is 16 elements
+// that are close in space (consecutive along a Morton curve of the element
+// centroids), listed in the patch table by element id; about two thirds of
+// its 48 element faces are shared by two of its own elements.  Each interior
+// edge's Riemann flux is then solved once, by one (edge, point) item, and
+// written into both elements' faces; only faces whose neighbour lies outside
+// the patch gather a neighbour column.
+//
+//   plan    one lane per face slot (element el, face f): the neighbour is in
+//           the patch (its slot is the partner), outside it (an off-tile
+//           face, numbered in the compact list of those), or the boundary.
+//           The edge is owned by its side-0 slot, or by the only slot in the
+//           patch; owners are numbered in the item list (wave ballots).
+//   traces  every element's traces on all its faces go to LDS, in the
+//           face's canonical point order (side-0 table), as k_dg forms them.
+//   off     the off-tile neighbours' traces: one (off-tile face, component)
+//           item per lane, the neighbour column gathered from global memory.
+//   flux    one (edge, point) item per lane: the owner's trace and the
+//           partner's, the off-tile trace or the precomputed boundary flux;
+//           the weighted flux is written back, signed and folded, into the
+//           owner's face and the partner's.  Signal speeds are combined per
+//           face with an LDS integer max of their bit patterns (exact for
+//           non-negative doubles; NaN speeds are dropped, as in k_dg).
+//   lift    each element lifts its faces in face order, as k_dg does.
+// Every flux sees exactly the operands k_dg gives it and every sum keeps
+// k_dg's order, so the rate and the stage are bit-identical to k_dg's.
+constexpr int PATCH_INTS = 18;  // per patch: 16 element ids (-1 padded), count, norm slot
+constexpr int XMAX = 12;        // off-tile faces with LDS ghost rows (a 4 x 2 cell block has 12)
 
-    // fused stage: time step, combination, norm partials, limiter, store
-    double h = 0.0;
-    if (live) {
-        if (mode == kStageFirst) {
-            const double d = wave < kFloor ? kFloor : wave;
-            h = dt_factor * 2.0 * P.area[e] / d;
-            if (s == 0) dt[e] = h;
-        } else {
-            h = dt[e];
+// LDS slot of component c of patch element el in face-point row r (rows
+// (face, point), 64 doubles each, packed).  The component blocks rotate
+// with the row and the upper two are XOR-swizzled as in at(): the lanes of
+// one (element, component) per lane pattern touch 32 distinct banks per
+// half-wave, and the four points of one flux item fall in four banks.
+__device__ __forceinline__ int atp(int r, int c, int el) {
+    const int cc = (c + r) & 3;
+    return r * 4 * TE + cc * TE + (el ^ ((cc >> 1) << 3));
+}
+
+template <class C>
+struct PTile {
+    static constexpr int N = C::N, NQF = C::NQF, NF = C::NF, NS = C::NS;
+    static constexpr int QR = DG_BS / TE;
+    static constexpr int SL = TE * NF;        // face slots of a patch
+    static constexpr int TR = NF * NQF * 4 * TE;  // traces, then weighted fluxes: rows (f, q)
+    static constexpr int GH = XMAX * NQF * 4; // off-tile neighbour traces
+    static constexpr int FACE = TR + GH;
+    static constexpr int VOL = 2 * QR * QS;
+    static constexpr int SIO = NS;
+    static constexpr int IO = TE * SIO;
+    static constexpr int WORK = FACE > VOL ? (FACE > IO ? FACE : IO) : (VOL > IO ? VOL : IO);
+    static constexpr int GEN = TE * 5;
+    static constexpr int LDS = 8 * (WORK + GEN + TE + 2 * SL) + 4 * (4 * SL + XMAX + TE);
+    // waves per SIMD that the LDS allows, at most four (128 VGPRs)
+    static constexpr int WPE = LDS * 16 <= 160 * 1024 ? 4 : (LDS * 12 <= 160 * 1024 ? 3 : 2);
+};
+
+template <class C, int FLUX, bool STAGE>
+__global__ __launch_bounds__(DG_BS) __attribute__((amdgpu_waves_per_eu(PTile<C>::WPE)))
+void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __restrict__ Uin,
+           const double* __restrict__ Un, double* __restrict__ Uout, double* __restrict__ dt,
+           double dt_factor, double ca, double cb, int mode, double* __restrict__ partial,
+           uint8_t* __restrict__ flags, unsigned long long* __restrict__ counters) {
+    using L = PTile<C>;
+    constexpr int N = C::N, NQF = C::NQF, NF = C::NF, NS = C::NS, SL = L::SL;
+    static_assert(C::AFF && C::STR, "patch tiles serve affine elements");
+    static_assert(L::SL <= DG_BS, "one lane per face slot");
+    __shared__ double sW[L::WORK];
+    __shared__ double sGe[L::GEN];
+    __shared__ double sRd[TE];               // 1 / det J
+    __shared__ unsigned long long sSp[SL];  // per face slot: max signal speed (bits)
+    __shared__ double sLen[SL];             // per face slot: |dX/dsigma| sum_q w_q
+    __shared__ int sNb[SL], sIn[SL], sPlan[SL], sItem[SL], sExt[XMAX], sId[TE];
+
+    const int t = threadIdx.x;
+    const int el = t >> 2, s = t & 3;
+    const int32_t* pt = P.patches + static_cast<int64_t>(blockIdx.x) * PATCH_INTS;
+    const int ne = pt[TE];
+    const int64_t slot = pt[TE + 1];
+    const bool live = el < ne;
+    const int myid = pt[el];
+    const int64_t e = live ? myid : 0;
+    const bool fold = STAGE || mode != kRes;
+    const int cs = at(0, s, el);  // the volume rows keep k_dg's layout
+
+    __builtin_amdgcn_s_setprio(2);
+    if (s == 0) sId[el] = myid;
+    wave_sync();
+    {
+        constexpr int NU = TE * NS / DG_BS;  // = N
+        constexpr int NM = (SL + DG_BS - 1) / DG_BS;
+        constexpr int NG = (L::GEN + DG_BS - 1) / DG_BS;
+        double ru[NU], rg[NG];
+        int rn[NM], ri[NM];
+#pragma unroll
+        for (int j = 0; j < NU; ++j) {
+            const int it = t + j * DG_BS, r = it / NS, c = it - r * NS;
+            ru[j] = r < ne ? Uin[static_cast<int64_t>(sId[r]) * NS + c] : 0.0;
         }
+#pragma unroll
+        for (int j = 0; j < NM; ++j) {
+            const int it = t + j * DG_BS, r = it / NF, f = it - r * NF;
+            const bool ok = it < SL && r < ne;
+            rn[j] = ok ? P.fe_nbr[static_cast<int64_t>(sId[r]) * NF + f] : -1;
+            ri[j] = ok ? P.fe_info[static_cast<int64_t>(sId[r]) * NF + f] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < NG; ++j) {
+            const int it = t + j * DG_BS, r = it / 5, c = it - r * 5;
+            rg[j] = it < L::GEN && r < ne ? P.geo[static_cast<int64_t>(sId[r]) * 5 + c] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < NU; ++j) sW[t + j * DG_BS] = ru[j];
+#pragma unroll
+        for (int j = 0; j < NM; ++j) {
+            const int it = t + j * DG_BS;
+            if (it < SL) {
+                sNb[it] = rn[j];
+                sIn[it] = ri[j];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NG; ++j)
+            if (t + j * DG_BS < L::GEN) sGe[t + j * DG_BS] = rg[j];
     }
-    double nacc = 0.0;
+    wave_sync();
+    double u[N], acc[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-        const double F = -acc[i];
-        if (mode == kStageFirst) {
-            u[i] = u[i] + h * F;
-            const double v = F * P.wnorm[s];
-            nacc += live ? v * v : 0.0;
+        u[i] = sW[el * L::SIO + i * 4 + s];
+        acc[i] = 0.0;
+    }
+    __builtin_amdgcn_s_setprio(0);
+    if (t < TE) sRd[t] = 1.0 / sGe[t * 5 + 4];
+
+    // ---- plan: one lane per face slot
+    bool owner = false, offt = false;
+    int plan = -1;
+    if (t < SL && t / NF < ne) {
+        const int nb = sNb[t];
+        if (nb < 0) {
+            owner = true;  // boundary edge: this slot is its side 0
         } else {
-            u[i] = live ? ca * Un[(e * N + i) * 4 + s] + cb * (u[i] + h * F) : 0.0;
+            int j = -1;
+#pragma unroll
+            for (int k = 0; k < TE; ++k) j = sId[k] == nb && k < ne ? k : j;
+            if (j >= 0) {
+                plan = j * NF + (sIn[t] & 3);
+                owner = ((sIn[t] >> 2) & 1) == 0;
+            } else {
+                offt = owner = true;
+            }
         }
     }
-    if (mode == kStageFirst) {
-        double tot = nacc;
-        for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o, 64);
-        if (t == 0) partial[slot] = tot;
-    }
-    if (P.limiter == 1 && N > 1) {  // superbee runs as separate passes
-        uint8_t fl = 0;
-        int scaled = 0;
-        if (live) {
-            int sidebits = 0;
-#pragma unroll
-            for (int f = 0; f < NF; ++f) sidebits |= ((sIn[el * NF + f] >> 2) & 1) << f;
-            double mw[N];
-#pragma unroll
-            for (int i = 0; i < N; ++i) mw[i] = C::AFF ? T[P.off.mean_w + i] : P.mean_w[e * N + i];
-            fl = limit_column<C>(u, s, mw, T, P, sidebits, &scaled);
+    const unsigned long long m_own = __ballot(owner), m_off = __ballot(offt);
+    const int n_items = __popcll(m_own), n_off = __popcll(m_off);
+    {
+        const unsigned long long below = (1ull << t) - 1ull;
+        if (owner) sItem[__popcll(m_own & below)] = t;
+        if (offt) {
+            const int k = __popcll(m_off & below);
+            if (k < XMAX) sExt[k] = t;
+            plan = -2 - k;
         }
-        if (counters) {
-            wave_count(&counters[0], live && s == 0 && scaled);
-            wave_count(&counters[1], live && s == 0 && (fl & 1));
+        if (t < SL) {
+            sPlan[t] = plan;
+            sSp[t] = 0ull;
         }
-        if (flags && live && s == 0) flags[e] = fl;
-    } else if (flags && live && s == 0) {
-        flags[e] = 0;
     }
+    wave_sync();  // the staging rows are reused below
+
+    double* sT = sW;             // traces, then weighted fluxes
+    double* sG = sW + L::TR;     // off-tile neighbour traces [q][c][k]
+    const int lift_off = fold ? P.off.lift : P.off.phi_face;
+
+    // ---- off-tile neighbour traces: one (off-tile face, component) item per
+    // lane for the first XMAX off-tile faces (the flux items of any further
+    // ones form their neighbour's trace themselves).  The neighbour columns
+    // are requested first; the own traces are formed while they arrive.
+    static_assert(4 * XMAX <= DG_BS, "one ghost item per lane");
+    const int n_gh = n_off < XMAX ? n_off : XMAX;
+    const bool gl = t < 4 * n_gh;
+    const int gk = t >> 2, gc = t & 3, gsl = gl ? sExt[gk] : 0;
+    double xn[N];
+    {
+        const double* ur = Uin + (gl ? static_cast<int64_t>(sNb[gsl]) * NS : 0) + gc;
 #pragma unroll
-    for (int i = 0; i < N; ++i) sW[el * L::SIO + i * 4 + s] = u[i];
+        for (int i = 0; i < N; ++i) xn[i] = ur[i * 4];
+    }
+    // ---- own traces on every face, canonical point order
+#pragma unroll 1
+    for (int f = 0; f < NF; ++f) {
+        const double* b = T + P.off.phi_face + f * N * NQF;
+#pragma unroll
+        for (int q = 0; q < NQF; ++q) {
+            double a = 0.0;
+#pragma unroll
+            for (int i = 0; i < N; ++i) a += b[i * NQF + q] * u[i];
+            sT[atp(f * NQF + q, s, el)] = a;
+        }
+    }
+    if (gl) {
+        const double* ph = T + P.off.phi_face + (sIn[gsl] & 3) * N * NQF;
+        double tn[NQF];
+#pragma unroll
+        for (int q = 0; q < NQF; ++q) tn[q] = 0.0;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+#pragma unroll
+            for (int q = 0; q < NQF; ++q) tn[q] += ph[i * NQF + q] * xn[i];
+        }
+#pragma unroll
+        for (int q = 0; q < NQF; ++q) sG[(q * 4 + gc) * XMAX + gk] = tn[q];
+    }
     wave_sync();
-    store_tile<L>(Uout + e0 * NS, sW, t, ne);
+    // ---- one Riemann solve per edge point
+#pragma unroll 1
+    for (int it = t; it < n_items * NQF; it += DG_BS) {
+        const int ui = it / NQF, q = it - ui * NQF;
+        const int A = sItem[ui], elA = A / NF, fA = A - elA * NF;
+        const int infA = sIn[A], sdA = (infA >> 2) & 1, pl = sPlan[A];
+        const int64_t k = infA >> 3;
+        const int qA = sdA ? NQF - 1 - q : q;
+        const double* eg = P.edge_geo + k * 3;
+        double fx[4], sp;
+        int B = -1, qB = 0;
+        if (sNb[A] < 0) {  // boundary: weighted flux precomputed by k_bflux
+            const double* bf = P.bflux + ((k - P.n_int) * NQF + q) * 5;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) fx[c] = bf[c];
+            sp = bf[4];
+        } else {
+            double a[4], b[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) a[c] = sT[atp(fA * NQF + qA, c, elA)];
+            if (pl >= 0) {  // partner in the patch; the owner is side 0
+                B = pl;
+                const int elB = B / NF, fB = B - elB * NF;
+                qB = NQF - 1 - q;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) b[c] = sT[atp(fB * NQF + qB, c, elB)];
+            } else {
+                const int kx = -2 - pl, qn = NQF - 1 - qA;
+                if (kx < XMAX) {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) b[c] = sG[(qn * 4 + c) * XMAX + kx];
+                } else {  // past the ghost rows: the same sums, from global memory
+                    const double* ur = Uin + static_cast<int64_t>(sNb[A]) * NS;
+                    const double* ph = T + P.off.phi_face + (infA & 3) * N * NQF + qn;
+#pragma unroll 1
+                    for (int c = 0; c < 4; ++c) {
+                        double a = 0.0;
+                        for (int i = 0; i < N; ++i) a += ph[i * NQF] * ur[i * 4 + c];
+                        b[c] = a;
+                    }
+                }
+            }
+            double ul[4], ur[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                ul[c] = sdA == 0 ? a[c] : b[c];
+                ur[c] = sdA == 0 ? b[c] : a[c];
+            }
+            sp = interface_flux<FLUX>(P, ul, ur, eg[0], eg[1], fx);
+            const double sc = eg[2] * T[P.off.w_face + q];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) fx[c] = fx[c] * sc;
+        }
+        const unsigned long long spb = __double_as_longlong(sp > 0.0 ? sp : 0.0);
+        double sg = sdA ? -1.0 : 1.0;
+        if (fold) sg = sdA ? -sRd[elA] : sRd[elA];  // = sg / det J exactly
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sT[atp(fA * NQF + qA, c, elA)] = sg * fx[c];
+        atomicMax(&sSp[A], spb);
+        double len = 0.0;  // the edge's length weight, for both sides' wave-speed sums
+        if (q == 0) {
+            const double jac = eg[2];
+            for (int qq = 0; qq < NQF; ++qq) len += jac * T[P.off.w_face + qq];
+            sLen[A] = len;
+        }
+        if (B >= 0) {
+            const int elB = B / NF, fB = B - elB * NF;
+            const double sgB = fold ? -sRd[elB] : -1.0;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) sT[atp(fB * NQF + qB, c, elB)] = sgB * fx[c];
+            atomicMax(&sSp[B], spb);
+            if (q == 0) sLen[B] = len;
+        }
+    }
+    wave_sync();
+    // ---- lift, face by face, and the wave-speed sum
+    double wave = 0.0;
+#pragma unroll 1
+    for (int f = 0; f < NF; ++f) {
+        double fw[NQF];
+#pragma unroll
+        for (int q = 0; q < NQF; ++q) fw[q] = sT[atp(f * NQF + q, s, el)];
+        const double* lb = T + lift_off + f * N * NQF;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+#pragma unroll
+            for (int q = 0; q < NQF; ++q) acc[i] += lb[i * NQF + q] * fw[q];
+        }
+        if (live) {
+            const int sl = el * NF + f;
+            wave += __longlong_as_double(static_cast<long long>(sSp[sl])) * sLen[sl];
+        }
+    }
+    wave_sync();
+
+    dg_volume<C, L>(P, T, sW, sGe, u, acc, t, cs, ne, fold,
+                    [&](int pe) -> int64_t { return sId[pe]; });
+    dg_epilogue<C, L, STAGE>(
+        P, T, sW, sIn, u, acc, wave, t, el, s, e, live, slot, Un, dt, dt_factor, ca, cb, mode,
+        partial, flags, counters, [&]() {
+            for (int it = t; it < ne * NS; it += DG_BS) {
+                const int r = it / NS, c = it - r * NS;
+                __builtin_nontemporal_store(sW[it], &Uout[static_cast<int64_t>(sId[r]) * NS + c]);
+            }
+        });
 }
 
 // ------------------------------------------------- boundary-face pre-pass
@@ -2152,9 +2510,30 @@ struct Launch {
         if (nbq > 0)
             hipLaunchKernelGGL((nz::k_bflux<C>), dim3(static_cast<unsigned>((nbq + nz::BS - 1) / nz::BS)),
                                dim3(nz::BS), 0, st, P, P.tables, Uin);
+        const bool stage = mode == nz::kStageFirst || mode == nz::kStageNext;
+        if constexpr (C::AFF) {
+            if (P.patches) {
+                if (P.n_patches <= 0) return;
+                const dim3 gp(static_cast<unsigned>(P.n_patches));
+#define NZ_DGP(FL, ST)                                                                             \
+    hipLaunchKernelGGL((nz::k_dgp<C, FL, ST>), gp, dim3(nz::DG_BS), 0, st, P, P.tables, Uin, Un,    \
+                       Uout, dt, fac, a, b, mode, partial, flags, counters)
+                if (P.flux_id == 1) {
+                    if (stage) NZ_DGP(1, true);
+                    else NZ_DGP(1, false);
+                } else if (P.flux_id == 2) {
+                    if (stage) NZ_DGP(2, true);
+                    else NZ_DGP(2, false);
+                } else {
+                    if (stage) NZ_DGP(0, true);
+                    else NZ_DGP(0, false);
+                }
+#undef NZ_DGP
+                return;
+            }
+        }
         if (nblk_dg <= 0) return;
         const dim3 g(static_cast<unsigned>(nblk_dg));
-        const bool stage = mode == nz::kStageFirst || mode == nz::kStageNext;
 #define NZ_DG(FL, ST)                                                                              \
     hipLaunchKernelGGL((nz::k_dg<C, FL, ST>), g, dim3(nz::DG_BS), 0, st, P, P.tables, Uin, Un, Uout, \
                        dt, fac, a, b, mode, partial, flags, counters)
@@ -2287,6 +2666,8 @@ int resolve(const nz_problem* p, Ops* ops) {
     if (p->vjp_tiles && (p->n_vjp_tiles < 0 || p->n_vjp_tiles > (int64_t{1} << 31)))
         return fail(NZ_E_ARG, "n_vjp_tiles out of range");
     if (p->vjp_kernel != 0 && p->vjp_kernel != 1) return fail(NZ_E_ARG, "vjp_kernel must be 0 or 1");
+    if (p->patches && (p->n_patches < 0 || p->n_patches > (int64_t{1} << 31)))
+        return fail(NZ_E_ARG, "n_patches out of range");
     return NZ_OK;
 }
 
@@ -2297,7 +2678,9 @@ int64_t vjp_te(const nz_problem* p, const Ops& o) { return p->vjp_kernel == 0 ? 
 int64_t n_blocks_vjp(const nz_problem* p, const Ops& o) {
     return p->vjp_tiles ? p->n_vjp_tiles : (p->n_elem + vjp_te(p, o) - 1) / vjp_te(p, o);
 }
+// norm partials of the element kernel: one per patch, tile-table entry or tile
 int64_t n_blocks_dg(const nz_problem* p) {
+    if (p->patches && p->affine) return p->n_patches;
     return p->tiles ? p->n_tiles : (p->n_elem + nz::TE - 1) / nz::TE;
 }
 

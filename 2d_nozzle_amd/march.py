@@ -243,6 +243,8 @@ def solve_steady(ops, flow, opts: SolverOptions | None = None, *, U0, backend: s
         raise ValueError(f"U0 has shape {U.shape}, expected {want}; change the order of the "
                          "state (initial.change_order) or rebuild the mesh to match")
     scale = residual_scale(flow, length)
+    if hasattr(bk, "tune_element_kernel"):
+        bk.tune_element_kernel(U)  # large affine meshes: tiles or patch tiles
     U, hist, conv, iters, res, res0, msg = march(bk.asarray(U), bk, opts, scale,
                                                  progress=progress)
     U_np = np.asarray(bk.to_numpy(U))

@@ -627,6 +627,7 @@ def main() -> int:
         bk = be.HipBackend(ops, cf.FlowConditions(flux=flux), cf.SolverOptions(limiter=args.limiter))
         if u0 is None:
             u0 = torch.from_numpy(U).to(bk.device)
+        bk.tune_element_kernel(u0)  # tiles or patch tiles, timed on this mesh
         out = torch.empty_like(u0)
         wave = torch.empty(E, dtype=torch.float64, device=bk.device)
 
@@ -635,7 +636,8 @@ def main() -> int:
                                    C.c_void_p(wave.data_ptr()), bk._stream()), "nz_rate")
 
         t, warm = _timed(rate, args.steps, args.warmup, st, min_warm_s=args.min_warm_s)
-        per_flux[flux] = {"s_per_eval": t / args.steps, "bk": bk, "warm": warm}
+        per_flux[flux] = {"s_per_eval": t / args.steps, "bk": bk, "warm": warm,
+                          "kernel": "k_dgp (patch tiles)" if bk.patches else "k_dg (tiles)"}
         print(f"  residual {flux}: {t / args.steps * 1e6:.1f} us (warm-up {warm})",
               file=sys.stderr, flush=True)
     t_eval = per_flux[args.flux]["s_per_eval"]
@@ -674,6 +676,7 @@ def main() -> int:
         "per_flux_gdof_stage_s": {f: round(E * N / v["s_per_eval"] / 1e9, 4)
                                   for f, v in per_flux.items()},
         "per_flux_us_per_eval": {f: round(v["s_per_eval"] * 1e6, 1) for f, v in per_flux.items()},
+        "element_kernel": {f: v["kernel"] for f, v in per_flux.items()},
     }
     skipped = []
 

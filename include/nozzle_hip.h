@@ -105,6 +105,18 @@ typedef struct {
      * kind, order, geometry order, flux) by timing both. */
     int32_t vjp_kernel;
     int32_t pad1;
+    /* optional patch table for the element kernel of nz_residual / nz_rate /
+     * nz_stage on affine (straight-sided triangle) meshes, used in place of
+     * `tiles` when set: one workgroup per patch, n_patches of them.  Patch k
+     * is the 18 int32 at patches[18k]: up to 16 element ids (the first
+     * `count` valid, the rest -1), count (1..16), norm slot (distinct,
+     * < n_patches).  Every element is in exactly one patch.  Patches of
+     * elements that share edges solve each shared edge's flux once; results
+     * are bit-identical to the tiled kernel's.  No counterpart in the
+     * reference (its edge pass solves every interior edge once by
+     * construction, src/backends/_numba_kernels.py:484-583). */
+    const int32_t *patches;
+    int64_t n_patches;
 } nz_problem;
 
 /* Number of scratch doubles nz_* calls need for norm partials. */
