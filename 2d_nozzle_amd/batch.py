@@ -420,10 +420,10 @@ class DesignGradient:
     converged: bool
 
 
-def _design_coords(dsg: Design, area_ratio: float) -> np.ndarray:
-    """Node coordinates of a design with the logical grid frozen at the design
-    (the node distributions of ``build_nozzle_mesh``, no connectivity)."""
-    from .mesh import nozzle_node_coords, nozzle_x_distribution
+def _design_grids(dsg: Design):
+    """The node fractions of a design's logical grid (the node distributions
+    of ``build_nozzle_mesh``, no connectivity) and its parameters."""
+    from .mesh import nozzle_x_distribution
 
     geom = dsg.geometry()
     q, sub = dsg.geometry_order, dsg.geometry_order * 2**dsg.refine
@@ -431,15 +431,21 @@ def _design_coords(dsg: Design, area_ratio: float) -> np.ndarray:
                                    spacing="throat", x_throat=geom.throat_location() / geom.length,
                                    cluster_strength=3.0, cluster_width=0.12)
     r_frac = np.linspace(0.0, 1.0, q * dsg.nr * 2**dsg.refine + 1)
-    p = dict(geom.params())
-    p["area_ratio"] = area_ratio
-    return nozzle_node_coords(x_frac, r_frac, p, geom.contour)
+    return x_frac, r_frac, dict(geom.params()), geom.contour
+
+
+def _design_tangent(dsg: Design, name: str = "area_ratio") -> np.ndarray:
+    """Exact ``d node coordinates / d name`` of a design (complex step)."""
+    from .mesh import node_coords_tangent
+
+    x_frac, r_frac, p, contour = _design_grids(dsg)
+    return node_coords_tangent(x_frac, r_frac, p, contour, name)
 
 
 def batched_gradients(results: list, flow: FlowConditions, opts: SolverOptions, *,
                       objective: str = "thrust", tolerance: float = 1e-8,
                       max_iterations: int = 100_000, check_interval: int = 100,
-                      fd_step: float = 1e-6, group_size: int | None = None,
+                      group_size: int | None = None,
                       device=None) -> list:
     """d objective / d area ratio for every converged design of a batch, from
     steady adjoints marched together (``src/sensitivity.py:289-460``).
@@ -462,7 +468,16 @@ def batched_gradients(results: list, flow: FlowConditions, opts: SolverOptions, 
 
     from .backend import HipBackend
     from .config import step_coefficient
-    from .sensitivity import edge_vjp, metric_vjp, objective_and_partials
+    from .sensitivity import _Throat, edge_vjp, metric_vjp
+
+    def design_objective(g):
+        # the device objective and its partials (nz_objective) on the
+        # design's own problem, as its lone adjoint (DifferentiableCase) takes
+        # them: the same kernel, the same summation order
+        r = results[g]
+        J, dU, dG = HipBackend(r.operators, flow, opts, device=device).objective(
+            r.U, objective, geom=_Throat(r.design.geometry().throat_half_height))
+        return J, dU.cpu().numpy(), dG.cpu().numpy()
 
     out: list = [None] * len(results)
     groups: dict = {}
@@ -506,8 +521,7 @@ def batched_gradients(results: list, flow: FlowConditions, opts: SolverOptions, 
             bk.set_patches(False)
         f.stream = torch.cuda.Stream(device=bk.device)
         dev = bk.device
-        f.parts = [objective_and_partials(results[g].U, results[g].operators, flow, objective)
-                   for g in idx]
+        f.parts = [design_objective(g) for g in idx]
         f.eoff = eoff = f.cat.elem_offset
         f.u = torch.from_numpy(np.concatenate([results[g].U for g in idx])).to(dev)
         f.s = torch.from_numpy(np.concatenate([p[1] for p in f.parts])).to(dev)
@@ -624,9 +638,7 @@ def batched_gradients(results: list, flow: FlowConditions, opts: SolverOptions, 
             J, _, J_edge = f.parts[j]
             gX = edge_vjp(ops, J_edge) - metric_vjp(ops, gm[f.sl[j]]) - edge_vjp(ops, ge_d)
             dsg = results[g].design
-            h = fd_step * max(1.0, abs(dsg.area_ratio))
-            dX = (_design_coords(dsg, dsg.area_ratio + h)
-                  - _design_coords(dsg, dsg.area_ratio - h)) / (2 * h)
+            dX = _design_tangent(dsg)
             conv = bool(np.isfinite(f.rels[j]) and f.rels[j] <= tolerance)
             out[g] = DesignGradient(design=dsg, objective=J,
                                     gradient={"area_ratio": float((gX * dX).sum())},

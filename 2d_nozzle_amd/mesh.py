@@ -304,6 +304,17 @@ def _bernstein9(s, ctrl):
     return out
 
 
+def _clip01(v):
+    """``numpy.clip(v, 0, 1)`` with the bounds decided on the real part, so a
+    complex-step tangent passes through the branch the value takes."""
+    r = np.real(v)
+    return np.where(r < 0.0, 0.0, np.where(r > 1.0, 1.0, v))
+
+
+def _tan(v):
+    return np.tan(v) if isinstance(v, complex) else math.tan(v)
+
+
 def _hermite(s, y0, y1, m0, m1):
     s2 = s * s
     s3 = s2 * s
@@ -389,10 +400,15 @@ class NozzleGeometry:
 
 
 def wall_half_height(x, params: Mapping[str, float], contour: str = "bell"):
-    """``y_wall(x)`` for a contour family (``contour.py:264-311``)."""
+    """``y_wall(x)`` for a contour family (``contour.py:264-311``).
+
+    Complex ``x`` or parameters are carried through (branches follow the
+    real part), which gives exact complex-step tangents of the wall."""
     if contour not in CONTOURS:
         raise ValueError(f"unknown contour {contour!r}; choose from {CONTOURS}")
-    x = np.asarray(x, dtype=float)
+    x = np.asarray(x)
+    if not np.iscomplexobj(x):
+        x = x.astype(float)
     length = params.get("length", 1.0)
     xi = x / length
     if contour == "analytic":
@@ -401,15 +417,15 @@ def wall_half_height(x, params: Mapping[str, float], contour: str = "bell"):
     y_in = params["inlet_half_height"]
     y_ex = y_th * params["area_ratio"]
     x_th = params["throat_x"]
-    y_conv = _hermite(np.clip(xi / x_th, 0.0, 1.0), y_in, y_th, 0.0, 0.0)
+    y_conv = _hermite(_clip01(xi / x_th), y_in, y_th, 0.0, 0.0)
     span = 1.0 - x_th
-    s = np.clip((xi - x_th) / span, 0.0, 1.0)
+    s = _clip01((xi - x_th) / span)
     dx_ds = span * length
     if contour == "conical":
         y_div = y_th + (y_ex - y_th) * s
     elif contour in ("bell", "smooth", "moc"):
-        y_div = _hermite(s, y_th, y_ex, math.tan(params["theta_initial"]) * dx_ds,
-                         math.tan(params["theta_exit"]) * dx_ds)
+        y_div = _hermite(s, y_th, y_ex, _tan(params["theta_initial"]) * dx_ds,
+                         _tan(params["theta_exit"]) * dx_ds)
     elif contour == "bezier9":
         ctrl = [y_th] + [params[f"bz{i}"] for i in range(1, BEZIER9_DEGREE)] + [y_ex]
         y_div = _bernstein9(s, ctrl)
@@ -418,17 +434,30 @@ def wall_half_height(x, params: Mapping[str, float], contour: str = "bell"):
         c2 = y_th + params["bezier_w2"] * (y_ex - y_th)
         u = 1.0 - s
         y_div = u**3 * y_th + 3.0 * u**2 * s * c1 + 3.0 * u * s**2 * c2 + s**3 * y_ex
-    return np.where(xi <= x_th, y_conv, y_div)
+    return np.where(np.real(xi) <= np.real(x_th), y_conv, y_div)
 
 
 def nozzle_node_coords(x_frac, r_frac, params, contour) -> np.ndarray:
-    """Nodes ``(x_i, r_j * y_wall(x_i))`` numbered ``j * n_ix + i``."""
+    """Nodes ``(x_i, r_j * y_wall(x_i))`` numbered ``j * n_ix + i`` (complex
+    parameters carried through, see :func:`node_coords_tangent`)."""
     x = np.asarray(x_frac, dtype=float) * params.get("length", 1.0)
     yw = wall_half_height(x, params, contour)
     r = np.asarray(r_frac, dtype=float)
     xx = np.broadcast_to(x[None, :], (r.size, x.size))
     yy = r[:, None] * yw[None, :]
     return np.stack([xx.reshape(-1), yy.reshape(-1)], axis=-1)
+
+
+def node_coords_tangent(x_frac, r_frac, params, contour, name: str) -> np.ndarray:
+    """``d nozzle_node_coords / d params[name]`` at frozen node fractions,
+    exact to rounding: the complex-step derivative ``Im f(p + i h) / h``
+    (h = 1e-30) of the closed-form node map, which has no subtractive
+    cancellation.  Clipped wall parameters differentiate through the branch
+    each node takes, as the reference's automatic differentiation does."""
+    h = 1e-30
+    p = dict(params)
+    p[name] = complex(float(p[name]), h)
+    return np.imag(nozzle_node_coords(x_frac, r_frac, p, contour)) / h
 
 
 def build_nozzle_mesh(geom: NozzleGeometry, *, kind="tri", nx=14, nr=5, refine=0,

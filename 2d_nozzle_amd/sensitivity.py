@@ -13,8 +13,9 @@ around the hand-written residual VJP:
    the metrics, edge normals/Jacobians and back pressure; they are chained to
    node coordinates here (reverse of ``build_operators``,
    ``src/spatial/operators.py:244-305``) and to design variables through the
-   node map ``nozzle_node_coords`` by central differences of that closed-form
-   map (the flow is not re-solved).
+   exact tangent of the closed-form node map ``nozzle_node_coords``
+   (``mesh.node_coords_tangent``, a complex-step derivative: no truncation
+   error, no cancellation).
 """
 
 from __future__ import annotations
@@ -26,7 +27,8 @@ from typing import Any
 import numpy as np
 
 from .config import FlowConditions, SolverOptions
-from .mesh import (BoundaryTag, NozzleGeometry, build_nozzle_mesh, nozzle_node_coords)
+from .mesh import (BoundaryTag, NozzleGeometry, build_nozzle_mesh, node_coords_tangent,
+                   nozzle_node_coords)
 from .operators import build_operators
 
 DESIGN_PARAMETERS = ("area_ratio", "throat_x", "throat_half_height", "inlet_half_height",
@@ -339,7 +341,7 @@ class DifferentiableCase:
 
     def value_and_gradient(self, objective: str = "thrust", params=None, *,
                            names: Sequence[str] | None = None, U0, adjoint_tolerance: float = 1e-8,
-                           fd_step: float = 1e-6, verbose: bool = False,
+                           verbose: bool = False,
                            adjoint_method: str = "rk4"):
         """``(J, {name: dJ/dname}, forward_result, adjoint_info)``.
 
@@ -374,8 +376,7 @@ class DifferentiableCase:
             if k == "back_pressure":
                 grad[k] = -gpb
                 continue
-            h = fd_step * max(1.0, abs(base[k]))
-            dX = (self.coords({**base, k: base[k] + h}) - self.coords({**base, k: base[k] - h})) / (2 * h)
+            dX = node_coords_tangent(self.x_frac, self.r_frac, base, self.geometry.contour, k)
             grad[k] = float((gX * dX).sum())
             if k == "throat_half_height" and objective == "thrust_coefficient":
                 grad[k] -= J / base[k]  # the coefficient's scale 1 / (p_t 2 y_throat)

@@ -103,3 +103,64 @@ def test_autograd_residual_rejects_host_inputs():
         sens.residual(_Stub(), torch.zeros((4, 3, 4), dtype=torch.float64))
     with pytest.raises(ValueError, match="float64 tensor"):
         sens.residual(_Stub(), torch.zeros((4, 3, 4), dtype=torch.float32))
+
+
+# ------------------------------------------------------------ node-map tangent
+def _grids(contour="bell", **gk):
+    mesh = nz_module("mesh")
+    geom = mesh.NozzleGeometry(contour=contour, **gk)
+    _, coords, grids = mesh.build_nozzle_mesh(geom, refine=1)
+    return mesh, geom, coords, grids["x_frac"], grids["r_frac"]
+
+
+def test_node_map_real_path_unchanged_by_complex_support():
+    """The complex-capable wall gives the same doubles as the mesh builder."""
+    for contour in ("bell", "smooth", "conical", "moc", "bezier", "bezier9", "analytic"):
+        mesh, geom, coords, xf, rf = _grids(contour)
+        X = mesh.nozzle_node_coords(xf, rf, geom.params(), contour)
+        assert X.dtype == np.float64 and np.array_equal(X, coords), contour
+
+
+@pytest.mark.parametrize("contour", ["conical", "smooth", "bell"])
+def test_node_map_tangent_closed_form_area_ratio(contour):
+    """d y / d area_ratio = r y_th h(s): the diverging wall is linear in its
+    exit height y_th * area_ratio, with h(s) = s (conical) or the Hermite
+    basis -2 s^3 + 3 s^2 (smooth, bell); the converging wall does not move."""
+    mesh, geom, _, xf, rf = _grids(contour, area_ratio=2.5)
+    p = geom.params()
+    d = mesh.node_coords_tangent(xf, rf, p, contour, "area_ratio")
+    x_th, y_th = p["throat_x"], p["throat_half_height"]
+    s = np.clip((xf - x_th) / (1.0 - x_th), 0.0, 1.0)
+    h = s if contour == "conical" else -2.0 * s**3 + 3.0 * s**2
+    h = np.where(xf <= x_th, 0.0, h)
+    want = (rf[:, None] * (y_th * h)[None, :]).reshape(-1)
+    assert np.all(d[:, 0] == 0.0)
+    np.testing.assert_allclose(d[:, 1], want, rtol=1e-14, atol=1e-16)
+
+
+@pytest.mark.parametrize("contour,name", [("bell", "throat_x"), ("bell", "theta_initial"),
+                                          ("bezier", "bezier_w1"), ("bezier9", "bz3"),
+                                          ("smooth", "throat_half_height"),
+                                          ("analytic", "length"), ("moc", "inlet_half_height")])
+def test_node_map_tangent_against_five_point_differences(contour, name):
+    """Against the fourth-order central difference at h = 1e-4 (its error
+    falls as h^4 towards the tangent: 7e-5, 1e-8, 2e-12 at h = 1e-2, 1e-3,
+    1e-4 for the throat position) away from the clip kinks."""
+    mesh, geom, _, xf, rf = _grids(contour)
+    p = geom.params()
+    d = mesh.node_coords_tangent(xf, rf, p, contour, name)
+    h = 1e-4 * max(1.0, abs(p[name]))
+
+    def at(v):
+        return mesh.nozzle_node_coords(xf, rf, {**p, name: v}, contour)
+
+    fd = (-at(p[name] + 2 * h) + 8 * at(p[name] + h) - 8 * at(p[name] - h)
+          + at(p[name] - 2 * h)) / (12 * h)
+    # nodes whose branch (converging / diverging wall) flips within 2h are
+    # not differentiable there: compare the rest
+    x = np.broadcast_to(xf[None, :] * p.get("length", 1.0), (rf.size, xf.size)).reshape(-1)
+    near = np.abs(x - p["throat_x"] * p.get("length", 1.0)) < 3 * h if contour != "analytic" \
+        else np.zeros_like(x, dtype=bool)
+    scale = np.abs(fd[~near]).max()
+    assert scale > 0
+    assert np.abs(d[~near] - fd[~near]).max() <= 1e-8 * scale
