@@ -919,9 +919,10 @@ void k_dg(const nz_problem P, const double* __restrict__ T,
 
 // ------------------------------------------------------ patch tiles
 // k_dgp: k_dg on patch tiles, for affine triangles.  A patch is 16 elements
-// that are close in space (consecutive along a Morton curve of the element
-// centroids), listed in the patch table by element id; about two thirds of
-// its 48 element faces are shared by two of its own elements.  Each interior
+// that are close in space (a block of 4 x 2 cells of a structured grid, or a
+// run of the Morton curve of the element centroids), listed in the patch
+// table by element id; 36 of the 48 element faces of a 4 x 2 cell block are
+// shared by two of its own elements.  Each interior
 // edge's Riemann flux is then solved once, by one (edge, point) item, and
 // written into both elements' faces; only faces whose neighbour lies outside
 // the patch gather a neighbour column.
@@ -946,31 +947,29 @@ void k_dg(const nz_problem P, const double* __restrict__ T,
 // k_dg's order, so the rate and the stage are bit-identical to k_dg's.
 constexpr int PATCH_INTS = 18;  // per patch: 16 element ids (-1 padded), count, norm slot
 constexpr int XMAX = 12;        // off-tile faces with LDS ghost rows (a 4 x 2 cell block has 12)
+constexpr int TRS = 66;         // doubles per face-point row of traces: 16 elements x 4, + 2
 
-// LDS slot of component c of patch element el in face-point row r (rows
-// (face, point), 64 doubles each, packed).  The component blocks rotate
-// with the row and the upper two are XOR-swizzled as in at(): the lanes of
-// one (element, component) per lane pattern touch 32 distinct banks per
-// half-wave, and the four points of one flux item fall in four banks.
-__device__ __forceinline__ int atp(int r, int c, int el) {
-    const int cc = (c + r) & 3;
-    return r * 4 * TE + cc * TE + (el ^ ((cc >> 1) << 3));
-}
+// LDS offset of the four components of patch element el in face-point row r
+// (rows (face, point)).  Components are contiguous, so a flux item reads an
+// element's state with one address; the row stride (66 = 2 mod 32 doubles)
+// moves the four points of one item onto distinct banks, and one row is
+// written and read by 64 lanes on 64 consecutive doubles.
+__device__ __forceinline__ int atq(int r, int el) { return r * TRS + el * 4; }
 
 template <class C>
 struct PTile {
     static constexpr int N = C::N, NQF = C::NQF, NF = C::NF, NS = C::NS;
     static constexpr int QR = DG_BS / TE;
-    static constexpr int SL = TE * NF;        // face slots of a patch
-    static constexpr int TR = NF * NQF * 4 * TE;  // traces, then weighted fluxes: rows (f, q)
-    static constexpr int GH = XMAX * NQF * 4; // off-tile neighbour traces
+    static constexpr int SL = TE * NF;           // face slots of a patch
+    static constexpr int TR = NF * NQF * TRS;    // traces, then weighted fluxes: rows (f, q)
+    static constexpr int GH = XMAX * NQF * 4;    // off-tile neighbour traces [k][q][c]
     static constexpr int FACE = TR + GH;
     static constexpr int VOL = 2 * QR * QS;
     static constexpr int SIO = NS;
     static constexpr int IO = TE * SIO;
     static constexpr int WORK = FACE > VOL ? (FACE > IO ? FACE : IO) : (VOL > IO ? VOL : IO);
     static constexpr int GEN = TE * 5;
-    static constexpr int LDS = 8 * (WORK + GEN + TE + 2 * SL) + 4 * (4 * SL + XMAX + TE);
+    static constexpr int LDS = 8 * (WORK + GEN + TE + 2 * SL) + 4 * (2 * SL + TE + XMAX) + 2 * SL;
     // waves per SIMD that the LDS allows, at most four (128 VGPRs)
     static constexpr int WPE = LDS * 16 <= 160 * 1024 ? 4 : (LDS * 12 <= 160 * 1024 ? 3 : 2);
 };
@@ -985,12 +984,15 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
     constexpr int N = C::N, NQF = C::NQF, NF = C::NF, NS = C::NS, SL = L::SL;
     static_assert(C::AFF && C::STR, "patch tiles serve affine elements");
     static_assert(L::SL <= DG_BS, "one lane per face slot");
-    __shared__ double sW[L::WORK];
+    static_assert(4 * XMAX <= DG_BS, "one ghost item per lane");
+    __shared__ __align__(16) double sW[L::WORK];
     __shared__ double sGe[L::GEN];
     __shared__ double sRd[TE];               // 1 / det J
-    __shared__ unsigned long long sSp[SL];  // per face slot: max signal speed (bits)
-    __shared__ double sLen[SL];             // per face slot: |dX/dsigma| sum_q w_q
-    __shared__ int sNb[SL], sIn[SL], sPlan[SL], sItem[SL], sExt[XMAX], sId[TE];
+    __shared__ unsigned long long sSp[SL];   // per face slot: max signal speed (bits)
+    __shared__ double sLen[SL];              // per face slot: |dX/dsigma| sum_q w_q
+    __shared__ int sNb[SL], sIn[SL], sExt[XMAX], sId[TE];
+    __shared__ int8_t sPlan[SL];             // partner slot, -1 boundary, -2 - k off-tile face k
+    __shared__ uint8_t sItem[SL];            // flux-owning slots, in slot order
 
     const int t = threadIdx.x;
     const int el = t >> 2, s = t & 3;
@@ -999,57 +1001,55 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
     const int64_t slot = pt[TE + 1];
     const bool live = el < ne;
     const int myid = pt[el];
-    const int64_t e = live ? myid : 0;
+    const int64_t e = live ? myid : 0;  // idle lanes read element 0 and discard it
     const bool fold = STAGE || mode != kRes;
     const int cs = at(0, s, el);  // the volume rows keep k_dg's layout
 
+    // ---- this lane's column of its element, the face maps (one lane per
+    // face slot) and the affine metrics, all requested before any is used
     __builtin_amdgcn_s_setprio(2);
-    if (s == 0) sId[el] = myid;
-    wave_sync();
+    double u[N], acc[N];
+    double g_nx, g_ny, g_jac;
     {
-        constexpr int NU = TE * NS / DG_BS;  // = N
-        constexpr int NM = (SL + DG_BS - 1) / DG_BS;
         constexpr int NG = (L::GEN + DG_BS - 1) / DG_BS;
-        double ru[NU], rg[NG];
-        int rn[NM], ri[NM];
+        const double* ur = Uin + e * NS + s;
 #pragma unroll
-        for (int j = 0; j < NU; ++j) {
-            const int it = t + j * DG_BS, r = it / NS, c = it - r * NS;
-            ru[j] = r < ne ? Uin[static_cast<int64_t>(sId[r]) * NS + c] : 0.0;
+        for (int i = 0; i < N; ++i) u[i] = ur[i * 4];
+        int rn = -1, ri = 0;
+        if (t < SL) {
+            const int r = t / NF;
+            if (r < ne) {
+                const int64_t o = static_cast<int64_t>(pt[r]) * NF + (t - r * NF);
+                rn = P.fe_nbr[o];
+                ri = P.fe_info[o];
+            }
         }
-#pragma unroll
-        for (int j = 0; j < NM; ++j) {
-            const int it = t + j * DG_BS, r = it / NF, f = it - r * NF;
-            const bool ok = it < SL && r < ne;
-            rn[j] = ok ? P.fe_nbr[static_cast<int64_t>(sId[r]) * NF + f] : -1;
-            ri[j] = ok ? P.fe_info[static_cast<int64_t>(sId[r]) * NF + f] : 0;
+        // this slot's edge geometry (nx, ny, |dX/dsigma|), kept in the slot
+        // lane's registers for the flux items to read across lanes
+        {
+            const double* eg = P.edge_geo + static_cast<int64_t>(ri >> 3) * 3;
+            g_nx = eg[0];
+            g_ny = eg[1];
+            g_jac = eg[2];
         }
+        double rg[NG];
 #pragma unroll
         for (int j = 0; j < NG; ++j) {
-            const int it = t + j * DG_BS, r = it / 5, c = it - r * 5;
-            rg[j] = it < L::GEN && r < ne ? P.geo[static_cast<int64_t>(sId[r]) * 5 + c] : 0.0;
+            const int it = t + j * DG_BS, r = it / 5;
+            rg[j] = it < L::GEN && r < ne ? P.geo[static_cast<int64_t>(pt[r]) * 5 + (it - 5 * r)] : 0.0;
         }
-#pragma unroll
-        for (int j = 0; j < NU; ++j) sW[t + j * DG_BS] = ru[j];
-#pragma unroll
-        for (int j = 0; j < NM; ++j) {
-            const int it = t + j * DG_BS;
-            if (it < SL) {
-                sNb[it] = rn[j];
-                sIn[it] = ri[j];
-            }
+        if (s == 0) sId[el] = live ? myid : -1;
+        if (t < SL) {
+            sNb[t] = rn;
+            sIn[t] = ri;
         }
 #pragma unroll
         for (int j = 0; j < NG; ++j)
             if (t + j * DG_BS < L::GEN) sGe[t + j * DG_BS] = rg[j];
     }
-    wave_sync();
-    double u[N], acc[N];
 #pragma unroll
-    for (int i = 0; i < N; ++i) {
-        u[i] = sW[el * L::SIO + i * 4 + s];
-        acc[i] = 0.0;
-    }
+    for (int i = 0; i < N; ++i) acc[i] = 0.0;
+    wave_sync();
     __builtin_amdgcn_s_setprio(0);
     if (t < TE) sRd[t] = 1.0 / sGe[t * 5 + 4];
 
@@ -1063,7 +1063,7 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
         } else {
             int j = -1;
 #pragma unroll
-            for (int k = 0; k < TE; ++k) j = sId[k] == nb && k < ne ? k : j;
+            for (int k = 0; k < TE; ++k) j = sId[k] == nb ? k : j;
             if (j >= 0) {
                 plan = j * NF + (sIn[t] & 3);
                 owner = ((sIn[t] >> 2) & 1) == 0;
@@ -1076,28 +1076,27 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
     const int n_items = __popcll(m_own), n_off = __popcll(m_off);
     {
         const unsigned long long below = (1ull << t) - 1ull;
-        if (owner) sItem[__popcll(m_own & below)] = t;
+        if (owner) sItem[__popcll(m_own & below)] = static_cast<uint8_t>(t);
         if (offt) {
             const int k = __popcll(m_off & below);
             if (k < XMAX) sExt[k] = t;
             plan = -2 - k;
         }
         if (t < SL) {
-            sPlan[t] = plan;
+            sPlan[t] = static_cast<int8_t>(plan);
             sSp[t] = 0ull;
         }
     }
-    wave_sync();  // the staging rows are reused below
+    wave_sync();
 
     double* sT = sW;             // traces, then weighted fluxes
-    double* sG = sW + L::TR;     // off-tile neighbour traces [q][c][k]
+    double* sG = sW + L::TR;     // off-tile neighbour traces [k][q][c]
     const int lift_off = fold ? P.off.lift : P.off.phi_face;
 
     // ---- off-tile neighbour traces: one (off-tile face, component) item per
     // lane for the first XMAX off-tile faces (the flux items of any further
     // ones form their neighbour's trace themselves).  The neighbour columns
     // are requested first; the own traces are formed while they arrive.
-    static_assert(4 * XMAX <= DG_BS, "one ghost item per lane");
     const int n_gh = n_off < XMAX ? n_off : XMAX;
     const bool gl = t < 4 * n_gh;
     const int gk = t >> 2, gc = t & 3, gsl = gl ? sExt[gk] : 0;
@@ -1116,7 +1115,7 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
             double a = 0.0;
 #pragma unroll
             for (int i = 0; i < N; ++i) a += b[i * NQF + q] * u[i];
-            sT[atp(f * NQF + q, s, el)] = a;
+            sT[atq(f * NQF + q, el) + s] = a;
         }
     }
     if (gl) {
@@ -1130,20 +1129,29 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
             for (int q = 0; q < NQF; ++q) tn[q] += ph[i * NQF + q] * xn[i];
         }
 #pragma unroll
-        for (int q = 0; q < NQF; ++q) sG[(q * 4 + gc) * XMAX + gk] = tn[q];
+        for (int q = 0; q < NQF; ++q) sG[(gk * NQF + q) * 4 + gc] = tn[q];
     }
     wave_sync();
-    // ---- one Riemann solve per edge point
+    // ---- one Riemann solve per edge point.  The rounds are wave-uniform so
+    // that every lane takes part in the cross-lane reads of the edge
+    // geometry (a lane outside the execution mask would return zero).
+    const int n_it = n_items * NQF;
 #pragma unroll 1
-    for (int it = t; it < n_items * NQF; it += DG_BS) {
-        const int ui = it / NQF, q = it - ui * NQF;
-        const int A = sItem[ui], elA = A / NF, fA = A - elA * NF;
+    for (int base = 0; base < n_it; base += DG_BS) {
+        const int it = base + t;
+        const bool act = it < n_it;
+        const int ui = act ? it / NQF : 0, q = act ? it - ui * NQF : 0;
+        const int A = sItem[ui];
+        const double e_nx = __shfl(g_nx, A, 64), e_ny = __shfl(g_ny, A, 64);
+        const double e_jac = __shfl(g_jac, A, 64);
+        if (!act) continue;
+        const int elA = A / NF, fA = A - elA * NF;
         const int infA = sIn[A], sdA = (infA >> 2) & 1, pl = sPlan[A];
         const int64_t k = infA >> 3;
         const int qA = sdA ? NQF - 1 - q : q;
-        const double* eg = P.edge_geo + k * 3;
+        double* pa = sT + atq(fA * NQF + qA, elA);
+        double* pb = nullptr;
         double fx[4], sp;
-        int B = -1, qB = 0;
         if (sNb[A] < 0) {  // boundary: weighted flux precomputed by k_bflux
             const double* bf = P.bflux + ((k - P.n_int) * NQF + q) * 5;
 #pragma unroll
@@ -1152,26 +1160,26 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
         } else {
             double a[4], b[4];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) a[c] = sT[atp(fA * NQF + qA, c, elA)];
+            for (int c = 0; c < 4; ++c) a[c] = pa[c];
             if (pl >= 0) {  // partner in the patch; the owner is side 0
-                B = pl;
-                const int elB = B / NF, fB = B - elB * NF;
-                qB = NQF - 1 - q;
+                const int elB = pl / NF, fB = pl - elB * NF;
+                pb = sT + atq(fB * NQF + (NQF - 1 - q), elB);
 #pragma unroll
-                for (int c = 0; c < 4; ++c) b[c] = sT[atp(fB * NQF + qB, c, elB)];
+                for (int c = 0; c < 4; ++c) b[c] = pb[c];
             } else {
                 const int kx = -2 - pl, qn = NQF - 1 - qA;
                 if (kx < XMAX) {
+                    const double* pg = sG + (kx * NQF + qn) * 4;
 #pragma unroll
-                    for (int c = 0; c < 4; ++c) b[c] = sG[(qn * 4 + c) * XMAX + kx];
+                    for (int c = 0; c < 4; ++c) b[c] = pg[c];
                 } else {  // past the ghost rows: the same sums, from global memory
                     const double* ur = Uin + static_cast<int64_t>(sNb[A]) * NS;
                     const double* ph = T + P.off.phi_face + (infA & 3) * N * NQF + qn;
 #pragma unroll 1
                     for (int c = 0; c < 4; ++c) {
-                        double a = 0.0;
-                        for (int i = 0; i < N; ++i) a += ph[i * NQF] * ur[i * 4 + c];
-                        b[c] = a;
+                        double v = 0.0;
+                        for (int i = 0; i < N; ++i) v += ph[i * NQF] * ur[i * 4 + c];
+                        b[c] = v;
                     }
                 }
             }
@@ -1181,8 +1189,8 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
                 ul[c] = sdA == 0 ? a[c] : b[c];
                 ur[c] = sdA == 0 ? b[c] : a[c];
             }
-            sp = interface_flux<FLUX>(P, ul, ur, eg[0], eg[1], fx);
-            const double sc = eg[2] * T[P.off.w_face + q];
+            sp = interface_flux<FLUX>(P, ul, ur, e_nx, e_ny, fx);
+            const double sc = e_jac * T[P.off.w_face + q];
 #pragma unroll
             for (int c = 0; c < 4; ++c) fx[c] = fx[c] * sc;
         }
@@ -1190,21 +1198,20 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
         double sg = sdA ? -1.0 : 1.0;
         if (fold) sg = sdA ? -sRd[elA] : sRd[elA];  // = sg / det J exactly
 #pragma unroll
-        for (int c = 0; c < 4; ++c) sT[atp(fA * NQF + qA, c, elA)] = sg * fx[c];
+        for (int c = 0; c < 4; ++c) pa[c] = sg * fx[c];
         atomicMax(&sSp[A], spb);
         double len = 0.0;  // the edge's length weight, for both sides' wave-speed sums
         if (q == 0) {
-            const double jac = eg[2];
-            for (int qq = 0; qq < NQF; ++qq) len += jac * T[P.off.w_face + qq];
+            for (int qq = 0; qq < NQF; ++qq) len += e_jac * T[P.off.w_face + qq];
             sLen[A] = len;
         }
-        if (B >= 0) {
-            const int elB = B / NF, fB = B - elB * NF;
+        if (pb) {
+            const int elB = pl / NF;
             const double sgB = fold ? -sRd[elB] : -1.0;
 #pragma unroll
-            for (int c = 0; c < 4; ++c) sT[atp(fB * NQF + qB, c, elB)] = sgB * fx[c];
-            atomicMax(&sSp[B], spb);
-            if (q == 0) sLen[B] = len;
+            for (int c = 0; c < 4; ++c) pb[c] = sgB * fx[c];
+            atomicMax(&sSp[pl], spb);
+            if (q == 0) sLen[pl] = len;
         }
     }
     wave_sync();
@@ -1214,7 +1221,7 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
     for (int f = 0; f < NF; ++f) {
         double fw[NQF];
 #pragma unroll
-        for (int q = 0; q < NQF; ++q) fw[q] = sT[atp(f * NQF + q, s, el)];
+        for (int q = 0; q < NQF; ++q) fw[q] = sT[atq(f * NQF + q, el) + s];
         const double* lb = T + lift_off + f * N * NQF;
 #pragma unroll
         for (int i = 0; i < N; ++i) {
@@ -1233,9 +1240,11 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
     dg_epilogue<C, L, STAGE>(
         P, T, sW, sIn, u, acc, wave, t, el, s, e, live, slot, Un, dt, dt_factor, ca, cb, mode,
         partial, flags, counters, [&]() {
-            for (int it = t; it < ne * NS; it += DG_BS) {
-                const int r = it / NS, c = it - r * NS;
-                __builtin_nontemporal_store(sW[it], &Uout[static_cast<int64_t>(sId[r]) * NS + c]);
+            if (live) {  // each lane its own column, as it was loaded
+                double* o = Uout + e * NS + s;
+#pragma unroll
+                for (int i = 0; i < N; ++i)
+                    __builtin_nontemporal_store(sW[el * L::SIO + i * 4 + s], o + i * 4);
             }
         });
 }
