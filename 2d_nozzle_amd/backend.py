@@ -176,6 +176,31 @@ def patch_table(order: np.ndarray, n_domains: int = XCDS) -> np.ndarray:
     return np.concatenate([ids[launch], cnt[launch, None], launch[:, None]], axis=1).astype(np.int32)
 
 
+def _least_squares_hessenberg(H: np.ndarray, beta: float) -> tuple[np.ndarray, float]:
+    """``min_y || beta e1 - H y ||`` for an upper Hessenberg ``H`` of shape
+    ``(k + 1, k)`` by Givens rotations; returns ``(y, residual norm)``.  A
+    zero subdiagonal entry (an exact Krylov space) ends the columns there."""
+    k = H.shape[1]
+    sub = np.flatnonzero(np.diag(H, -1) == 0.0)
+    if sub.size:
+        k = int(sub[0]) + 1
+    R = H[:k + 1, :k].copy()
+    g = np.zeros(k + 1)
+    g[0] = beta
+    for j in range(k):
+        a, b = R[j, j], R[j + 1, j]
+        den = math.hypot(a, b)
+        c, sn = (1.0, 0.0) if den == 0.0 else (a / den, b / den)
+        rj, rj1 = R[j, j:].copy(), R[j + 1, j:].copy()
+        R[j, j:] = c * rj + sn * rj1
+        R[j + 1, j:] = -sn * rj + c * rj1
+        g[j], g[j + 1] = c * g[j], -sn * g[j]
+    y = np.zeros(k)
+    for i in range(k - 1, -1, -1):  # back substitution
+        y[i] = (g[i] - R[i, i + 1:k] @ y[i + 1:k]) / R[i, i]
+    return y, abs(g[k])
+
+
 class HipBackend:
     """MI355X execution of the DG residual and the pseudo-time march."""
 
@@ -871,7 +896,8 @@ class HipBackend:
         (``restart`` vectors), right-preconditioned with that same local step
         times the inverse mass matrix, so ``D = dt M^-1`` is the operator the
         march integrates; GMRES minimises the true residual, whose norm is
-        tested after every VJP and recomputed at each restart.
+        estimated every 8 VJPs (the Arnoldi process stays on the device in
+        between) and recomputed at each restart.
         ``iterations`` counts VJPs.  Same fixed point, same test: the
         gradients agree with the march's to the tolerance.
 
@@ -928,10 +954,14 @@ class HipBackend:
         a["lam"] = [lam, tmp, nxt]
         return self._out(lam.clone(), npy), res / norm0, it, converged
 
-    def _adjoint_gmres(self, u, s, a, tolerance, max_iterations, restart, verbose):
+    def _adjoint_gmres(self, u, s, a, tolerance, max_iterations, restart, verbose, check=8):
         """Restarted GMRES on ``(dR/dU)^T D y = seed``, ``lam = D y``, with
         ``D = dt M^-1`` (the frozen local step), classical Gram-Schmidt with
-        one reorthogonalisation and Givens rotations on the host."""
+        one reorthogonalisation.  The Arnoldi process runs on the device: the
+        Hessenberg matrix accumulates in device memory and the host reads it
+        once every ``check`` VJPs (and at the end of a cycle) to solve the
+        small least-squares problem by Givens rotations, so the VJPs between
+        reads are queued without waiting for the host."""
         torch = self._torch
         st = self._stream()
         work, pu = self._p(a["work"]), self._p(u)
@@ -955,6 +985,7 @@ class HipBackend:
         if bnorm == 0.0:
             return lam, 0.0, 0, True
         V = torch.empty((m + 1, s.numel()), dtype=torch.float64, device=s.device)
+        H = torch.zeros((m + 1, m), dtype=torch.float64, device=s.device)
         r, its, rel = s.clone(), 0, 1.0
         while True:
             beta = float(torch.linalg.vector_norm(r).item())
@@ -964,40 +995,26 @@ class HipBackend:
             if not math.isfinite(rel) or rel <= tolerance or its >= max_iterations:
                 break
             V[0] = r.view(-1) / beta
-            H = np.zeros((m + 1, m))
-            g = np.zeros(m + 1)
-            g[0] = beta
-            cs, sn = np.zeros(m), np.zeros(m)
-            k = 0
+            H.zero_()
+            y = np.zeros(0)
             for j in range(m):
                 w = jt(prec(V[j].view_as(s))).view(-1)
                 its += 1
-                h = V[:j + 1] @ w
-                w -= V[:j + 1].T @ h
-                h2 = V[:j + 1] @ w
-                w -= V[:j + 1].T @ h2
-                hj = (h + h2).cpu().numpy()
-                hn = float(torch.linalg.vector_norm(w).item())
-                col = np.append(hj, hn)
-                for i in range(j):  # earlier rotations
-                    t = cs[i] * col[i] + sn[i] * col[i + 1]
-                    col[i + 1] = -sn[i] * col[i] + cs[i] * col[i + 1]
-                    col[i] = t
-                den = math.hypot(col[j], col[j + 1])
-                cs[j], sn[j] = (1.0, 0.0) if den == 0.0 else (col[j] / den, col[j + 1] / den)
-                col[j], col[j + 1] = den, 0.0
-                g[j + 1] = -sn[j] * g[j]
-                g[j] = cs[j] * g[j]
-                H[:j + 2, j] = col
+                Vj = V[:j + 1]
+                h = torch.mv(Vj, w)
+                w -= torch.mv(Vj.T, h)
+                h2 = torch.mv(Vj, w)
+                w -= torch.mv(Vj.T, h2)
+                hn = torch.linalg.vector_norm(w)
+                H[:j + 1, j] = h + h2
+                H[j + 1, j] = hn
+                V[j + 1] = w / torch.where(hn > 0.0, hn, torch.ones_like(hn))
                 k = j + 1
-                if hn > 0.0:
-                    V[j + 1] = w / hn
-                if hn == 0.0 or abs(g[j + 1]) <= tolerance * bnorm or its >= max_iterations:
-                    break
-            y = np.zeros(k)
-            for i in range(k - 1, -1, -1):  # back substitution
-                y[i] = (g[i] - H[i, i + 1:k] @ y[i + 1:k]) / H[i, i]
+                if k % check == 0 or k == m or its >= max_iterations:
+                    y, res = _least_squares_hessenberg(H[:k + 1, :k].cpu().numpy(), beta)
+                    if res <= tolerance * bnorm or len(y) < k or its >= max_iterations:
+                        break
             yt = torch.from_numpy(y).to(s.device)
-            lam += prec((V[:k].T @ yt).view_as(s))
+            lam += prec((V[:len(y)].T @ yt).view_as(s))
             r = s - jt(lam)
         return lam, rel, its, bool(math.isfinite(rel) and rel <= tolerance)

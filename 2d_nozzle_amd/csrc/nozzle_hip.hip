@@ -1869,7 +1869,7 @@ struct VTile {
     // register budget: three waves per SIMD (168 VGPRs) up to P2 where the
     // LDS allows them (measured 2 % faster than two waves at P2), two above
     static constexpr int LDS = 8 * (WORK + NF * N * NQF + EGN + GEN + TE) + 8 * TE * NF;
-    static constexpr int WPE = N <= 6 && LDS * 12 <= 160 * 1024 ? 3 : 2;
+    static constexpr int WPE = N <= 3 && LDS * 12 <= 160 * 1024 ? 3 : 2;
     // HLLC's reverse pass is capped at two waves per SIMD (256 registers): at
     // three it spills and runs 60 % slower
 };

@@ -99,3 +99,27 @@ def test_locality_tiles_deal_contiguous_shares_per_domain():
     for d in range(nd):
         mine = rank[tab[d::nd, 2]]
         assert np.array_equal(mine, np.arange(d * share, min((d + 1) * share, T)))
+
+
+def test_gmres_least_squares_matches_lstsq():
+    """The Givens solve of the GMRES least-squares problem against
+    numpy.linalg.lstsq on random upper Hessenberg matrices, and the early end
+    at an exact Krylov space (a zero subdiagonal entry)."""
+    be = nz_module("backend")
+    rng = np.random.default_rng(3)
+    for k in (1, 2, 5, 8, 30):
+        H = np.triu(rng.standard_normal((k + 1, k)), -1)
+        beta = 2.5
+        y, res = be._least_squares_hessenberg(H, beta)
+        rhs = np.zeros(k + 1)
+        rhs[0] = beta
+        y_ref = np.linalg.lstsq(H, rhs, rcond=None)[0]
+        np.testing.assert_allclose(y, y_ref, rtol=1e-10, atol=1e-12)
+        assert abs(res - np.linalg.norm(rhs - H @ y_ref)) <= 1e-12 * beta
+    H = np.triu(rng.standard_normal((6, 5)), -1)
+    H[3, 2] = 0.0  # the Krylov space is exact after three columns
+    y, res = be._least_squares_hessenberg(H, 1.0)
+    assert len(y) == 3
+    rhs = np.zeros(4)
+    rhs[0] = 1.0
+    np.testing.assert_allclose(y, np.linalg.lstsq(H[:4, :3], rhs, rcond=None)[0], rtol=1e-10)
