@@ -344,10 +344,20 @@ class BatchedSteady:
     def n_elements(self) -> int:
         return int(sum(o.n_elem for o in self.ops_list))
 
-    def solve(self, *, use_graph: bool = True, progress=None) -> list[DesignResult]:
-        """March every design to its own stop; results in input order."""
+    def solve(self, *, use_graph: bool = True, progress=None,
+              concurrent: bool = True) -> list[DesignResult]:
+        """March every design to its own stop; results in input order.
+
+        ``concurrent=False`` queues every family on one stream, one after the
+        other (the same launches and results; for measuring what running the
+        families side by side on their own streams gains)."""
         import torch
 
+        if not concurrent:
+            shared = torch.cuda.Stream(device=self.families[0].bk.device)
+            for fam in self.families:
+                fam.stream = shared
+                fam.graph = None
         opts = self.opts
         scale = residual_scale(self.flow, 1.0)
         chunk = min(max(1, opts.check_interval), opts.max_iterations)

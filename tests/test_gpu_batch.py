@@ -72,6 +72,17 @@ def test_eager_and_graph_replay_agree(sweep):
         assert np.array_equal(a.U, b.U)
 
 
+def test_serialized_families_give_the_same_bits(sweep):
+    """Families queued on one stream instead of side by side on their own."""
+    run, results, flow, opts = sweep
+    batch = nz_module("batch")
+    serial = batch.BatchedSteady(run.designs, flow, opts, ops_list=run.ops_list,
+                                 U0_list=run.U0_list).solve(concurrent=False)
+    for a, b in zip(results, serial):
+        assert a.iterations == b.iterations
+        assert np.array_equal(a.U, b.U)
+
+
 def test_batched_gradients_match_lone_adjoints(sweep):
     """Thrust gradients w.r.t. the area ratio from the batched adjoint march
     agree with each design's own adjoint solve (DifferentiableCase)."""
