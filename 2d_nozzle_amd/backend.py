@@ -85,6 +85,8 @@ _VJP_KERNEL: dict = {}
 # kernel on patch tiles measured faster than on tiles (affine meshes of at
 # least LOCALITY_MIN_ELEMENTS elements); see HipBackend.tune_element_kernel
 _DG_PATCHES: dict = {}
+# the same key -> milliseconds of the three timed steps, (tiles, patches)
+_DG_COSTS: dict = {}
 
 
 def _morton2(ix: np.ndarray, iy: np.ndarray) -> np.ndarray:
@@ -388,6 +390,7 @@ class HipBackend:
                 e1.synchronize()
                 cost.append(e0.elapsed_time(e1))
             _DG_PATCHES[key] = cost[1] < cost[0]
+            _DG_COSTS[key] = (cost[0], cost[1])
         self.set_patches(_DG_PATCHES[key])
         return _DG_PATCHES[key]
 

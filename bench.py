@@ -421,6 +421,29 @@ def adjoint_bench(refine: int, fd_check: bool = False) -> dict:
     return rec
 
 
+def element_kernel_bench(ncx: int, ncy: int) -> dict:
+    """The element kernel's two work shapes (tiles of 16 consecutive
+    elements; patch tiles, each shared edge's flux solved once) as the
+    autotuner times them on the channel: three fused SSP-RK3 steps each after
+    one, Roe, positivity limiter; P1 and P2 here, P3 per flux from the
+    headline's own tuning."""
+    wl, cf, be = _pkg("workloads"), _pkg("config"), _pkg("backend")
+    out = {}
+    for order in (1, 2):
+        ops, U = wl.channel_case(ncx, ncy, order)
+        bk = be.HipBackend(ops, cf.FlowConditions(), cf.SolverOptions())
+        bk.tune_element_kernel(U)
+        del bk
+    for key, (t_tiles, t_patch) in sorted(be._DG_COSTS.items()):
+        kind, order, flux_id, _ = key
+        name = f"{kind}-p{order}-{('roe', 'hllc', 'ausmup')[flux_id]}"
+        out[name] = {"tiles_ms_per_step": round(t_tiles / 3, 4),
+                     "patches_ms_per_step": round(t_patch / 3, 4),
+                     "kept": "patches" if be._DG_PATCHES[key] else "tiles"}
+    print(f"element kernel shapes: {out}", file=sys.stderr, flush=True)
+    return {"workload": f"channel-tri-{ncx}x{ncy}", "shapes": out}
+
+
 def _stops(res) -> dict:
     """Converged / total per (kind, order) family of the sweep."""
     fam: dict = {}
@@ -726,6 +749,7 @@ def main() -> int:
         }
 
     leg("march", not args.no_march, march_leg)
+    leg("element_kernel", not args.no_march, lambda: element_kernel_bench(args.ncx, args.ncy))
     leg("cpu_baseline", not args.no_cpu, lambda: cpu_baseline(args.order, args.cpu_seconds))
     leg("config1", not args.no_config1, config1_bench)
     leg("shocked", not args.no_shocked, lambda: shocked_bench(args.shocked_refine, 20))
