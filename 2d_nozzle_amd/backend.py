@@ -26,6 +26,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import time
 
 import numpy as np
 
@@ -362,13 +363,17 @@ class HipBackend:
         return (self.ops.topology.kind, int(self.ops.ref.order), int(self.P.flux_id),
                 bool(lg.get("nx_cells")))
 
-    def tune_element_kernel(self, U) -> bool:
+    def tune_element_kernel(self, U, *, warm_s: float = 0.25, rounds: int = 5,
+                            steps: int = 3) -> bool:
         """Autotuner for the element kernel's work shape on affine meshes of
-        at least LOCALITY_MIN_ELEMENTS elements: times three fused SSP-RK3
-        steps from ``U`` on locality-ordered tiles and on patch tiles, keeps
-        the faster for this (element kind, order, flux, structured or not) for
-        the rest of the process, and returns True for patches.  The two give
-        the same states bit for bit.  Smaller or curved meshes keep tiles."""
+        at least LOCALITY_MIN_ELEMENTS elements: after ``warm_s`` seconds of
+        alternating launches (the device's clocks settle over the first
+        hundreds of launches), ``rounds`` alternating timings of ``steps``
+        fused SSP-RK3 steps from ``U`` on locality-ordered tiles and on patch
+        tiles; the shape with the smaller median is kept for this (element
+        kind, order, flux, structured or not) for the rest of the process.
+        Returns True for patches.  The two give the same states bit for bit.
+        Smaller or curved meshes keep tiles."""
         if not self.affine or self._tiles is None:
             return False
         key = self._dg_key
@@ -377,20 +382,28 @@ class HipBackend:
             self._check_shape(U, "state", "U")
             u, _ = self._in(U)
             st = torch.cuda.current_stream(self.device)
-            cost = []
-            for on in (False, True):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+            def timed(on: bool) -> float:
                 self.set_patches(on)
                 self._S[0].copy_(u)
-                self.advance(self._S[0], self._S[1], 1, "ssprk3")
-                self._S[0].copy_(u)
-                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                 e0.record(st)
-                self.advance(self._S[0], self._S[1], 3, "ssprk3")
+                self.advance(self._S[0], self._S[1], steps, "ssprk3")
                 e1.record(st)
                 e1.synchronize()
-                cost.append(e0.elapsed_time(e1))
+                return e0.elapsed_time(e1)
+
+            t_end = time.perf_counter() + warm_s
+            while time.perf_counter() < t_end:
+                timed(False)
+                timed(True)
+            ta, tb = [], []
+            for _ in range(max(int(rounds), 1)):
+                ta.append(timed(False))
+                tb.append(timed(True))
+            cost = (float(np.median(ta)), float(np.median(tb)))
             _DG_PATCHES[key] = cost[1] < cost[0]
-            _DG_COSTS[key] = (cost[0], cost[1])
+            _DG_COSTS[key] = cost
         self.set_patches(_DG_PATCHES[key])
         return _DG_PATCHES[key]
 

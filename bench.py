@@ -424,8 +424,9 @@ def adjoint_bench(refine: int, fd_check: bool = False) -> dict:
 def element_kernel_bench(ncx: int, ncy: int) -> dict:
     """The element kernel's two work shapes (tiles of 16 consecutive
     elements; patch tiles, each shared edge's flux solved once) as the
-    autotuner times them on the channel: three fused SSP-RK3 steps each after
-    one, Roe, positivity limiter; P1 and P2 here, P3 per flux from the
+    autotuner times them on the channel: the median of five alternating
+    timings of three fused SSP-RK3 steps each, after a quarter second of
+    warm-up, Roe, positivity limiter; P1 and P2 here, P3 per flux from the
     headline's own tuning."""
     wl, cf, be = _pkg("workloads"), _pkg("config"), _pkg("backend")
     out = {}
