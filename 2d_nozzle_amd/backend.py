@@ -138,16 +138,18 @@ def patch_order(topo, coords: np.ndarray, E: int) -> np.ndarray:
     On the structured triangle grids (``topo.logical`` names ``nx_cells`` by
     ``nr_cells`` cells, elements ``c`` and ``c + ncx ncy`` being the lower and
     upper triangle of cell ``c = i ncy + j``) the cells follow a Morton curve
-    of ``(i, j)`` with both triangles of a cell together, so aligned runs of 16
-    are blocks of 4 x 2 cells; otherwise the Morton curve of the element
-    centroids.  The order only decides which elements share a workgroup.
+    of ``(j, i)`` with both triangles of a cell together, so aligned runs of
+    16 are blocks of 2 x 4 cells whose four cells along ``j`` have consecutive
+    ids (rows of U, dt and flags in runs of four elements); otherwise the
+    Morton curve of the element centroids.  The order only decides which
+    elements share a workgroup.
     """
     lg = getattr(topo, "logical", None) or {}
     ncx, ncy = int(lg.get("nx_cells", 0)), int(lg.get("nr_cells", 0))
     e = np.arange(E, dtype=np.int64)
     if topo.kind == "tri" and ncx > 0 and ncy > 0 and E == 2 * ncx * ncy and max(ncx, ncy) <= 65536:
         c = e % (ncx * ncy)
-        key = (_morton2(c // ncy, c % ncy) << np.uint64(1)) | (e // (ncx * ncy)).astype(np.uint64)
+        key = (_morton2(c % ncy, c // ncy) << np.uint64(1)) | (e // (ncx * ncy)).astype(np.uint64)
     else:
         cent = coords[topo.elem_nodes].mean(axis=1)
         lo, span = cent.min(axis=0), np.ptp(cent, axis=0) + 1e-300

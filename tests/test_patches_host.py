@@ -1,7 +1,8 @@
 """Patch tables of the element kernel, on the host: every element in exactly
 one patch, distinct norm slots, and compact patches on the structured grids
-(4 x 2 cell blocks: 12 faces off the patch, 30 flux-owning faces, so the
-flux items of a patch fill two wave rounds at four edge points)."""
+(2 x 4 cell blocks: 12 faces off the patch, 30 flux-owning faces, so the
+flux items of a patch fill two wave rounds at four edge points; the ids come
+in runs of four consecutive elements)."""
 
 from __future__ import annotations
 
@@ -48,12 +49,16 @@ def test_patch_table_covers_every_element_once():
     assert np.array_equal(tab[pos < tab.shape[0], 17], pos[pos < tab.shape[0]])
 
 
-def test_structured_patches_are_four_by_two_cell_blocks():
+def test_structured_patches_are_two_by_four_cell_blocks():
     be, mesh = nz_module("backend"), nz_module("mesh")
     topo, coords = mesh.synthetic_channel(64, 32)
     _, n_off, owners = _patch_stats(topo, be.patch_table(be.patch_order(topo, coords, topo.n_elem)))
     print("off-patch faces: mean", n_off.mean(), "max", n_off.max(), "owners max", owners.max())
     assert n_off.max() <= 12 and owners.max() <= 32
+    tab = be.patch_table(be.patch_order(topo, coords, topo.n_elem))
+    ids = np.sort(tab[:, :16], axis=1)
+    runs = (np.diff(ids, axis=1) != 1).sum(axis=1) + 1
+    assert runs.max() == 4  # four runs of four consecutive element ids
 
 
 def test_centroid_patches_without_grid_indices():
