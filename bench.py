@@ -25,9 +25,9 @@ The rest of the metric rides in the same line:
   converged figures (the committed fixture), and the same march on the CPU
   oracle.
 * ``sweep``: designs/s and gradients/s of the 256-design batched sweep
-  against one-at-a-time solves.
+  against one-at-a-time solves (256 mixed designs, seed 1).
 * ``adjoint``: gradients/s of the eight Bezier-wall design derivatives of
-  thrust on 200,704 P2 triangles (BASELINE config 4).
+  thrust on 200,704 P2 triangles.
 * ``cpu_baseline``: the CPU oracle (an OpenMP restatement of the reference's
   compiled kernels), strict IEEE and -ffast-math builds, every host core and
   one thread, median of 5 repeats of at least 50 evaluations on a bounded
@@ -208,7 +208,7 @@ def _timed(fn, steps: int, warmup: int, stream, min_warm_s: float = 1.0,
 
 
 def config1_bench() -> dict:
-    """Time to solution of BASELINE config 1: the smooth nozzle (area ratio
+    """Time to solution of the smooth nozzle (area ratio
     2.5, P1 triangles, refine 1, Roe, positivity limiter, SSP-RK3) from the
     quasi-1D state to a scaled residual of 1e-10, on the device and on the CPU
     oracle, with each result's relative error against the reference's

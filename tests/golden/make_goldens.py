@@ -235,7 +235,7 @@ def case_cfg1_converged(src, out: dict) -> None:
 
 
 def synthetic_channel(ncx: int, ncy: int, order: int, seed: int = 0):
-    """Synthetic converging-diverging channel state generator (config-2 form)."""
+    """Synthetic converging-diverging channel state generator (the 2^20-triangle benchmark form)."""
     from src.geometry.mesh import structured_nozzle_topology
     from src.reference import elements as el
 

@@ -1,4 +1,4 @@
-"""Config 1 end to end on the device: the smooth nozzle (AR 2.5, P1 triangles,
+"""The smooth nozzle end to end on the device (AR 2.5, P1 triangles,
 refine 1, Roe, positivity limiter, SSP-RK3) marched from the reference's
 quasi-1D initial state to a scaled residual of 1e-10.
 

@@ -182,7 +182,7 @@ def test_device_build_at_config3_size():
     topo_h, coords, _ = mesh.build_nozzle_mesh(geom, kind="quad", refine=7, geometry_order=2)
     host = opm.build_operators(topo_h, coords, 2, dense=False)
     t_host = time.perf_counter() - t0
-    print(f"config-3 mesh + operators: device {t_dev:.2f} s, host {t_host:.2f} s", flush=True)
+    print(f"1.15M-quad Q2 mesh + operators: device {t_dev:.2f} s, host {t_host:.2f} s", flush=True)
     assert dev.n_elem == 1146880
     _assert_maps_equal(topo.edges, topo_h.edges, "refine 7")
     # The metric derivatives sum O(1) node coordinates into O(h) results

@@ -85,10 +85,11 @@ def test_device_initial_state_on_device_operators(name):
     assert err < 1e-10
 
 
-def test_config1_solved_from_device_built_inputs():
-    """Config 1 end to end with the mesh, operators and initial state built
-    on the device: the same stop iteration as the reference (3,601) and its
-    converged thrust, mass flow and exit Mach to 1e-10."""
+def test_smooth_nozzle_solved_from_device_built_inputs():
+    """The smooth nozzle (P1, refine 1) end to end with the mesh, operators
+    and initial state built on the device: the same stop iteration as the
+    reference (3,601) and its converged thrust, mass flow and exit Mach to
+    1e-10."""
     cf, dm, ini, mesh, mr = (nz_module("config"), nz_module("device_mesh"),
                              nz_module("initial"), nz_module("mesh"), nz_module("march"))
     be = nz_module("backend")
@@ -104,6 +105,6 @@ def test_config1_solved_from_device_built_inputs():
     errs = {k: abs(getattr(perf, a) / float(g[f"perf_{a}"]) - 1) for k, a in
             (("thrust", "thrust"), ("mass_flow_in", "mass_flow_in"),
              ("mass_flow_out", "mass_flow_out"), ("exit_mach", "exit_mach_area_averaged"))}
-    print("device-built config 1:", res.iterations, "iterations; errors", errs)
+    print("device-built smooth nozzle:", res.iterations, "iterations; errors", errs)
     assert res.converged and res.iterations == int(g["iterations"])
     assert max(errs.values()) <= 1e-10
