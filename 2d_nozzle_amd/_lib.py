@@ -135,6 +135,7 @@ SIGNATURES = {
     "nz_superbee_work_doubles": (C.c_int64, [C.POINTER(Problem)]),
     "nz_adjoint_work_doubles": (C.c_int64, [C.POINTER(Problem)]),
     "nz_adjoint_tile_elements": (C.c_int64, [C.POINTER(Problem)]),
+    "nz_face_node": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int]),
     "nz_residual_vjp": (C.c_int, [C.POINTER(Problem), P, P, P, P, P, P, P, P]),
     "nz_adjoint_rk4_stage": (C.c_int, [C.POINTER(Problem), P, P, P, P, P, P, P, C.c_int, P, P]),
     "nz_adjoint_defect": (C.c_int, [C.POINTER(Problem), P, P, P, P, P, P, P]),

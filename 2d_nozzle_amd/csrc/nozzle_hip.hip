@@ -76,7 +76,152 @@ struct Cfg {
     static constexpr int AWPE = (160 * 1024) / ALDS < 1   ? 1
                                 : (160 * 1024) / ALDS > 3 ? 3
                                                           : (160 * 1024) / ALDS;
+    // polynomial order and the Lagrange nodes on one face (p + 1 of them)
+    static constexpr int PO = NF == 3 ? (N == 1 ? 0 : N == 3 ? 1 : N == 6 ? 2 : 3)
+                                      : (N == 1 ? 0 : N == 4 ? 1 : N == 9 ? 2 : 3);
+    static constexpr int FN = PO + 1;
+    // Face traces from the face's nodes alone.  The table entries of the other
+    // nodes are round-off of an identically vanishing function (at most
+    // 2.6e-15 for triangles and for quadrilaterals up to p = 2); p = 3
+    // quadrilaterals carry up to 4.3e-14 there and keep the full sums, and
+    // p = 0 has no node on a face.
+    static constexpr bool SPT = PO >= 1 && !(NF == 4 && PO == 3);
 };
+
+// Lagrange node j (0..p) of face f, in the reference's node numbering (an
+// equispaced lattice (ix, iy), row by row in iy; elements.py:57-75).  Every
+// other node's basis function vanishes identically on face f.  Triangles:
+// face 0 is ix + iy = p, face 1 is ix = 0, face 2 is iy = 0; quadrilaterals:
+// face 0 is iy = 0, face 1 is ix = p, face 2 is iy = p, face 3 is ix = 0.
+__host__ __device__ constexpr int face_node(int nface, int p, int f, int j) {
+    if (nface == 3) {
+        const int ix = f == 0 ? p - j : (f == 1 ? 0 : j);
+        const int iy = f == 2 ? 0 : j;
+        return iy * (p + 1) - iy * (iy - 1) / 2 + ix;
+    }
+    const int ix = f == 0 ? j : (f == 1 ? p : (f == 2 ? j : 0));
+    const int iy = f == 0 ? 0 : (f == 1 ? j : (f == 2 ? p : j));
+    return iy * (p + 1) + ix;
+}
+
+// Sums sum_i tab[i * NQF + q] * col[i] at the face points of face F (its
+// side-0 table `tab`): over the face's nodes alone when C::SPT, in node
+// order j = 0..p, else over every basis function.  Every trace of the
+// element kernels goes through here, so a trace formed by an element and the
+// same trace formed by its neighbour round alike.
+template <class C, int F, int NC, class OUT>
+__device__ __forceinline__ void trace_face(const double* tab, const double (*const col[NC])[C::N],
+                                           OUT out) {
+    constexpr int N = C::N, NQF = C::NQF;
+#pragma unroll
+    for (int q = 0; q < NQF; ++q) {
+        double a[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) a[c] = 0.0;
+        if constexpr (C::SPT) {
+#pragma unroll
+            for (int j = 0; j < C::FN; ++j) {
+                const int i = face_node(C::NF, C::PO, F, j);
+#pragma unroll
+                for (int c = 0; c < NC; ++c) a[c] += tab[i * NQF + q] * (*col[c])[i];
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+#pragma unroll
+                for (int c = 0; c < NC; ++c) a[c] += tab[i * NQF + q] * (*col[c])[i];
+        }
+        out(q, a);
+    }
+}
+
+// trace_face for a wave-uniform runtime face f (one branch per face)
+template <class C, int NC, class OUT>
+__device__ __forceinline__ void trace_face_rt(int f, const double* phi_face,
+                                              const double (*const col[NC])[C::N], OUT out) {
+    constexpr int S = C::N * C::NQF;
+    if (f == 0) {
+        trace_face<C, 0, NC>(phi_face, col, out);
+    } else if (f == 1) {
+        trace_face<C, 1, NC>(phi_face + S, col, out);
+    } else if constexpr (C::NF == 3) {
+        trace_face<C, 2, NC>(phi_face + 2 * S, col, out);
+    } else {
+        if (f == 2) trace_face<C, 2, NC>(phi_face + 2 * S, col, out);
+        else trace_face<C, 3, NC>(phi_face + 3 * S, col, out);
+    }
+}
+
+// The same sums for a neighbour's face: `cn` holds the neighbour's
+// coefficients at the nodes of its face key (C::FN of them, node order) when
+// C::SPT, else all C::N.  `tab` is the key's side-0 table (wave-uniform
+// pointer: scalar loads, or a per-lane LDS copy).
+template <class C, int NC, class TAB, class OUT>
+__device__ __forceinline__ void trace_nbr(int key, TAB tab, const double (*const cn[NC])[C::N],
+                                          OUT out) {
+    constexpr int NQF = C::NQF;
+    constexpr int NI = C::SPT ? C::FN : C::N;
+#pragma unroll
+    for (int q = 0; q < NQF; ++q) {
+        double a[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) a[c] = 0.0;
+#pragma unroll
+        for (int j = 0; j < NI; ++j) {
+            const int i = C::SPT ? face_node(C::NF, C::PO, key, j) : j;
+            const double b = tab(i * NQF + q);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) a[c] += b * (*cn[c])[j];
+        }
+        out(q, a);
+    }
+}
+
+// node of slot j of a neighbour column: face nodes of face `key` when C::SPT
+template <class C>
+__device__ __forceinline__ int nbr_node(int key, int j) {
+    return C::SPT ? face_node(C::NF, C::PO, key, j) : j;
+}
+
+// whether basis function i enters the traces of face f
+template <class C>
+__device__ __forceinline__ bool on_face(int f, int i) {
+    if (!C::SPT) return true;
+    bool on = false;
+#pragma unroll
+    for (int j = 0; j < C::FN; ++j) on = on || face_node(C::NF, C::PO, f, j) == i;
+    return on;
+}
+
+// acc[i] += sum_q tab[i * NQF + q] fw[q] over the nodes of face F (all nodes
+// unless C::SPT): the transpose of the face trace, with the side-0 table
+template <class C, int F>
+__device__ __forceinline__ void lift_face(const double* tab, const double (&fw)[C::NQF],
+                                          double (&acc)[C::N]) {
+    constexpr int NQF = C::NQF, NI = C::SPT ? C::FN : C::N;
+#pragma unroll
+    for (int j = 0; j < NI; ++j) {
+        const int i = C::SPT ? face_node(C::NF, C::PO, F, j) : j;
+#pragma unroll
+        for (int q = 0; q < NQF; ++q) acc[i] += tab[i * NQF + q] * fw[q];
+    }
+}
+
+template <class C>
+__device__ __forceinline__ void lift_face_rt(int f, const double* phi_face,
+                                             const double (&fw)[C::NQF], double (&acc)[C::N]) {
+    constexpr int S = C::N * C::NQF;
+    if (f == 0) {
+        lift_face<C, 0>(phi_face, fw, acc);
+    } else if (f == 1) {
+        lift_face<C, 1>(phi_face + S, fw, acc);
+    } else if constexpr (C::NF == 3) {
+        lift_face<C, 2>(phi_face + 2 * S, fw, acc);
+    } else {
+        if (f == 2) lift_face<C, 2>(phi_face + 2 * S, fw, acc);
+        else lift_face<C, 3>(phi_face + 3 * S, fw, acc);
+    }
+}
 
 // ---------------------------------------------------------------- limiter
 // Zhang-Shu positivity limiter on one element whose coefficients sit at
@@ -579,14 +724,18 @@ __device__ __forceinline__ void dg_volume(const nz_problem& P, const double* __r
 // mass of non-affine elements, then either the residual / rate rows, or the
 // fused Runge-Kutta stage (time step, combination, norm partial, positivity
 // limiter); the tile's rows staged in LDS go out through store().
-template <class C, class L, bool STAGE, class STORE>
+template <class C, class L, bool STAGE, class STORE, class LOADN>
 __device__ __forceinline__ void dg_epilogue(
     const nz_problem& P, const double* __restrict__ T, double* sW, const int* sIn,
     double (&u)[C::N], double (&acc)[C::N], double wave, int t, int el, int s, int64_t e, bool live,
-    int64_t slot, const double* __restrict__ Un, double* __restrict__ dt, double dt_factor,
-    double ca, double cb, int mode, double* __restrict__ partial, uint8_t* __restrict__ flags,
-    unsigned long long* __restrict__ counters, STORE store) {
+    int64_t slot, double* __restrict__ dt, double dt_factor, double ca, double cb, int mode,
+    double* __restrict__ partial, uint8_t* __restrict__ flags,
+    unsigned long long* __restrict__ counters, STORE store, LOADN load_un) {
     constexpr int N = C::N, NF = C::NF;
+    // a later stage combines with Un: its rows are requested first, in the
+    // tile's row layout (coalesced), and handed to the lanes through LDS
+    double rn[TE * C::NS / DG_BS];
+    if (STAGE && mode == kStageNext) load_un(rn);
     // acc holds R (unfolded) or -rate (affine, folded) for component s
     if ((STAGE || mode != kRes) && !C::AFF && live) {
         double r[N];
@@ -613,6 +762,11 @@ __device__ __forceinline__ void dg_epilogue(
     }
 
     // fused stage: time step, combination, norm partials, limiter, store
+    if (mode == kStageNext) {
+#pragma unroll
+        for (int j = 0; j < TE * C::NS / DG_BS; ++j) sW[t + j * DG_BS] = rn[j];
+        wave_sync();
+    }
     double h = 0.0;
     if (live) {
         if (mode == kStageFirst) {
@@ -632,7 +786,7 @@ __device__ __forceinline__ void dg_epilogue(
             const double v = F * P.wnorm[s];
             nacc += live ? v * v : 0.0;
         } else {
-            u[i] = live ? ca * Un[(e * N + i) * 4 + s] + cb * (u[i] + h * F) : 0.0;
+            u[i] = live ? ca * sW[el * L::SIO + i * 4 + s] + cb * (u[i] + h * F) : 0.0;
         }
     }
     if (mode == kStageFirst) {
@@ -681,6 +835,7 @@ void k_dg(const nz_problem P, const double* __restrict__ T,
     using L = Tile<C>;
     constexpr int N = C::N, NQV = C::NQV, NQF = C::NQF, NF = C::NF, NS = C::NS;
     constexpr int NPF = NF * N * NQF;  // canonical (side-0) face table
+    constexpr int NI = C::SPT ? C::FN : N;  // neighbour coefficients a face trace reads
     __shared__ double sW[L::WORK];
     __shared__ double sPF[NPF];
     __shared__ double sEG[L::EGN];
@@ -796,25 +951,22 @@ void k_dg(const nz_problem P, const double* __restrict__ T,
     for (int f = 0; f < NF; ++f) {
         const int nb = sNb[el * NF + f], inf = sIn[el * NF + f];
         const bool has = live && nb >= 0;
-        // the neighbour's column of component s, requested before the own
-        // trace is formed (lanes without a neighbour read the tile's first
-        // element and discard it)
+        const int key = has ? (inf & 3) : -1;  // the neighbour's local face
+        // the neighbour's column of component s at the nodes of its face,
+        // requested before the own trace is formed (lanes without a
+        // neighbour read the tile's first element and discard it)
         double xn[N];
         {
             const double* ur = Uin + (has ? static_cast<int64_t>(nb) : e0) * NS + s;
+            const int kk = has ? key : 0;
 #pragma unroll
-            for (int i = 0; i < N; ++i) xn[i] = ur[i * 4];
+            for (int j = 0; j < NI; ++j) xn[j] = ur[nbr_node<C>(kk, j) * 4];
         }
         // (e,s): own and neighbour traces of component s, canonical order
         {
-            const double* b = T + P.off.phi_face + f * N * NQF;
-#pragma unroll
-            for (int q = 0; q < NQF; ++q) {
-                double a = 0.0;
-#pragma unroll
-                for (int i = 0; i < N; ++i) a += b[i * NQF + q] * u[i];
-                sTo[q * QS + cs] = a;
-            }
+            const double (*cols[1])[N] = {&u};
+            trace_face_rt<C, 1>(f, T + P.off.phi_face, cols,
+                                [&](int q, const double* a) { sTo[q * QS + cs] = a[0]; });
         }
         {
             // The neighbour's face table is read with scalar loads when the
@@ -823,22 +975,15 @@ void k_dg(const nz_problem P, const double* __restrict__ T,
             double tn[NQF];
 #pragma unroll
             for (int q = 0; q < NQF; ++q) tn[q] = 0.0;
-            const int key = has ? (inf & 3) : -1;
             const int key0 = __builtin_amdgcn_readfirstlane(key);
+            const double (*cn[1])[N] = {&xn};
+            auto put = [&](int q, const double* a) { tn[q] = a[0]; };
             if (__all(key == key0) && key0 >= 0) {
                 const double* ph = T + P.off.phi_face + key0 * N * NQF;
-#pragma unroll
-                for (int i = 0; i < N; ++i) {
-#pragma unroll
-                    for (int q = 0; q < NQF; ++q) tn[q] += ph[i * NQF + q] * xn[i];
-                }
+                trace_nbr<C, 1>(key0, [&](int k) { return ph[k]; }, cn, put);
             } else if (has) {
                 const double* ph = sPF + key * N * NQF;
-#pragma unroll
-                for (int i = 0; i < N; ++i) {
-#pragma unroll
-                    for (int q = 0; q < NQF; ++q) tn[q] += ph[i * NQF + q] * xn[i];
-                }
+                trace_nbr<C, 1>(key, [&](int k) { return ph[k]; }, cn, put);
             }
 #pragma unroll
             for (int q = 0; q < NQF; ++q) sTn[q * QS + cs] = tn[q];
@@ -912,9 +1057,18 @@ void k_dg(const nz_problem P, const double* __restrict__ T,
 
     dg_volume<C, L>(P, T, sW, sGe, u, acc, t, cs, ne, fold,
                     [&](int pe) -> int64_t { return e0 + pe; });
-    dg_epilogue<C, L, STAGE>(P, T, sW, sIn, u, acc, wave, t, el, s, e, live, slot, Un, dt,
+    dg_epilogue<C, L, STAGE>(P, T, sW, sIn, u, acc, wave, t, el, s, e, live, slot, dt,
                              dt_factor, ca, cb, mode, partial, flags, counters,
-                             [&]() { store_tile<L>(Uout + e0 * NS, sW, t, ne); });
+                             [&]() { store_tile<L>(Uout + e0 * NS, sW, t, ne); },
+                             [&](double (&rn)[TE * NS / DG_BS]) {  // the tile's rows of Un
+                                 constexpr int NU = TE * NS / DG_BS;
+                                 const double* src = Un + e0 * NS;
+#pragma unroll
+                                 for (int j = 0; j < NU; ++j) {
+                                     const int x = t + j * DG_BS;
+                                     rn[j] = x < ne * NS ? src[x] : 0.0;
+                                 }
+                             });
 }
 
 // ------------------------------------------------------ patch tiles
@@ -982,6 +1136,7 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
            uint8_t* __restrict__ flags, unsigned long long* __restrict__ counters) {
     using L = PTile<C>;
     constexpr int N = C::N, NQF = C::NQF, NF = C::NF, NS = C::NS, SL = L::SL;
+    constexpr int NI = C::SPT ? C::FN : N;  // neighbour coefficients a face trace reads
     static_assert(C::AFF && C::STR, "patch tiles serve affine elements");
     static_assert(L::SL <= DG_BS, "one lane per face slot");
     static_assert(4 * XMAX <= DG_BS, "one ghost item per lane");
@@ -1005,16 +1160,25 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
     const bool fold = STAGE || mode != kRes;
     const int cs = at(0, s, el);  // the volume rows keep k_dg's layout
 
-    // ---- this lane's column of its element, the face maps (one lane per
-    // face slot) and the affine metrics, all requested before any is used
+    // ---- the patch's rows of U, the face maps (one lane per face slot) and
+    // the affine metrics, all requested before any is used.  The rows are
+    // read as one virtual array of TE rows of NS doubles: the entry at tile
+    // offset x belongs to patch element x / NS, so each load instruction
+    // covers at most three contiguous runs of memory (an element's NS
+    // doubles are contiguous), not one 32-byte piece per element.
     __builtin_amdgcn_s_setprio(2);
     double u[N], acc[N];
     double g_nx, g_ny, g_jac;
     {
         constexpr int NG = (L::GEN + DG_BS - 1) / DG_BS;
-        const double* ur = Uin + e * NS + s;
+        constexpr int NU = TE * NS / DG_BS;  // = N
+        double ru[NU];
 #pragma unroll
-        for (int i = 0; i < N; ++i) u[i] = ur[i * 4];
+        for (int j = 0; j < NU; ++j) {
+            const int x = t + j * DG_BS, r = x / NS;
+            const int id = __shfl(myid, 4 * r, 64);
+            ru[j] = r < ne ? Uin[static_cast<int64_t>(id) * NS + (x - r * NS)] : 0.0;
+        }
         int rn = -1, ri = 0;
         if (t < SL) {
             const int r = t / NF;
@@ -1046,10 +1210,16 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
 #pragma unroll
         for (int j = 0; j < NG; ++j)
             if (t + j * DG_BS < L::GEN) sGe[t + j * DG_BS] = rg[j];
-    }
 #pragma unroll
-    for (int i = 0; i < N; ++i) acc[i] = 0.0;
+        for (int j = 0; j < NU; ++j) sW[t + j * DG_BS] = ru[j];
+    }
     wave_sync();
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        u[i] = sW[el * L::SIO + i * 4 + s];
+        acc[i] = 0.0;
+    }
+    wave_sync();  // the staging rows are reused below
     __builtin_amdgcn_s_setprio(0);
     if (t < TE) sRd[t] = 1.0 / sGe[t * 5 + 4];
 
@@ -1100,36 +1270,27 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
     const int n_gh = n_off < XMAX ? n_off : XMAX;
     const bool gl = t < 4 * n_gh;
     const int gk = t >> 2, gc = t & 3, gsl = gl ? sExt[gk] : 0;
+    const int gkey = gl ? (sIn[gsl] & 3) : 0;  // the neighbour's local face
     double xn[N];
     {
         const double* ur = Uin + (gl ? static_cast<int64_t>(sNb[gsl]) * NS : 0) + gc;
 #pragma unroll
-        for (int i = 0; i < N; ++i) xn[i] = ur[i * 4];
+        for (int j = 0; j < NI; ++j) xn[j] = ur[nbr_node<C>(gkey, j) * 4];
     }
     // ---- own traces on every face, canonical point order
+    {
+        const double (*cols[1])[N] = {&u};
 #pragma unroll 1
-    for (int f = 0; f < NF; ++f) {
-        const double* b = T + P.off.phi_face + f * N * NQF;
-#pragma unroll
-        for (int q = 0; q < NQF; ++q) {
-            double a = 0.0;
-#pragma unroll
-            for (int i = 0; i < N; ++i) a += b[i * NQF + q] * u[i];
-            sT[atq(f * NQF + q, el) + s] = a;
-        }
+        for (int f = 0; f < NF; ++f)
+            trace_face_rt<C, 1>(f, T + P.off.phi_face, cols, [&](int q, const double* a) {
+                sT[atq(f * NQF + q, el) + s] = a[0];
+            });
     }
     if (gl) {
-        const double* ph = T + P.off.phi_face + (sIn[gsl] & 3) * N * NQF;
-        double tn[NQF];
-#pragma unroll
-        for (int q = 0; q < NQF; ++q) tn[q] = 0.0;
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-#pragma unroll
-            for (int q = 0; q < NQF; ++q) tn[q] += ph[i * NQF + q] * xn[i];
-        }
-#pragma unroll
-        for (int q = 0; q < NQF; ++q) sG[(gk * NQF + q) * 4 + gc] = tn[q];
+        const double* ph = T + P.off.phi_face + gkey * N * NQF;
+        const double (*cn[1])[N] = {&xn};
+        trace_nbr<C, 1>(gkey, [&](int k) { return ph[k]; }, cn,
+                        [&](int q, const double* a) { sG[(gk * NQF + q) * 4 + gc] = a[0]; });
     }
     wave_sync();
     // ---- one Riemann solve per edge point.  The rounds are wave-uniform so
@@ -1178,7 +1339,10 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
 #pragma unroll 1
                     for (int c = 0; c < 4; ++c) {
                         double v = 0.0;
-                        for (int i = 0; i < N; ++i) v += ph[i * NQF] * ur[i * 4 + c];
+                        for (int j = 0; j < NI; ++j) {
+                            const int i = nbr_node<C>(infA & 3, j);
+                            v += ph[i * NQF] * ur[i * 4 + c];
+                        }
                         b[c] = v;
                     }
                 }
@@ -1238,13 +1402,24 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
     dg_volume<C, L>(P, T, sW, sGe, u, acc, t, cs, ne, fold,
                     [&](int pe) -> int64_t { return sId[pe]; });
     dg_epilogue<C, L, STAGE>(
-        P, T, sW, sIn, u, acc, wave, t, el, s, e, live, slot, Un, dt, dt_factor, ca, cb, mode,
-        partial, flags, counters, [&]() {
-            if (live) {  // each lane its own column, as it was loaded
-                double* o = Uout + e * NS + s;
+        P, T, sW, sIn, u, acc, wave, t, el, s, e, live, slot, dt, dt_factor, ca, cb, mode,
+        partial, flags, counters,
+        [&]() {  // the staged rows back, as they were loaded
+            constexpr int NU = TE * NS / DG_BS;
 #pragma unroll
-                for (int i = 0; i < N; ++i)
-                    __builtin_nontemporal_store(sW[el * L::SIO + i * 4 + s], o + i * 4);
+            for (int j = 0; j < NU; ++j) {
+                const int x = t + j * DG_BS, r = x / NS;
+                if (r < ne)
+                    __builtin_nontemporal_store(
+                        sW[x], Uout + static_cast<int64_t>(sId[r]) * NS + (x - r * NS));
+            }
+        },
+        [&](double (&rn)[TE * NS / DG_BS]) {  // the rows of Un, as U was loaded
+            constexpr int NU = TE * NS / DG_BS;
+#pragma unroll
+            for (int j = 0; j < NU; ++j) {
+                const int x = t + j * DG_BS, r = x / NS;
+                rn[j] = r < ne ? Un[static_cast<int64_t>(sId[r]) * NS + (x - r * NS)] : 0.0;
             }
         });
 }
@@ -1567,7 +1742,8 @@ void k_vjp(const nz_problem P, const double* __restrict__ T,
                     const double* ur = U + static_cast<int64_t>(nb) * NS + s;
                     const double* lr = Lin + static_cast<int64_t>(nb) * NS + s;
                     const double* ph = sPF + (oside * NF + of) * N * NQF;
-                    for (int i = 0; i < N; ++i) {
+                    for (int j = 0; j < (C::SPT ? C::FN : N); ++j) {
+                        const int i = nbr_node<C>(of, j);
                         const double x = ur[i * 4], y = lr[i * 4];
 #pragma unroll
                         for (int q = 0; q < NQF; ++q) {
@@ -1595,7 +1771,8 @@ void k_vjp(const nz_problem P, const double* __restrict__ T,
                 if (nb >= 0) {
                     const double* ph = sPF + (side * NF + f) * N * NQF + q;
                     double uo[4] = {0.0, 0.0, 0.0, 0.0}, lo[4] = {0.0, 0.0, 0.0, 0.0};
-                    for (int i = 0; i < N; ++i) {
+                    for (int j = 0; j < (C::SPT ? C::FN : N); ++j) {
+                        const int i = nbr_node<C>(f, j);
                         const double b = ph[i * NQF];
                         for (int s = 0; s < 4; ++s) {
                             uo[s] += b * sU[(i * 4 + s) * TP + el];
@@ -1672,7 +1849,7 @@ void k_vjp(const nz_problem P, const double* __restrict__ T,
 #pragma unroll
         for (int kk = 0; kk < C::NIT; ++kk) {
             const int i = slot + kk * SLOTS;
-            if (i < N && live) {
+            if (i < N && live && on_face<C>(f, i)) {
                 const int side = sSide[f * TE + el];
                 for (int q = 0; q < NQF; ++q) {
                     const double b0 = T[P.off.phi_face + ((0 * NF + f) * N + i) * NQF + q];
@@ -1890,6 +2067,7 @@ void k_vjpw(const nz_problem P, const double* __restrict__ T,
     constexpr int N = C::N, NQV = C::NQV, NQF = C::NQF, NF = C::NF, NS = C::NS;
     constexpr int NPF = NF * N * NQF;
     constexpr int NU = TE * NS / DG_BS;  // = N
+    constexpr int NI = C::SPT ? C::FN : N;  // neighbour coefficients a face trace reads
     __shared__ double sW[L::WORK];
     __shared__ double sPF[NPF];
     __shared__ double sEG[L::EGN];
@@ -1999,55 +2177,41 @@ void k_vjpw(const nz_problem P, const double* __restrict__ T,
     for (int f = 0; f < NF; ++f) {
         const int nb = sNb[el * NF + f], inf = sIn[el * NF + f];
         const bool has = live && nb >= 0;
+        const int key = has ? (inf & 3) : -1;  // the neighbour's local face
         double xn[N], yn[N];
         {
             const int64_t base = (has ? static_cast<int64_t>(nb) : e0) * NS + s;
+            const int kk = has ? key : 0;
 #pragma unroll
-            for (int i = 0; i < N; ++i) {
-                xn[i] = U[base + i * 4];
-                yn[i] = Lin[base + i * 4];
+            for (int j = 0; j < NI; ++j) {
+                const int i = nbr_node<C>(kk, j);
+                xn[j] = U[base + i * 4];
+                yn[j] = Lin[base + i * 4];
             }
         }
         {
-            const double* b = T + P.off.phi_face + f * N * NQF;
-#pragma unroll
-            for (int q = 0; q < NQF; ++q) {
-                double a = 0.0, c = 0.0;
-#pragma unroll
-                for (int i = 0; i < N; ++i) {
-                    a += b[i * NQF + q] * u[i];
-                    c += b[i * NQF + q] * l[i];
-                }
-                sUo[q * QS + cs] = a;
-                sLo[q * QS + cs] = c;
-            }
+            const double (*cols[2])[N] = {&u, &l};
+            trace_face_rt<C, 2>(f, T + P.off.phi_face, cols, [&](int q, const double* a) {
+                sUo[q * QS + cs] = a[0];
+                sLo[q * QS + cs] = a[1];
+            });
         }
         {
             double tu[NQF], tl[NQF];
 #pragma unroll
             for (int q = 0; q < NQF; ++q) tu[q] = tl[q] = 0.0;
-            const int key = has ? (inf & 3) : -1;
             const int key0 = __builtin_amdgcn_readfirstlane(key);
+            const double (*cn[2])[N] = {&xn, &yn};
+            auto put = [&](int q, const double* a) {
+                tu[q] = a[0];
+                tl[q] = a[1];
+            };
             if (__all(key == key0) && key0 >= 0) {
                 const double* ph = T + P.off.phi_face + key0 * N * NQF;
-#pragma unroll
-                for (int i = 0; i < N; ++i) {
-#pragma unroll
-                    for (int q = 0; q < NQF; ++q) {
-                        tu[q] += ph[i * NQF + q] * xn[i];
-                        tl[q] += ph[i * NQF + q] * yn[i];
-                    }
-                }
+                trace_nbr<C, 2>(key0, [&](int k) { return ph[k]; }, cn, put);
             } else if (has) {
                 const double* ph = sPF + key * N * NQF;
-#pragma unroll
-                for (int i = 0; i < N; ++i) {
-#pragma unroll
-                    for (int q = 0; q < NQF; ++q) {
-                        tu[q] += ph[i * NQF + q] * xn[i];
-                        tl[q] += ph[i * NQF + q] * yn[i];
-                    }
-                }
+                trace_nbr<C, 2>(key, [&](int k) { return ph[k]; }, cn, put);
             }
 #pragma unroll
             for (int q = 0; q < NQF; ++q) {
@@ -2099,12 +2263,7 @@ void k_vjpw(const nz_problem P, const double* __restrict__ T,
             double fw[NQF];
 #pragma unroll
             for (int q = 0; q < NQF; ++q) fw[q] = sUo[q * QS + cs];
-            const double* lb = T + P.off.phi_face + f * N * NQF;
-#pragma unroll
-            for (int i = 0; i < N; ++i) {
-#pragma unroll
-                for (int q = 0; q < NQF; ++q) acc[i] += lb[i * NQF + q] * fw[q];
-            }
+            lift_face_rt<C>(f, T + P.off.phi_face, fw, acc);
         }
         // the next face rewrites these rows after the reads above
     }
@@ -2706,6 +2865,12 @@ int64_t nz_scratch_doubles(const nz_problem* p) {
     int64_t a = n_blocks(p, o) > n_blocks_dg(p) ? n_blocks(p, o) : n_blocks_dg(p);
     if (n_blocks_vjp(p, o) > a) a = n_blocks_vjp(p, o);
     return a > 1024 ? a : 1024;
+}
+
+int nz_face_node(int nface, int p, int f, int j) {
+    if ((nface != 3 && nface != 4) || p < 1 || p > 3 || f < 0 || f >= nface || j < 0 || j > p)
+        return -1;
+    return nz::face_node(nface, p, f, j);
 }
 
 int64_t nz_adjoint_tile_elements(const nz_problem* p) {

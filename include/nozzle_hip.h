@@ -129,6 +129,16 @@ int64_t nz_scratch_doubles(const nz_problem *p);
  * src/sensitivity.py:317). */
 int64_t nz_adjoint_tile_elements(const nz_problem *p);
 
+/* Lagrange node j (0 <= j <= p) on local face f of the reference element
+ * (nface 3: triangles, 4: quadrilaterals) of order p, in the reference's
+ * node numbering (src/geometry/elements.py:57-75); -1 for arguments out of
+ * range.  The basis functions of all other nodes vanish on that face, so the
+ * element kernels form face traces from these p + 1 coefficients alone
+ * (triangles, and quadrilaterals up to p = 2).  A host-side query with no
+ * counterpart in the reference: tests pin it against the reference's face
+ * tables. */
+int nz_face_node(int nface, int p, int f, int j);
+
 /* R(U) and the wave-speed sum, unfolded.
  * Replaces edge_pass + element_pass(apply_mass=False)
  * (src/backends/_numba_kernels.py:484-583, 667-759) and the NumPy

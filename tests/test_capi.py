@@ -71,3 +71,38 @@ def test_unsupported_configuration_is_an_error():
     assert rc == -1
     assert b"unsupported element configuration" in dll.nz_last_error()
     assert dll.nz_scratch_doubles(C.byref(P)) == -1
+
+
+def test_face_nodes_match_the_reference_tables():
+    """nz_face_node (a host query) against the reference's face tables: for
+    every element kind, order and geometry order, and both sides, the rows of
+    phi_face that are not round-off are exactly the p + 1 listed nodes, in
+    lattice order.  Where the kernels drop the other rows (triangles, and
+    quadrilaterals up to p = 2) those rows are at most 2.6e-15; p = 3
+    quadrilaterals, which keep the full sums, carry up to 4.3e-14 there."""
+    import numpy as np
+
+    lib = nz_module("_lib")
+    dll = C.CDLL(lib.LIB_PATH)
+    fn = dll.nz_face_node
+    fn.restype, fn.argtypes = C.c_int, [C.c_int] * 4
+    tabs = np.load(os.path.join(ROOT, "tests", "golden", "tables.npz"))
+    seen = 0
+    for kind, nface in (("tri", 3), ("quad", 4)):
+        for p in (1, 2, 3):
+            for q in (1, 2):
+                pf = tabs[f"{kind}_p{p}_q{q}_phi_face"]
+                assert pf.shape[1] == nface
+                for f in range(nface):
+                    nodes = [fn(nface, p, f, j) for j in range(p + 1)]
+                    off = np.ones(pf.shape[2], bool)
+                    off[nodes] = False
+                    for side in range(2):
+                        big = np.flatnonzero(np.abs(pf[side, f]).max(axis=1) > 1e-13)
+                        assert sorted(nodes) == list(big), (kind, p, q, f, side, nodes, big)
+                        small = np.abs(pf[side, f][off]).max()
+                        bound = 4.3e-14 if (kind == "quad" and p == 3) else 2.6e-15
+                        assert small <= bound, (kind, p, q, f, side, small)
+                    seen += 1
+    assert seen == 2 * 3 * (3 + 4)
+    assert fn(3, 0, 0, 0) == -1 and fn(3, 2, 3, 0) == -1 and fn(4, 2, 0, 3) == -1
