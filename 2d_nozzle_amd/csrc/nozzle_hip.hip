@@ -1991,8 +1991,9 @@ void k_vjp(const nz_problem P, const double* __restrict__ T,
         const int i = slot + kk * SLOTS;
         if (i < N && live) {
             double r[4] = {0, 0, 0, 0};
+            const double rdet = C::AFF ? 1.0 / sG[4 * TE + el] : 0.0;  // M^-1 = M_ref^-1 / det J
             for (int j = 0; j < N; ++j) {
-                const double m = C::AFF ? T[P.off.minv_ref + i * N + j] / sG[4 * TE + el]
+                const double m = C::AFF ? T[P.off.minv_ref + i * N + j] * rdet
                                         : P.minv[e * sym_size(N) + sym_index(N, i, j)];
                 for (int s = 0; s < 4; ++s) r[s] += m * sR[(j * 4 + s) * TP + el];
             }
@@ -2379,33 +2380,53 @@ void k_vjpw(const nz_problem P, const double* __restrict__ T,
         return;
     }
     // adjoint pseudo-time rate -M^-1 (vjp - seed) and one classical RK4
-    // stage (src/sensitivity.py:344-357), the update in coalesced rows
+    // stage (src/sensitivity.py:344-357), the update in coalesced rows.
+    // Affine elements: M^-1 = M_ref^-1 / det J, applied as M_ref^-1 times
+    // 1 / det J (one division per element instead of one per entry).
+    const double rdet = C::AFF ? 1.0 / sGe[el * 5 + 4] : 0.0;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
         double r = 0.0;
 #pragma unroll
         for (int j = 0; j < N; ++j) {
-            const double m = C::AFF ? T[P.off.minv_ref + i * N + j] / sGe[el * 5 + 4]
+            const double m = C::AFF ? T[P.off.minv_ref + i * N + j] * rdet
                                     : (live ? P.minv[e * sym_size(N) + sym_index(N, i, j)] : 0.0);
             r += m * d[j];
         }
         sW[el * NS + i * 4 + s] = -r;
     }
     wave_sync();
-    for (int it = t; it < ne * NS; it += DG_BS) {
-        const int64_t x = e0 * NS + it;
-        const double kv = sW[it], h = sDt[it / NS], lb = Lbase[x];
+    auto rk = [&](int64_t x, double kv, double h, double lb, double ac) {
         if (rk_stage == 1) {
             acc_rk[x] = kv;
             Lnext[x] = lb + 0.5 * h * kv;
         } else if (rk_stage == 2) {
-            acc_rk[x] = acc_rk[x] + 2.0 * kv;
+            acc_rk[x] = ac + 2.0 * kv;
             Lnext[x] = lb + 0.5 * h * kv;
         } else if (rk_stage == 3) {
-            acc_rk[x] = acc_rk[x] + 2.0 * kv;
+            acc_rk[x] = ac + 2.0 * kv;
             Lnext[x] = lb + h * kv;
         } else {
-            Lnext[x] = lb + h / 6.0 * (acc_rk[x] + kv);
+            Lnext[x] = lb + h / 6.0 * (ac + kv);
+        }
+    };
+    if (ne == TE) {  // a full tile: every row requested before the first update
+        double lb[NU], ac[NU];
+#pragma unroll
+        for (int j = 0; j < NU; ++j) {
+            const int64_t x = e0 * NS + t + j * DG_BS;
+            lb[j] = Lbase[x];
+            ac[j] = rk_stage == 1 ? 0.0 : acc_rk[x];
+        }
+#pragma unroll
+        for (int j = 0; j < NU; ++j) {
+            const int it = t + j * DG_BS;
+            rk(e0 * NS + it, sW[it], sDt[it / NS], lb[j], ac[j]);
+        }
+    } else {
+        for (int it = t; it < ne * NS; it += DG_BS) {
+            const int64_t x = e0 * NS + it;
+            rk(x, sW[it], sDt[it / NS], Lbase[x], rk_stage == 1 ? 0.0 : acc_rk[x]);
         }
     }
 }
