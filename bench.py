@@ -455,7 +455,8 @@ def _stops(res) -> dict:
     return {k: f"{c}/{t}" for k, (c, t) in sorted(fam.items())}
 
 
-def sweep_bench(n: int, tol: float, n_lone: int, max_it: int = 50000) -> dict:
+def sweep_bench(n: int, tol: float, n_lone: int, max_it: int = 50000,
+                serial_check: bool = True) -> dict:
     """Designs/s of the batched march against one-at-a-time solves.
 
     The mixed sweep (area ratio U[1.5, 4], refine and order in {1, 2, 3},
@@ -503,6 +504,20 @@ def sweep_bench(n: int, tol: float, n_lone: int, max_it: int = 50000) -> dict:
     conv = int(sum(r.converged for r in res))
     print(f"sweep: batch {t_batch:.1f} s, {conv}/{n} converged, iterations "
           f"{iters.min()}..{iters.max()}", file=sys.stderr, flush=True)
+    serial = {}
+    if serial_check:
+        # the same batch with every family queued on one stream, one family
+        # after the other (captured chunks as well), against the families
+        # side by side on their own streams
+        run2 = batch.BatchedSteady(designs, flow, opts, device_setup=True)
+        res2 = run2.solve(use_graph=True, progress=progress, concurrent=False)
+        same = sum(int(a.iterations == b.iterations and np.array_equal(a.U, b.U, equal_nan=True))
+                   for a, b in zip(res, res2))
+        serial = {"one_stream_s": round(run2.wall_s, 3),
+                  "one_stream_same_results": f"{same}/{n}"}
+        print(f"sweep: one stream {run2.wall_s:.1f} s ({same}/{n} identical)", file=sys.stderr,
+              flush=True)
+        del run2, res2
     pick = np.linspace(0, n - 1, min(n_lone, n)).round().astype(int)
     t_lone, lone_match, t_fwd = 0.0, 0, {}
     for g in pick:
@@ -560,6 +575,8 @@ def sweep_bench(n: int, tol: float, n_lone: int, max_it: int = 50000) -> dict:
         "stopped_by_family": _stops(res),
         "iterations_min": int(iters.min()), "iterations_max": int(iters.max()),
         "batch_s": round(t_batch, 3), "designs_per_s": round(n / t_batch, 3),
+        "batch": "families side by side, one HIP stream each, captured 50-step chunks",
+        **serial,
         "lone_sample": int(len(pick)), "lone_s": round(t_lone, 3),
         "lone_designs_per_s": round(lone_rate, 3),
         "speedup": round((n / t_batch) / lone_rate, 2),
@@ -616,6 +633,10 @@ def main() -> int:
                     help="designs of the mixed sweep (seed 1) marched in one batch")
     ap.add_argument("--sweep-tol", type=float, default=1e-10)
     ap.add_argument("--sweep-maxit", type=int, default=50000)
+    ap.add_argument("--copy-only", action="store_true",
+                    help="only the 1 GiB device copy (the counter calibration of traffic)")
+    ap.add_argument("--no-sweep-serial", action="store_true",
+                    help="skip the sweep's one-stream comparison run")
     ap.add_argument("--sweep-lone", type=int, default=8,
                     help="designs of the sweep also solved one at a time, for the comparison")
     args = ap.parse_args()
@@ -629,6 +650,14 @@ def main() -> int:
     import torch
 
     t_start = time.perf_counter()
+    if args.copy_only:
+        st = torch.cuda.current_stream()
+        ca = torch.full((2**27,), 1.0, dtype=torch.float64, device="cuda")
+        cb = torch.empty_like(ca)
+        t_copy, _ = _timed(lambda: cb.copy_(ca), 10, 2, st, min_warm_s=0.3)
+        print(json.dumps({"hbm_copy_gbs": round(2 * ca.numel() * 8 * 10 / t_copy / 1e9, 1),
+                          "bytes_read_per_copy": ca.numel() * 8}), flush=True)
+        return 0
     wl, cf, be = _pkg("workloads"), _pkg("config"), _pkg("backend")
     ops, U = wl.channel_case(args.ncx, args.ncy, args.order)
     E, N = ops.n_elem, ops.ref.n_basis
@@ -713,7 +742,8 @@ def main() -> int:
             return
         t0 = time.perf_counter()
         try:
-            out[name] = fn()
+            with contextlib.redirect_stdout(sys.stderr):  # progress lines of marches
+                out[name] = fn()
         except Exception as exc:  # a failed optional leg is reported, not hidden
             out[name] = {"error": f"{type(exc).__name__}: {exc}"}
         print(f"leg {name}: {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
@@ -755,7 +785,8 @@ def main() -> int:
     leg("config1", not args.no_config1, config1_bench)
     leg("shocked", not args.no_shocked, lambda: shocked_bench(args.shocked_refine, 20))
     leg("sweep", not args.no_sweep and args.sweep > 0,
-        lambda: sweep_bench(args.sweep, args.sweep_tol, args.sweep_lone, args.sweep_maxit))
+        lambda: sweep_bench(args.sweep, args.sweep_tol, args.sweep_lone, args.sweep_maxit,
+                            serial_check=not args.no_sweep_serial))
     leg("adjoint", not args.no_adjoint,
         lambda: adjoint_bench(args.adjoint_refine, fd_check=args.adjoint_fd))
 
