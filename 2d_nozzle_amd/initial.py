@@ -15,8 +15,11 @@ the volume quadrature points, a transverse velocity ``vy = vx (y / y_wall)
 y_wall'`` is added, and the pointwise state is L2-projected,
 ``U_e = M_e^-1 int phi_i u0``.
 
-Everything here is NumPy on the host: it runs once per solve, before the
-device path starts.
+The regime logic and the quasi-1D bisections are scalar NumPy work on the
+host, once per design.  The per-point work has device forms:
+``quasi1d_values_device`` evaluates the point values with ``nz_quasi1d_values``
+and ``quasi1d_initial(..., backend=bk)`` projects with ``nz_project``; the
+host functions here are their pinned counterparts.
 """
 
 from __future__ import annotations
