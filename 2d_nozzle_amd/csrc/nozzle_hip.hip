@@ -183,6 +183,18 @@ __device__ __forceinline__ int nbr_node(int key, int j) {
     return C::SPT ? face_node(C::NF, C::PO, key, j) : j;
 }
 
+// The face quadrature weight of point q (a per-lane index): the weights are
+// read with wave-uniform scalar loads and picked by a select chain, so no
+// lane waits on a vector load for them.
+template <class C>
+__device__ __forceinline__ double face_weight(const double* __restrict__ T, const nz_problem& P,
+                                              int q) {
+    double w = T[P.off.w_face];
+#pragma unroll
+    for (int k = 1; k < C::NQF; ++k) w = q == k ? T[P.off.w_face + k] : w;
+    return w;
+}
+
 // whether basis function i enters the traces of face f
 template <class C>
 __device__ __forceinline__ bool on_face(int f, int i) {
@@ -684,7 +696,9 @@ __device__ __forceinline__ void dg_volume(const nz_problem& P, const double* __r
                     gy = ge[1];
                     hx = ge[2];
                     hy = ge[3];
-                    wd = T[P.off.w_vol + q] * ge[4];
+                    // the weight only matters unfolded (exact 1 otherwise)
+                    if (fold) wd = 1.0;
+                    else wd = T[P.off.w_vol + q] * ge[4];
                 } else {
                     const double* gq = P.geo + (ep * NQV + q) * 5;
                     gx = gq[0];
@@ -1011,7 +1025,7 @@ void k_dg(const nz_problem P, const double* __restrict__ T,
                     const double* eg = sEG + (pe * NF + f) * L::GF + (C::STR ? 0 : q * 3);
                     const double nx = eg[0], ny = eg[1];
                     sp = interface_flux<FLUX>(P, ul, ur, nx, ny, fx);
-                    const double sc = eg[2] * T[P.off.w_face + q];
+                    const double sc = eg[2] * face_weight<C>(T, P, q);
 #pragma unroll
                     for (int c = 0; c < 4; ++c) fx[c] = fx[c] * sc;
                 } else {  // boundary faces: weighted flux precomputed by k_bflux
@@ -1354,7 +1368,7 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
                 ur[c] = sdA == 0 ? b[c] : a[c];
             }
             sp = interface_flux<FLUX>(P, ul, ur, e_nx, e_ny, fx);
-            const double sc = e_jac * T[P.off.w_face + q];
+            const double sc = e_jac * face_weight<C>(T, P, q);
 #pragma unroll
             for (int c = 0; c < 4; ++c) fx[c] = fx[c] * sc;
         }
@@ -2074,10 +2088,13 @@ void k_vjpw(const nz_problem P, const double* __restrict__ T,
     __shared__ double sEG[L::EGN];
     __shared__ double sGe[L::GEN];
     __shared__ double sDt[TE];
+    __shared__ double sWv[NQV];  // volume quadrature weights
     __shared__ int sNb[TE * NF], sIn[TE * NF];
 
     const int t = threadIdx.x;
     const int el = t >> 2, s = t & 3;
+    static_assert(NQV <= DG_BS, "one lane per volume weight");
+    if (t < NQV) sWv[t] = T[P.off.w_vol + t];
     int64_t e0;
     int ne;
     if (P.vjp_tiles) {
@@ -2233,7 +2250,7 @@ void k_vjpw(const nz_problem P, const double* __restrict__ T,
                 if (nbp >= 0) {
                     const int qn = NQF - 1 - qo;
                     const double* eg = sEG + (pe * NF + f) * L::GF + (C::STR ? 0 : q * 3);
-                    const double jw = eg[2] * T[P.off.w_face + q];
+                    const double jw = eg[2] * face_weight<C>(T, P, q);
                     double uL[4], uR[4], ob[4];
 #pragma unroll
                     for (int c = 0; c < 4; ++c) {
@@ -2306,7 +2323,7 @@ void k_vjpw(const nz_problem P, const double* __restrict__ T,
                         xy = ge[1];
                         ex = ge[2];
                         ey = ge[3];
-                        wd = T[P.off.w_vol + q] * ge[4];
+                        wd = sWv[q] * ge[4];
                     } else {
                         const double* gq = P.geo + ((e0 + pe) * NQV + q) * 5;
                         xx = gq[0];
