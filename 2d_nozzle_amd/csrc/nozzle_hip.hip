@@ -881,43 +881,57 @@ void k_dg(const nz_problem P, const double* __restrict__ T,
     // ---- stage the tile (coalesced), the face maps, the face table and the
     // affine metrics; every load is issued before the first LDS write
     {
+        // Every load is unconditional (entries past the tile read a valid
+        // address and are masked: loads under branches would end in a full
+        // wait where the branches join), and the edge geometry is requested
+        // with the rest: lane r holds element-face r's face map (a tile has
+        // at most 64 faces).
+        static_assert(TE * NF <= DG_BS, "one lane per element face");
         constexpr int NU = TE * NS / DG_BS;  // = N
-        constexpr int NM = (TE * NF + DG_BS - 1) / DG_BS;
         constexpr int NT = (NPF + DG_BS - 1) / DG_BS;
         constexpr int NG = (L::GEN + DG_BS - 1) / DG_BS;
-        double ru[NU], rt[NT], rg[NG];
-        int rn[NM], ri[NM];
+        constexpr int NE_ = (L::EGN + DG_BS - 1) / DG_BS;
+        double ru[NU], rt[NT], rg[NG], rq[NE_];
+        int rn, ri;
+        {
+            const bool ok = t < ne * NF;
+            const int a = P.fe_nbr[e0 * NF + (ok ? t : 0)], b = P.fe_info[e0 * NF + (ok ? t : 0)];
+            rn = ok ? a : -1;
+            ri = ok ? b : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < NE_; ++j) {
+            const int it = t + j * DG_BS;
+            const int r = it / L::GF, jj = it - r * L::GF;  // r = element-face
+            const bool ok = it < L::EGN && r < ne * NF;
+            const int64_t k = __shfl(ri, ok ? r : 0, 64) >> 3;
+            const double v = P.edge_geo[(ok ? k : 0) * L::GF + (ok ? jj : 0)];
+            rq[j] = ok ? v : 0.0;
+        }
 #pragma unroll
         for (int j = 0; j < NU; ++j) {
             const int it = t + j * DG_BS, r = it / NS;
-            ru[j] = r < ne ? Uin[e0 * NS + it] : 0.0;
-        }
-#pragma unroll
-        for (int j = 0; j < NM; ++j) {
-            const int it = t + j * DG_BS;
-            const bool ok = it < ne * NF;
-            rn[j] = ok ? P.fe_nbr[e0 * NF + it] : -1;
-            ri[j] = ok ? P.fe_info[e0 * NF + it] : 0;
+            const double v = Uin[e0 * NS + (r < ne ? it : 0)];
+            ru[j] = r < ne ? v : 0.0;
         }
 #pragma unroll
         for (int j = 0; j < NT; ++j) {
             const int it = t + j * DG_BS;
-            rt[j] = it < NPF ? T[P.off.phi_face + it] : 0.0;
+            const double v = T[P.off.phi_face + (it < NPF ? it : 0)];
+            rt[j] = it < NPF ? v : 0.0;
         }
 #pragma unroll
         for (int j = 0; j < NG; ++j) {
             const int it = t + j * DG_BS;
-            rg[j] = C::AFF && it < ne * 5 ? P.geo[e0 * 5 + it] : 0.0;
+            const bool ok = C::AFF && it < ne * 5;
+            const double v = C::AFF ? P.geo[e0 * 5 + (ok ? it : 0)] : 0.0;
+            rg[j] = ok ? v : 0.0;
         }
 #pragma unroll
         for (int j = 0; j < NU; ++j) sW[t + j * DG_BS] = ru[j];
-#pragma unroll
-        for (int j = 0; j < NM; ++j) {
-            const int it = t + j * DG_BS;
-            if (it < TE * NF) {
-                sNb[it] = rn[j];
-                sIn[it] = ri[j];
-            }
+        if (t < TE * NF) {
+            sNb[t] = rn;
+            sIn[t] = ri;
         }
 #pragma unroll
         for (int j = 0; j < NT; ++j)
@@ -925,20 +939,6 @@ void k_dg(const nz_problem P, const double* __restrict__ T,
 #pragma unroll
         for (int j = 0; j < NG; ++j)
             if (t + j * DG_BS < L::GEN) sGe[t + j * DG_BS] = rg[j];
-    }
-    wave_sync();
-    // edge geometry of the tile's faces, gathered by edge id
-    {
-        constexpr int NE_ = (L::EGN + DG_BS - 1) / DG_BS;
-        double rq[NE_];
-#pragma unroll
-        for (int j = 0; j < NE_; ++j) {
-            const int it = t + j * DG_BS;
-            const int r = it / L::GF, jj = it - r * L::GF;  // r = element-face
-            const bool ok = it < L::EGN && r < ne * NF;
-            const int64_t k = ok ? (sIn[r] >> 3) : 0;
-            rq[j] = ok ? P.edge_geo[k * L::GF + jj] : 0.0;
-        }
 #pragma unroll
         for (int j = 0; j < NE_; ++j)
             if (t + j * DG_BS < L::EGN) sEG[t + j * DG_BS] = rq[j];
@@ -1180,27 +1180,37 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
     // offset x belongs to patch element x / NS, so each load instruction
     // covers at most three contiguous runs of memory (an element's NS
     // doubles are contiguous), not one 32-byte piece per element.
+    // Every address below comes from the ids in registers (lane 4 r holds
+    // element r's), so the face maps and metrics are requested first, then
+    // the rows; the edge geometry waits only for the face maps (a wave's
+    // vector loads complete in the order they were issued).
     __builtin_amdgcn_s_setprio(2);
     double u[N], acc[N];
     double g_nx, g_ny, g_jac;
     {
         constexpr int NG = (L::GEN + DG_BS - 1) / DG_BS;
         constexpr int NU = TE * NS / DG_BS;  // = N
-        double ru[NU];
-#pragma unroll
-        for (int j = 0; j < NU; ++j) {
-            const int x = t + j * DG_BS, r = x / NS;
-            const int id = __shfl(myid, 4 * r, 64);
-            ru[j] = r < ne ? Uin[static_cast<int64_t>(id) * NS + (x - r * NS)] : 0.0;
-        }
-        int rn = -1, ri = 0;
-        if (t < SL) {
+        // The face maps and metrics are unconditional (a missing element
+        // reads the patch's first element, then is masked): loads under
+        // branches end in a full wait where the branches join.
+        int rn, ri;
+        {
             const int r = t / NF;
-            if (r < ne) {
-                const int64_t o = static_cast<int64_t>(pt[r]) * NF + (t - r * NF);
-                rn = P.fe_nbr[o];
-                ri = P.fe_info[o];
-            }
+            const bool ok = t < SL && r < ne;
+            const int id = __shfl(myid, 4 * (ok ? r : 0), 64);
+            const int64_t o = static_cast<int64_t>(id) * NF + (ok ? t - r * NF : 0);
+            const int a = P.fe_nbr[o], b = P.fe_info[o];
+            rn = ok ? a : -1;
+            ri = ok ? b : 0;
+        }
+        double rg[NG];
+#pragma unroll
+        for (int j = 0; j < NG; ++j) {
+            const int it = t + j * DG_BS, r = it / 5;
+            const bool ok = it < L::GEN && r < ne;
+            const int id = __shfl(myid, 4 * (ok ? r : 0), 64);
+            const double v = P.geo[static_cast<int64_t>(id) * 5 + (ok ? it - 5 * r : 0)];
+            rg[j] = ok ? v : 0.0;
         }
         // this slot's edge geometry (nx, ny, |dX/dsigma|), kept in the slot
         // lane's registers for the flux items to read across lanes
@@ -1210,11 +1220,14 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
             g_ny = eg[1];
             g_jac = eg[2];
         }
-        double rg[NG];
+        // the rows last: the wait where their branches join is the one the
+        // staging needs anyway
+        double ru[NU];
 #pragma unroll
-        for (int j = 0; j < NG; ++j) {
-            const int it = t + j * DG_BS, r = it / 5;
-            rg[j] = it < L::GEN && r < ne ? P.geo[static_cast<int64_t>(pt[r]) * 5 + (it - 5 * r)] : 0.0;
+        for (int j = 0; j < NU; ++j) {
+            const int x = t + j * DG_BS, r = x / NS;
+            const int id = __shfl(myid, 4 * r, 64);
+            ru[j] = r < ne ? Uin[static_cast<int64_t>(id) * NS + (x - r * NS)] : 0.0;
         }
         if (s == 0) sId[el] = live ? myid : -1;
         if (t < SL) {
@@ -2111,46 +2124,58 @@ void k_vjpw(const nz_problem P, const double* __restrict__ T,
     __builtin_amdgcn_s_setprio(2);  // staging at raised priority, as in k_dg
     // ---- stage U and lambda (coalesced), the maps, the face table, the metrics
     {
-        constexpr int NM = (TE * NF + DG_BS - 1) / DG_BS;
+        // unconditional loads (masked past the tile) and the edge geometry in
+        // the same batch, from the face map in lane r, as in k_dg
+        static_assert(TE * NF <= DG_BS, "one lane per element face");
         constexpr int NT = (NPF + DG_BS - 1) / DG_BS;
         constexpr int NG = (L::GEN + DG_BS - 1) / DG_BS;
-        double ru[NU], rl[NU], rt[NT], rg[NG];
-        int rn[NM], ri[NM];
+        constexpr int NE_ = (L::EGN + DG_BS - 1) / DG_BS;
+        double ru[NU], rl[NU], rt[NT], rg[NG], rq[NE_];
+        int rn, ri;
+        {
+            const bool ok = t < ne * NF;
+            const int a = P.fe_nbr[e0 * NF + (ok ? t : 0)], b = P.fe_info[e0 * NF + (ok ? t : 0)];
+            rn = ok ? a : -1;
+            ri = ok ? b : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < NE_; ++j) {
+            const int it = t + j * DG_BS;
+            const int r = it / L::GF, jj = it - r * L::GF;
+            const bool ok = it < L::EGN && r < ne * NF;
+            const int64_t k = __shfl(ri, ok ? r : 0, 64) >> 3;
+            const double v = P.edge_geo[(ok ? k : 0) * L::GF + (ok ? jj : 0)];
+            rq[j] = ok ? v : 0.0;
+        }
 #pragma unroll
         for (int j = 0; j < NU; ++j) {
             const int it = t + j * DG_BS, r = it / NS;
-            ru[j] = r < ne ? U[e0 * NS + it] : 0.0;
-            rl[j] = r < ne ? Lin[e0 * NS + it] : 0.0;
-        }
-#pragma unroll
-        for (int j = 0; j < NM; ++j) {
-            const int it = t + j * DG_BS;
-            const bool ok = it < ne * NF;
-            rn[j] = ok ? P.fe_nbr[e0 * NF + it] : -1;
-            ri[j] = ok ? P.fe_info[e0 * NF + it] : 0;
+            const double a = U[e0 * NS + (r < ne ? it : 0)];
+            const double b = Lin[e0 * NS + (r < ne ? it : 0)];
+            ru[j] = r < ne ? a : 0.0;
+            rl[j] = r < ne ? b : 0.0;
         }
 #pragma unroll
         for (int j = 0; j < NT; ++j) {
             const int it = t + j * DG_BS;
-            rt[j] = it < NPF ? T[P.off.phi_face + it] : 0.0;
+            const double v = T[P.off.phi_face + (it < NPF ? it : 0)];
+            rt[j] = it < NPF ? v : 0.0;
         }
 #pragma unroll
         for (int j = 0; j < NG; ++j) {
             const int it = t + j * DG_BS;
-            rg[j] = C::AFF && it < ne * 5 ? P.geo[e0 * 5 + it] : 0.0;
+            const bool ok = C::AFF && it < ne * 5;
+            const double v = C::AFF ? P.geo[e0 * 5 + (ok ? it : 0)] : 0.0;
+            rg[j] = ok ? v : 0.0;
         }
 #pragma unroll
         for (int j = 0; j < NU; ++j) {
             sW[t + j * DG_BS] = ru[j];
             sW[TE * NS + t + j * DG_BS] = rl[j];
         }
-#pragma unroll
-        for (int j = 0; j < NM; ++j) {
-            const int it = t + j * DG_BS;
-            if (it < TE * NF) {
-                sNb[it] = rn[j];
-                sIn[it] = ri[j];
-            }
+        if (t < TE * NF) {
+            sNb[t] = rn;
+            sIn[t] = ri;
         }
 #pragma unroll
         for (int j = 0; j < NT; ++j)
@@ -2158,23 +2183,11 @@ void k_vjpw(const nz_problem P, const double* __restrict__ T,
 #pragma unroll
         for (int j = 0; j < NG; ++j)
             if (t + j * DG_BS < L::GEN) sGe[t + j * DG_BS] = rg[j];
-    }
-    wave_sync();
-    {
-        constexpr int NE_ = (L::EGN + DG_BS - 1) / DG_BS;
-        double rq[NE_];
-#pragma unroll
-        for (int j = 0; j < NE_; ++j) {
-            const int it = t + j * DG_BS;
-            const int r = it / L::GF, jj = it - r * L::GF;
-            const bool ok = it < L::EGN && r < ne * NF;
-            const int64_t k = ok ? (sIn[r] >> 3) : 0;
-            rq[j] = ok ? P.edge_geo[k * L::GF + jj] : 0.0;
-        }
 #pragma unroll
         for (int j = 0; j < NE_; ++j)
             if (t + j * DG_BS < L::EGN) sEG[t + j * DG_BS] = rq[j];
     }
+    wave_sync();
     __builtin_amdgcn_s_setprio(0);
     double u[N], l[N], acc[N];
 #pragma unroll
