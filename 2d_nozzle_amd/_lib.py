@@ -46,7 +46,7 @@ class Problem(C.Structure):
         ("wnorm", C.c_double * 4),
         ("tiles", C.c_void_p), ("n_tiles", C.c_int64),
         ("vjp_tiles", C.c_void_p), ("n_vjp_tiles", C.c_int64),
-        ("vjp_kernel", C.c_int32), ("pad1", C.c_int32),
+        ("vjp_kernel", C.c_int32), ("tiles_cover_mesh", C.c_int32),
         ("patches", C.c_void_p), ("n_patches", C.c_int64),
     ]
 

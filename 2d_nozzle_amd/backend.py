@@ -330,6 +330,7 @@ class HipBackend:
             tab = locality_tiles(np.asarray(ops.node_coords), np.asarray(topo.elem_nodes), E)
             self._tiles = torch.from_numpy(tab.reshape(-1)).to(self.device)
             P.tiles, P.n_tiles = C.c_void_p(self._tiles.data_ptr()), tab.shape[0]
+            P.tiles_cover_mesh = 1  # a permutation of every tile: the adjoint uses it too
             if self.affine:
                 self.set_patches(_DG_PATCHES.get(self._dg_key, False))
 

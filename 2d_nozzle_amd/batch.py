@@ -214,6 +214,7 @@ class _Family:
         P = self.bk.P
         P.tiles = C.c_void_p(self.tiles.data_ptr()) if tiles else C.c_void_p(0)
         P.n_tiles = len(tiles)
+        P.tiles_cover_mesh = 0  # the live designs' tiles only
         if self.bk._scratch.numel() < max(len(tiles), 1024):
             self.bk._scratch = torch.empty(max(len(tiles), 1024), dtype=torch.float64,
                                            device=dev)

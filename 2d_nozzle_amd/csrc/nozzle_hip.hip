@@ -2108,14 +2108,24 @@ void k_vjpw(const nz_problem P, const double* __restrict__ T,
     const int el = t >> 2, s = t & 3;
     static_assert(NQV <= DG_BS, "one lane per volume weight");
     if (t < NQV) sWv[t] = T[P.off.w_vol + t];
-    int64_t e0;
+    // The tile: an entry of the adjoint's own tile table (a batch's live
+    // designs), else of the element kernel's locality table (the neighbour
+    // gathers of a spatially coherent launch hit L2; the slot keeps the
+    // defect's partial sums in element order), else consecutive.
+    int64_t e0, slot;
     int ne;
     if (P.vjp_tiles) {
         e0 = P.vjp_tiles[2 * blockIdx.x];
         ne = static_cast<int>(P.vjp_tiles[2 * blockIdx.x + 1] - e0);
+        slot = blockIdx.x;
+    } else if (P.tiles && P.tiles_cover_mesh) {
+        e0 = P.tiles[3 * blockIdx.x];
+        ne = static_cast<int>(P.tiles[3 * blockIdx.x + 1] - e0);
+        slot = P.tiles[3 * blockIdx.x + 2];
     } else {
         e0 = static_cast<int64_t>(blockIdx.x) * TE;
         ne = static_cast<int>(P.n_elem - e0 < TE ? P.n_elem - e0 : TE);
+        slot = blockIdx.x;
     }
     const bool live = el < ne;
     const int64_t e = e0 + el;
@@ -2406,7 +2416,7 @@ void k_vjpw(const nz_problem P, const double* __restrict__ T,
         if (live)
             for (int i = 0; i < N; ++i) sq += d[i] * d[i];
         for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o, 64);
-        if (t == 0) out[blockIdx.x] = sq;
+        if (t == 0) out[slot] = sq;
         return;
     }
     // adjoint pseudo-time rate -M^-1 (vjp - seed) and one classical RK4
@@ -2882,6 +2892,10 @@ int resolve(const nz_problem* p, Ops* ops) {
         return fail(NZ_E_ARG, "boundary edges need bedge and the bflux workspace");
     if (p->tiles && (p->n_tiles < 0 || p->n_tiles > (int64_t{1} << 31)))
         return fail(NZ_E_ARG, "n_tiles out of range");
+    if (p->tiles_cover_mesh != 0 && p->tiles_cover_mesh != 1)
+        return fail(NZ_E_ARG, "tiles_cover_mesh must be 0 or 1");
+    if (p->tiles && p->tiles_cover_mesh && p->n_tiles != (p->n_elem + nz::TE - 1) / nz::TE)
+        return fail(NZ_E_ARG, "tiles_cover_mesh needs one tile-table entry per tile of the mesh");
     if (p->vjp_tiles && (p->n_vjp_tiles < 0 || p->n_vjp_tiles > (int64_t{1} << 31)))
         return fail(NZ_E_ARG, "n_vjp_tiles out of range");
     if (p->vjp_kernel != 0 && p->vjp_kernel != 1) return fail(NZ_E_ARG, "vjp_kernel must be 0 or 1");

@@ -104,7 +104,12 @@ typedef struct {
      * N <= 4); the two agree to round-off.  HipBackend picks it per (element
      * kind, order, geometry order, flux) by timing both. */
     int32_t vjp_kernel;
-    int32_t pad1;
+    /* 1 when `tiles` lists every consecutive tile of nz::TE (16) elements
+     * exactly once, with norm slot = the tile's index (HipBackend's locality
+     * order): the one-wave adjoint launches then take their tiles from it as
+     * well (neighbour gathers of a spatially coherent launch hit L2).  0 for
+     * any other table (a batch's live designs). */
+    int32_t tiles_cover_mesh;
     /* optional patch table for the element kernel of nz_residual / nz_rate /
      * nz_stage on affine (straight-sided triangle) meshes, used in place of
      * `tiles` when set: one workgroup per patch, n_patches of them.  Patch k

@@ -111,3 +111,41 @@ def test_full_size_steps_bit_identical_and_oracle(channel_full):
     print("full-size 2 steps vs oracle", rel_err(U1, Uo), r1, ro, "limited", loc.n_limited,
           flush=True)
     assert rel_err(U1, Uo) <= 1e-12 and abs(r1 / ro - 1) <= 1e-12
+
+
+@pytest.mark.parametrize("flux", FLUXES)
+def test_adjoint_on_locality_tiles_bit_identical(channel_65k, flux):
+    """The one-wave adjoint kernel takes its tiles from the element kernel's
+    locality table when that table covers the mesh (``tiles_cover_mesh``):
+    the VJP, three adjoint RK4 steps and the adjoint's defect equal plain
+    tiling's bit for bit (the table's slots keep the defect's partial sums in
+    element order)."""
+    be = nz_module("backend")
+    ops, U = channel_65k
+    rng = np.random.default_rng(3)
+    lam = rng.standard_normal(U.shape)
+    seed = rng.standard_normal(U.shape) * 1e-3
+    out = []
+    for plain in (False, True):
+        bk = _backend(ops, flux)
+        assert bk.P.tiles_cover_mesh == 1 and bk._tiles is not None
+        if plain:
+            _plain(bk)
+        key = bk._vjp_key
+        had = be._VJP_KERNEL.get(key)
+        be._VJP_KERNEL[key] = 0  # the one-wave kernel, the one that reads the table
+        try:
+            v = bk.residual_vjp(U, lam)
+            la, rel, its, _ = bk.adjoint_solve(U, seed, tolerance=1e-30, max_iterations=3,
+                                               check_interval=3)
+        finally:
+            if had is None:
+                be._VJP_KERNEL.pop(key, None)
+            else:
+                be._VJP_KERNEL[key] = had
+        out.append((v, la, rel, its))
+    (v0, l0, r0, i0), (v1, l1, r1, i1) = out
+    print(flux, "defect", r0, r1, flush=True)
+    assert np.array_equal(v0, v1)
+    assert np.array_equal(l0, l1)
+    assert r0 == r1 and i0 == i1 == 3
