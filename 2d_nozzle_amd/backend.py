@@ -945,19 +945,41 @@ class HipBackend:
             return self._out(lam.clone(), npy), 0.0, 0, True
         work, acc, dt = self._p(a["work"]), self._p(a["acc"]), self._p(a["dt"])
         pu, ps = self._p(u), self._p(s)
+        bufs = [lam, nxt]  # the step's input and output swap roles every step
+
+        def march(n):
+            stream = self._stream()  # the capture stream while a graph records
+            for _ in range(n):
+                l_, n_ = bufs
+                for stage, (src, dst) in enumerate(((l_, tmp), (tmp, n_), (n_, tmp), (tmp, n_)),
+                                                   start=1):
+                    self._chk(self.lib.nz_adjoint_rk4_stage(
+                        self._pp, pu, self._p(src), self._p(l_), ps, dt, acc, self._p(dst),
+                        stage, work, stream), "nz_adjoint_rk4_stage")
+                bufs.reverse()
+
+        # An even chunk leaves the buffers in their roles, so from its second
+        # occurrence it is replayed from a captured HIP graph (kernels only).
+        graphs: dict = {}
         it, res, converged = 0, norm0, False
         while it < max_iterations:
             steps = min(check_interval, max_iterations - it)
-            for _ in range(steps):
-                self._chk(self.lib.nz_adjoint_rk4_stage(self._pp, pu, self._p(lam), self._p(lam), ps,
-                                                        dt, acc, self._p(tmp), 1, work, st), "rk4")
-                self._chk(self.lib.nz_adjoint_rk4_stage(self._pp, pu, self._p(tmp), self._p(lam), ps,
-                                                        dt, acc, self._p(nxt), 2, work, st), "rk4")
-                self._chk(self.lib.nz_adjoint_rk4_stage(self._pp, pu, self._p(nxt), self._p(lam), ps,
-                                                        dt, acc, self._p(tmp), 3, work, st), "rk4")
-                self._chk(self.lib.nz_adjoint_rk4_stage(self._pp, pu, self._p(tmp), self._p(lam), ps,
-                                                        dt, acc, self._p(nxt), 4, work, st), "rk4")
-                lam, nxt = nxt, lam
+            if self.use_graphs and steps >= 2 and steps % 2 == 0:
+                g = graphs.get(steps)
+                if g is None and ("seen", steps) not in graphs:
+                    graphs[("seen", steps)] = True
+                    march(steps)
+                else:
+                    if g is None:
+                        torch.cuda.current_stream(self.device).synchronize()
+                        g = torch.cuda.CUDAGraph()
+                        with torch.cuda.graph(g):
+                            march(steps)
+                        graphs[steps] = g
+                    g.replay()
+            else:
+                march(steps)
+            lam, nxt = bufs
             it += steps
             self._chk(self.lib.nz_adjoint_defect(self._pp, pu, self._p(lam), ps, work,
                                                  self._p(self._scratch), self._p(a["res"]), st),

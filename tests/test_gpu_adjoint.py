@@ -235,3 +235,26 @@ def test_vjp_tile_shapes_agree(name, key, flux):
     print(name, "tile shapes differ by", err)
     assert err <= 1e-12
     assert bk.tune_vjp(U, lam) in (0, 1)
+
+
+@pytest.mark.parametrize("flux", FLUXES)
+def test_adjoint_march_graph_replay_matches_eager(flux):
+    """The adjoint march's 100-step chunks replayed from a captured HIP graph
+    (first chunk eager, second captured, then four replays, each followed by
+    the eager defect launch and a host read): the same adjoint state and
+    residual bit for bit as the march launched eagerly throughout."""
+    g = golden("cfg1")
+    ops, flow = build_case("cfg1", dense=False)
+    U = g["Ur"]
+    seed = np.random.default_rng(17).standard_normal(U.shape) * 1e-3
+    out = []
+    for graphs in (True, False):
+        bk = backend(ops, flow.replace(flux=flux))
+        bk.use_graphs = graphs
+        lam, rel, its, _ = bk.adjoint_solve(U, seed, tolerance=1e-30, max_iterations=600,
+                                            check_interval=100)
+        out.append((lam, rel, its))
+    (l0, r0, i0), (l1, r1, i1) = out
+    print(flux, r0, r1, i0, i1)
+    assert i0 == i1 == 600
+    assert np.array_equal(l0, l1) and r0 == r1
