@@ -19,7 +19,7 @@ The rest of the metric rides in the same line:
   ``traffic_source``; ``hbm_copy_gbs`` is a 1 GiB device copy timed the same
   way.
 * ``march``: the fused SSP-RK3 stage loop on the same mesh and state.
-* ``config1``: time to solution of the smooth nozzle (P1 triangles, refine
+* ``smooth_nozzle``: time to solution of the smooth nozzle (P1 triangles, refine
   1, Roe, tolerance 1e-10) on the device, with the stop iteration and the
   relative error of thrust, mass flow and exit Mach against the reference's
   converged figures (the committed fixture), and the same march on the CPU
@@ -40,7 +40,7 @@ after the headline are skipped, and say so, when the elapsed time passes
 ``--budget-s``.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-march]
-                    [--no-config1] [--no-shocked] [--no-sweep] [--no-adjoint]
+                    [--no-smooth-nozzle] [--no-shocked] [--no-sweep] [--no-adjoint]
                     [--adjoint-stages K] [--adjoint-fd]
 """
 
@@ -207,7 +207,7 @@ def _timed(fn, steps: int, warmup: int, stream, min_warm_s: float = 1.0,
     return t, {"calls": calls, "seconds": round(el, 2), "settled": settled}
 
 
-def config1_bench() -> dict:
+def smooth_nozzle_bench() -> dict:
     """Time to solution of the smooth nozzle (area ratio
     2.5, P1 triangles, refine 1, Roe, positivity limiter, SSP-RK3) from the
     quasi-1D state to a scaled residual of 1e-10, on the device and on the CPU
@@ -616,7 +616,7 @@ def main() -> int:
     ap.add_argument("--limiter", default="positivity", choices=("positivity", "none"))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-march", action="store_true")
-    ap.add_argument("--no-config1", action="store_true")
+    ap.add_argument("--no-smooth-nozzle", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
@@ -782,7 +782,7 @@ def main() -> int:
     leg("march", not args.no_march, march_leg)
     leg("element_kernel", not args.no_march, lambda: element_kernel_bench(args.ncx, args.ncy))
     leg("cpu_baseline", not args.no_cpu, lambda: cpu_baseline(args.order, args.cpu_seconds))
-    leg("config1", not args.no_config1, config1_bench)
+    leg("smooth_nozzle", not args.no_smooth_nozzle, smooth_nozzle_bench)
     leg("shocked", not args.no_shocked, lambda: shocked_bench(args.shocked_refine, 20))
     leg("sweep", not args.no_sweep and args.sweep > 0,
         lambda: sweep_bench(args.sweep, args.sweep_tol, args.sweep_lone, args.sweep_maxit,
@@ -799,8 +799,8 @@ def main() -> int:
         summary["sweep_designs_per_s"] = out["sweep"]["designs_per_s"]
         summary["sweep_gradients_per_s"] = out["sweep"]["gradients_per_s"]
         summary["sweep_gradient_speedup_vs_one_at_a_time"] = out["sweep"]["gradient_speedup"]
-    if isinstance(out.get("config1"), dict) and "hip_seconds" in out["config1"]:
-        summary["config1_time_to_solution_s"] = out["config1"]["hip_seconds"]
+    if isinstance(out.get("smooth_nozzle"), dict) and "hip_seconds" in out["smooth_nozzle"]:
+        summary["smooth_nozzle_time_to_solution_s"] = out["smooth_nozzle"]["hip_seconds"]
     out["metric_parts"] = summary
     if skipped:
         out["legs_skipped"] = skipped

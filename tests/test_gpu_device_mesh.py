@@ -164,7 +164,7 @@ def test_topology_reuse_across_designs():
     assert rel_err(ops1.elem_area, ops2.elem_area) > 1e-3  # a different design
 
 
-def test_device_build_at_config3_size():
+def test_device_build_of_the_refine7_quad_mesh():
     """Quadrilaterals, Q2, refine 7 (1,146,880 elements): maps bit-identical
     to the host build, geometry within 1e-13, and the setup times."""
     import time
