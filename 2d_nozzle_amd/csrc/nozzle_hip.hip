@@ -668,6 +668,17 @@ __device__ __forceinline__ void dg_volume(const nz_problem& P, const double* __r
     const int vproj = fold ? 3 * N : N;
 #pragma unroll 1
     for (int q0 = 0; q0 < NQV; q0 += L::QR) {
+        // per-point metrics of this lane's (point, element) item: requested
+        // before the interpolation, whose FMAs cover their latency
+        // (unconditional, at a valid point, so no branch join waits on them)
+        double gm[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if constexpr (!C::AFF) {
+            const int qq = t / TE, pe = t % TE, q = q0 + qq;
+            const bool ok = q < NQV && pe < ne;
+            const double* gq = P.geo + (eid(ok ? pe : 0) * NQV + (ok ? q : 0)) * 5;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) gm[k] = gq[k];
+        }
 #pragma unroll 2  // two points' table rows in flight
         for (int qq = 0; qq < L::QR; ++qq) {
             if (q0 + qq < NQV) {
@@ -681,7 +692,6 @@ __device__ __forceinline__ void dg_volume(const nz_problem& P, const double* __r
         wave_sync();
         {
             const int qq = t / TE, pe = t % TE, q = q0 + qq;
-            const int64_t ep = eid(pe);
             if (q < NQV && pe < ne) {
                 double uq[4];
 #pragma unroll
@@ -700,12 +710,11 @@ __device__ __forceinline__ void dg_volume(const nz_problem& P, const double* __r
                     if (fold) wd = 1.0;
                     else wd = T[P.off.w_vol + q] * ge[4];
                 } else {
-                    const double* gq = P.geo + (ep * NQV + q) * 5;
-                    gx = gq[0];
-                    gy = gq[1];
-                    hx = gq[2];
-                    hy = gq[3];
-                    wd = gq[4];
+                    gx = gm[0];
+                    gy = gm[1];
+                    hx = gm[2];
+                    hy = gm[3];
+                    wd = gm[4];
                 }
                 const double sc = fold ? 1.0 : wd;  // exact when folded
 #pragma unroll
@@ -2318,6 +2327,16 @@ void k_vjpw(const nz_problem P, const double* __restrict__ T,
         const double* vt = T + P.off.vol_t;  // per point: phi, dxi, deta, ... (N each)
 #pragma unroll 1
         for (int q0 = 0; q0 < NQV; q0 += L::QR) {
+            // per-point metrics of this lane's item, requested before the
+            // interpolation (unconditional, at a valid point)
+            double gm[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            if constexpr (!C::AFF) {
+                const int qq = t / TE, pe = t % TE, q = q0 + qq;
+                const bool ok = q < NQV && pe < ne;
+                const double* gq = P.geo + ((e0 + (ok ? pe : 0)) * NQV + (ok ? q : 0)) * 5;
+#pragma unroll
+                for (int k = 0; k < 5; ++k) gm[k] = gq[k];
+            }
 #pragma unroll 1
             for (int qq = 0; qq < L::QR; ++qq) {
                 if (q0 + qq < NQV) {
@@ -2348,12 +2367,11 @@ void k_vjpw(const nz_problem P, const double* __restrict__ T,
                         ey = ge[3];
                         wd = sWv[q] * ge[4];
                     } else {
-                        const double* gq = P.geo + ((e0 + pe) * NQV + q) * 5;
-                        xx = gq[0];
-                        xy = gq[1];
-                        ex = gq[2];
-                        ey = gq[3];
-                        wd = gq[4];
+                        xx = gm[0];
+                        xy = gm[1];
+                        ex = gm[2];
+                        ey = gm[3];
+                        wd = gm[4];
                     }
 #pragma unroll
                     for (int c = 0; c < 4; ++c) {
