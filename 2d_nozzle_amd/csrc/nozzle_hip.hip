@@ -387,6 +387,15 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 // writes the weighted flux back at the element's canonical index and the
 // lift is side-independent as well.
 constexpr int DG_BS = 64;  // one wave per workgroup: no cross-wave barriers
+using d2 = double __attribute__((ext_vector_type(2)));  // a 16-byte pair of doubles
+
+// A copy of x that the compiler cannot prove equal to x: indices derived from
+// it are recomputed where they are used instead of being kept live (or
+// spilled) from an earlier computation of the same expression.
+__device__ __forceinline__ int opaque(int x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
 constexpr int TE = 16;     // elements per wave tile (four lanes each)
 constexpr int QS = 80;     // LDS doubles per point: four component rows of TE, = 16 mod 64
 
@@ -434,6 +443,13 @@ struct Tile {
     // spilled registers cost.
     static constexpr int LDS = 8 * (WORK + NF * N * NQF + EGN + GEN + TE) + 8 * TE * NF;
     static constexpr int WPE = LDS * 16 <= 160 * 1024 ? 4 : 1;
+    // a later stage's rows of Un in registers: one double per lane per round
+    using Row = double;
+    static constexpr int NR = TE * NS / DG_BS;
+    __device__ static void put_rows(double* w, int t, const Row (&r)[NR]) {
+#pragma unroll
+        for (int j = 0; j < NR; ++j) w[t + j * DG_BS] = r[j];
+    }
 };
 
 // Adds the number of active lanes whose pred holds to *c with one atomic per
@@ -699,28 +715,33 @@ __device__ __forceinline__ void dg_volume(const nz_problem& P, const double* __r
                 const Prim<double> w = primitives(uq, P.gamma);
                 const double F[4] = {w.r * w.u, w.r * w.u * w.u + w.p, w.r * w.u * w.v, w.r * w.u * w.H};
                 const double G[4] = {w.r * w.v, w.r * w.u * w.v, w.r * w.v * w.v + w.p, w.r * w.v * w.H};
-                double gx, gy, hx, hy, wd;
+                // contravariant metrics; unfolded, they carry the quadrature
+                // weight times det J (the folded tables carry it already)
+                double gx, gy, hx, hy;
                 if (C::AFF) {
                     const double* ge = sGe + pe * 5;
                     gx = ge[0];
                     gy = ge[1];
                     hx = ge[2];
                     hy = ge[3];
-                    // the weight only matters unfolded (exact 1 otherwise)
-                    if (fold) wd = 1.0;
-                    else wd = T[P.off.w_vol + q] * ge[4];
+                    if (!fold) {
+                        const double wd = T[P.off.w_vol + q] * ge[4];
+                        gx *= wd;
+                        gy *= wd;
+                        hx *= wd;
+                        hy *= wd;
+                    }
                 } else {
-                    gx = gm[0];
-                    gy = gm[1];
-                    hx = gm[2];
-                    hy = gm[3];
-                    wd = gm[4];
+                    const double wd = gm[4];
+                    gx = gm[0] * wd;
+                    gy = gm[1] * wd;
+                    hx = gm[2] * wd;
+                    hy = gm[3] * wd;
                 }
-                const double sc = fold ? 1.0 : wd;  // exact when folded
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
-                    sUq[at(qq, c, pe)] = (gx * F[c] + gy * G[c]) * sc;
-                    sFy[at(qq, c, pe)] = (hx * F[c] + hy * G[c]) * sc;
+                    sUq[at(qq, c, pe)] = gx * F[c] + gy * G[c];
+                    sFy[at(qq, c, pe)] = hx * F[c] + hy * G[c];
                 }
             }
         }
@@ -757,7 +778,7 @@ __device__ __forceinline__ void dg_epilogue(
     constexpr int N = C::N, NF = C::NF;
     // a later stage combines with Un: its rows are requested first, in the
     // tile's row layout (coalesced), and handed to the lanes through LDS
-    double rn[TE * C::NS / DG_BS];
+    typename L::Row rn[L::NR];
     if (STAGE && mode == kStageNext) load_un(rn);
     // acc holds R (unfolded) or -rate (affine, folded) for component s
     if ((STAGE || mode != kRes) && !C::AFF && live) {
@@ -786,8 +807,7 @@ __device__ __forceinline__ void dg_epilogue(
 
     // fused stage: time step, combination, norm partials, limiter, store
     if (mode == kStageNext) {
-#pragma unroll
-        for (int j = 0; j < TE * C::NS / DG_BS; ++j) sW[t + j * DG_BS] = rn[j];
+        L::put_rows(sW, t, rn);
         wave_sync();
     }
     double h = 0.0;
@@ -1083,8 +1103,8 @@ void k_dg(const nz_problem P, const double* __restrict__ T,
     dg_epilogue<C, L, STAGE>(P, T, sW, sIn, u, acc, wave, t, el, s, e, live, slot, dt,
                              dt_factor, ca, cb, mode, partial, flags, counters,
                              [&]() { store_tile<L>(Uout + e0 * NS, sW, t, ne); },
-                             [&](double (&rn)[TE * NS / DG_BS]) {  // the tile's rows of Un
-                                 constexpr int NU = TE * NS / DG_BS;
+                             [&](typename L::Row (&rn)[L::NR]) {  // the tile's rows of Un
+                                 constexpr int NU = L::NR;
                                  const double* src = Un + e0 * NS;
 #pragma unroll
                                  for (int j = 0; j < NU; ++j) {
@@ -1146,9 +1166,21 @@ struct PTile {
     static constexpr int IO = TE * SIO;
     static constexpr int WORK = FACE > VOL ? (FACE > IO ? FACE : IO) : (VOL > IO ? VOL : IO);
     static constexpr int GEN = TE * 5;
-    static constexpr int LDS = 8 * (WORK + GEN + TE + 2 * SL) + 4 * (2 * SL + TE + XMAX) + 2 * SL;
+    static constexpr int LDS = 8 * (WORK + GEN + TE + 2 * SL + NQF) + 4 * (2 * SL + TE + XMAX) + 2 * SL;
     // waves per SIMD that the LDS allows, at most four (128 VGPRs)
     static constexpr int WPE = LDS * 16 <= 160 * 1024 ? 4 : (LDS * 12 <= 160 * 1024 ? 3 : 2);
+    // The rows move as 16-byte pairs of doubles: an element's NS doubles are
+    // an even count, so no pair straddles two elements, and half as many
+    // loads and stores need an element id and an address.
+    using Row = d2;
+    static constexpr int EP = NS / 2;                           // pairs per element
+    static constexpr int NR = (TE * EP + DG_BS - 1) / DG_BS;    // pairs per lane
+    __device__ static void put_rows(double* w, int t, const Row (&r)[NR]) {
+#pragma unroll
+        for (int j = 0; j < NR; ++j)
+            if ((TE * EP) % DG_BS == 0 || t + j * DG_BS < TE * EP)
+                reinterpret_cast<Row*>(w)[t + j * DG_BS] = r[j];
+    }
 };
 
 template <class C, int FLUX, bool STAGE>
@@ -1171,6 +1203,7 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
     __shared__ int sNb[SL], sIn[SL], sExt[XMAX], sId[TE];
     __shared__ int8_t sPlan[SL];             // partner slot, -1 boundary, -2 - k off-tile face k
     __shared__ uint8_t sItem[SL];            // flux-owning slots, in slot order
+    __shared__ double sWq[NQF];              // face quadrature weights
 
     const int t = threadIdx.x;
     const int el = t >> 2, s = t & 3;
@@ -1198,7 +1231,6 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
     double g_nx, g_ny, g_jac;
     {
         constexpr int NG = (L::GEN + DG_BS - 1) / DG_BS;
-        constexpr int NU = TE * NS / DG_BS;  // = N
         // The face maps and metrics are unconditional (a missing element
         // reads the patch's first element, then is masked): loads under
         // branches end in a full wait where the branches join.
@@ -1231,12 +1263,13 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
         }
         // the rows last: the wait where their branches join is the one the
         // staging needs anyway
-        double ru[NU];
+        typename L::Row ru[L::NR];
 #pragma unroll
-        for (int j = 0; j < NU; ++j) {
-            const int x = t + j * DG_BS, r = x / NS;
-            const int id = __shfl(myid, 4 * r, 64);
-            ru[j] = r < ne ? Uin[static_cast<int64_t>(id) * NS + (x - r * NS)] : 0.0;
+        for (int j = 0; j < L::NR; ++j) {
+            const int x = t + j * DG_BS, r = x / L::EP;
+            const int id = __shfl(myid, 4 * (r < TE ? r : 0), 64);
+            const d2* src = reinterpret_cast<const d2*>(Uin + static_cast<int64_t>(id) * NS);
+            ru[j] = r < ne ? src[x - r * L::EP] : d2{0.0, 0.0};
         }
         if (s == 0) sId[el] = live ? myid : -1;
         if (t < SL) {
@@ -1246,8 +1279,7 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
 #pragma unroll
         for (int j = 0; j < NG; ++j)
             if (t + j * DG_BS < L::GEN) sGe[t + j * DG_BS] = rg[j];
-#pragma unroll
-        for (int j = 0; j < NU; ++j) sW[t + j * DG_BS] = ru[j];
+        L::put_rows(sW, t, ru);
     }
     wave_sync();
 #pragma unroll
@@ -1292,6 +1324,7 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
             sPlan[t] = static_cast<int8_t>(plan);
             sSp[t] = 0ull;
         }
+        if (t < NQF) sWq[t] = T[P.off.w_face + t];
     }
     wave_sync();
 
@@ -1355,42 +1388,45 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
             for (int c = 0; c < 4; ++c) fx[c] = bf[c];
             sp = bf[4];
         } else {
-            double a[4], b[4];
-#pragma unroll
-            for (int c = 0; c < 4; ++c) a[c] = pa[c];
-            if (pl >= 0) {  // partner in the patch; the owner is side 0
+            // The other side's trace: the partner's row (a partner in the
+            // patch is side 1 of the owner's edge), a ghost row, or, past the
+            // ghost rows, the same sums from global memory.  The two sides
+            // are read through swapped LDS addresses rather than swapped
+            // values.
+            const int kx = -2 - pl, qn = NQF - 1 - qA;
+            const double* po = nullptr;
+            if (pl >= 0) {
                 const int elB = pl / NF, fB = pl - elB * NF;
-                pb = sT + atq(fB * NQF + (NQF - 1 - q), elB);
-#pragma unroll
-                for (int c = 0; c < 4; ++c) b[c] = pb[c];
-            } else {
-                const int kx = -2 - pl, qn = NQF - 1 - qA;
-                if (kx < XMAX) {
-                    const double* pg = sG + (kx * NQF + qn) * 4;
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) b[c] = pg[c];
-                } else {  // past the ghost rows: the same sums, from global memory
-                    const double* ur = Uin + static_cast<int64_t>(sNb[A]) * NS;
-                    const double* ph = T + P.off.phi_face + (infA & 3) * N * NQF + qn;
-#pragma unroll 1
-                    for (int c = 0; c < 4; ++c) {
-                        double v = 0.0;
-                        for (int j = 0; j < NI; ++j) {
-                            const int i = nbr_node<C>(infA & 3, j);
-                            v += ph[i * NQF] * ur[i * 4 + c];
-                        }
-                        b[c] = v;
-                    }
-                }
+                po = pb = sT + atq(fB * NQF + (NQF - 1 - q), elB);
+            } else if (kx < XMAX) {
+                po = sG + (kx * NQF + qn) * 4;
             }
             double ul[4], ur[4];
+            if (po) {
+                const double* pl_ = sdA == 0 ? pa : po;
+                const double* pr_ = sdA == 0 ? po : pa;
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                ul[c] = sdA == 0 ? a[c] : b[c];
-                ur[c] = sdA == 0 ? b[c] : a[c];
+                for (int c = 0; c < 4; ++c) {
+                    ul[c] = pl_[c];
+                    ur[c] = pr_[c];
+                }
+            } else {
+                const double* un = Uin + static_cast<int64_t>(sNb[A]) * NS;
+                const double* ph = T + P.off.phi_face + (infA & 3) * N * NQF + qn;
+#pragma unroll 1
+                for (int c = 0; c < 4; ++c) {
+                    double v = 0.0;
+                    for (int j = 0; j < NI; ++j) {
+                        const int i = nbr_node<C>(infA & 3, j);
+                        v += ph[i * NQF] * un[i * 4 + c];
+                    }
+                    const double a = pa[c];
+                    ul[c] = sdA == 0 ? a : v;
+                    ur[c] = sdA == 0 ? v : a;
+                }
             }
             sp = interface_flux<FLUX>(P, ul, ur, e_nx, e_ny, fx);
-            const double sc = e_jac * face_weight<C>(T, P, q);
+            const double sc = e_jac * sWq[q];
 #pragma unroll
             for (int c = 0; c < 4; ++c) fx[c] = fx[c] * sc;
         }
@@ -1441,21 +1477,25 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
         P, T, sW, sIn, u, acc, wave, t, el, s, e, live, slot, dt, dt_factor, ca, cb, mode,
         partial, flags, counters,
         [&]() {  // the staged rows back, as they were loaded
-            constexpr int NU = TE * NS / DG_BS;
+            const int tq = opaque(t);
 #pragma unroll
-            for (int j = 0; j < NU; ++j) {
-                const int x = t + j * DG_BS, r = x / NS;
+            for (int j = 0; j < L::NR; ++j) {
+                const int x = tq + j * DG_BS, r = x / L::EP;
                 if (r < ne)
                     __builtin_nontemporal_store(
-                        sW[x], Uout + static_cast<int64_t>(sId[r]) * NS + (x - r * NS));
+                        reinterpret_cast<const d2*>(sW)[x],
+                        reinterpret_cast<d2*>(Uout + static_cast<int64_t>(sId[r]) * NS) +
+                            (x - r * L::EP));
             }
         },
-        [&](double (&rn)[TE * NS / DG_BS]) {  // the rows of Un, as U was loaded
-            constexpr int NU = TE * NS / DG_BS;
+        [&](typename L::Row (&rn)[L::NR]) {  // the rows of Un, as U was loaded
+            const int tq = opaque(t);
 #pragma unroll
-            for (int j = 0; j < NU; ++j) {
-                const int x = t + j * DG_BS, r = x / NS;
-                rn[j] = r < ne ? Un[static_cast<int64_t>(sId[r]) * NS + (x - r * NS)] : 0.0;
+            for (int j = 0; j < L::NR; ++j) {
+                const int x = tq + j * DG_BS, r = x / L::EP;
+                const d2* src = reinterpret_cast<const d2*>(
+                    Un + static_cast<int64_t>(sId[r < TE ? r : 0]) * NS);
+                rn[j] = r < ne ? src[x - r * L::EP] : d2{0.0, 0.0};
             }
         });
 }
