@@ -1225,16 +1225,20 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
     // Every address below comes from the ids in registers (lane 4 r holds
     // element r's), so the face maps and metrics are requested first, then
     // the rows; the edge geometry waits only for the face maps (a wave's
-    // vector loads complete in the order they were issued).
+    // vector loads complete in the order they were issued).  The plan needs
+    // only the face maps, so it runs, and the off-tile neighbour columns are
+    // requested, while the rows are still on their way.
     __builtin_amdgcn_s_setprio(2);
     double u[N], acc[N];
     double g_nx, g_ny, g_jac;
+    typename L::Row ru[L::NR];
+    int rn, ri;
+    double wq;  // face quadrature weight t (t < NQF), for sWq
     {
         constexpr int NG = (L::GEN + DG_BS - 1) / DG_BS;
         // The face maps and metrics are unconditional (a missing element
         // reads the patch's first element, then is masked): loads under
         // branches end in a full wait where the branches join.
-        int rn, ri;
         {
             const int r = t / NF;
             const bool ok = t < SL && r < ne;
@@ -1261,15 +1265,16 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
             g_ny = eg[1];
             g_jac = eg[2];
         }
-        // the rows last: the wait where their branches join is the one the
-        // staging needs anyway
-        typename L::Row ru[L::NR];
+        wq = T[P.off.w_face + (t < NQF ? t : 0)];
+        // the rows last, unconditionally too: the entries of missing
+        // elements repeat the patch's first element, a valid state that only
+        // idle lanes compute on and that is never stored
 #pragma unroll
         for (int j = 0; j < L::NR; ++j) {
             const int x = t + j * DG_BS, r = x / L::EP;
-            const int id = __shfl(myid, 4 * (r < TE ? r : 0), 64);
+            const int id = __shfl(myid, 4 * (r < ne ? r : 0), 64);
             const d2* src = reinterpret_cast<const d2*>(Uin + static_cast<int64_t>(id) * NS);
-            ru[j] = r < ne ? src[x - r * L::EP] : d2{0.0, 0.0};
+            ru[j] = src[x - r * L::EP];
         }
         if (s == 0) sId[el] = live ? myid : -1;
         if (t < SL) {
@@ -1279,32 +1284,22 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
 #pragma unroll
         for (int j = 0; j < NG; ++j)
             if (t + j * DG_BS < L::GEN) sGe[t + j * DG_BS] = rg[j];
-        L::put_rows(sW, t, ru);
     }
     wave_sync();
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-        u[i] = sW[el * L::SIO + i * 4 + s];
-        acc[i] = 0.0;
-    }
-    wave_sync();  // the staging rows are reused below
-    __builtin_amdgcn_s_setprio(0);
-    if (t < TE) sRd[t] = 1.0 / sGe[t * 5 + 4];
 
     // ---- plan: one lane per face slot
     bool owner = false, offt = false;
     int plan = -1;
     if (t < SL && t / NF < ne) {
-        const int nb = sNb[t];
-        if (nb < 0) {
+        if (rn < 0) {
             owner = true;  // boundary edge: this slot is its side 0
         } else {
             int j = -1;
 #pragma unroll
-            for (int k = 0; k < TE; ++k) j = sId[k] == nb ? k : j;
+            for (int k = 0; k < TE; ++k) j = sId[k] == rn ? k : j;
             if (j >= 0) {
-                plan = j * NF + (sIn[t] & 3);
-                owner = ((sIn[t] >> 2) & 1) == 0;
+                plan = j * NF + (ri & 3);
+                owner = ((ri >> 2) & 1) == 0;
             } else {
                 offt = owner = true;
             }
@@ -1324,7 +1319,7 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
             sPlan[t] = static_cast<int8_t>(plan);
             sSp[t] = 0ull;
         }
-        if (t < NQF) sWq[t] = T[P.off.w_face + t];
+        if (t < NQF) sWq[t] = wq;
     }
     wave_sync();
 
@@ -1335,7 +1330,8 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
     // ---- off-tile neighbour traces: one (off-tile face, component) item per
     // lane for the first XMAX off-tile faces (the flux items of any further
     // ones form their neighbour's trace themselves).  The neighbour columns
-    // are requested first; the own traces are formed while they arrive.
+    // are requested before the rows are written out of their registers;
+    // the own traces are formed while they arrive.
     const int n_gh = n_off < XMAX ? n_off : XMAX;
     const bool gl = t < 4 * n_gh;
     const int gk = t >> 2, gc = t & 3, gsl = gl ? sExt[gk] : 0;
@@ -1346,6 +1342,16 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
 #pragma unroll
         for (int j = 0; j < NI; ++j) xn[j] = ur[nbr_node<C>(gkey, j) * 4];
     }
+    L::put_rows(sW, t, ru);
+    wave_sync();
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        u[i] = sW[el * L::SIO + i * 4 + s];
+        acc[i] = 0.0;
+    }
+    wave_sync();  // the staging rows are reused below
+    __builtin_amdgcn_s_setprio(0);
+    if (t < TE) sRd[t] = 1.0 / sGe[t * 5 + 4];
     // ---- own traces on every face, canonical point order
     {
         const double (*cols[1])[N] = {&u};
