@@ -86,6 +86,13 @@ struct Cfg {
     // quadrilaterals carry up to 4.3e-14 there and keep the full sums, and
     // p = 0 has no node on a face.
     static constexpr bool SPT = PO >= 1 && !(NF == 4 && PO == 3);
+    // Large face tables (quadrilaterals at P3, curved triangles at P3, Q2
+    // quadrilaterals at P2): the traces over every node and the lifts loop
+    // over the face points instead of unrolling them, so that only one
+    // point's column of the table is held in scalar registers at a time (the
+    // unrolled sums hoisted every entry and spilled hundreds of them).
+    static constexpr bool ROLL_T = !SPT && N * NQF > 32;
+    static constexpr bool ROLL_L = N * NQF > 32;
 };
 
 // Lagrange node j (0..p) of face f, in the reference's node numbering (an
@@ -113,6 +120,20 @@ template <class C, int F, int NC, class OUT>
 __device__ __forceinline__ void trace_face(const double* tab, const double (*const col[NC])[C::N],
                                            OUT out) {
     constexpr int N = C::N, NQF = C::NQF;
+    if constexpr (C::ROLL_T) {
+#pragma unroll 1
+        for (int q = 0; q < NQF; ++q) {
+            double a[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) a[c] = 0.0;
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+#pragma unroll
+                for (int c = 0; c < NC; ++c) a[c] += tab[i * NQF + q] * (*col[c])[i];
+            out(q, a);
+        }
+        return;
+    }
 #pragma unroll
     for (int q = 0; q < NQF; ++q) {
         double a[NC];
@@ -161,6 +182,22 @@ __device__ __forceinline__ void trace_nbr(int key, TAB tab, const double (*const
                                           OUT out) {
     constexpr int NQF = C::NQF;
     constexpr int NI = C::SPT ? C::FN : C::N;
+    if constexpr (C::ROLL_T) {
+#pragma unroll 1
+        for (int q = 0; q < NQF; ++q) {
+            double a[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) a[c] = 0.0;
+#pragma unroll
+            for (int j = 0; j < NI; ++j) {
+                const double b = tab(j * NQF + q);
+#pragma unroll
+                for (int c = 0; c < NC; ++c) a[c] += b * (*cn[c])[j];
+            }
+            out(q, a);
+        }
+        return;
+    }
 #pragma unroll
     for (int q = 0; q < NQF; ++q) {
         double a[NC];
@@ -207,6 +244,8 @@ __device__ __forceinline__ bool on_face(int f, int i) {
 
 // acc[i] += sum_q tab[i * NQF + q] fw[q] over the nodes of face F (all nodes
 // unless C::SPT): the transpose of the face trace, with the side-0 table
+// acc[i] += sum_q tab[i * NQF + q] fw[q] over the nodes of face F (all nodes
+// unless C::SPT): the transpose of the face trace, with the side-0 table
 template <class C, int F>
 __device__ __forceinline__ void lift_face(const double* tab, const double (&fw)[C::NQF],
                                           double (&acc)[C::N]) {
@@ -232,6 +271,19 @@ __device__ __forceinline__ void lift_face_rt(int f, const double* phi_face,
     } else {
         if (f == 2) lift_face<C, 2>(phi_face + 2 * S, fw, acc);
         else lift_face<C, 3>(phi_face + 3 * S, fw, acc);
+    }
+}
+
+// The same lift over every node with the face points in a loop (large
+// tables, see C::ROLL_T): fwq(q) gives the weighted flux at point q.  Each
+// acc[i] receives its NQF terms in point order, as in lift_face.
+template <class C, class FW>
+__device__ __forceinline__ void lift_face_rolled(const double* tab, FW fwq, double (&acc)[C::N]) {
+#pragma unroll 1
+    for (int q = 0; q < C::NQF; ++q) {
+        const double w = fwq(q);
+#pragma unroll
+        for (int i = 0; i < C::N; ++i) acc[i] += tab[i * C::NQF + q] * w;
     }
 }
 
@@ -778,22 +830,51 @@ __device__ __forceinline__ void dg_epilogue(
     constexpr int N = C::N, NF = C::NF;
     // a later stage combines with Un: its rows are requested first, in the
     // tile's row layout (coalesced), and handed to the lanes through LDS
+    // (non-affine elements request them after the inverse mass, whose rows
+    // need the registers)
     typename L::Row rn[L::NR];
-    if (STAGE && mode == kStageNext) load_un(rn);
+    if (STAGE && mode == kStageNext && C::AFF) load_un(rn);
     // acc holds R (unfolded) or -rate (affine, folded) for component s
-    if ((STAGE || mode != kRes) && !C::AFF && live) {
-        double r[N];
+    if ((STAGE || mode != kRes) && !C::AFF) {
+        // (idle lanes read element 0's matrix and discard the products)
+        const double* me = P.minv + (live ? e : 0) * sym_size(N);
+        if constexpr (N >= 9) {
+            // Large matrices: two rows of M^-1 in flight at a time (row i + 1
+            // is requested before row i is summed; the fence keeps row i + 2
+            // behind it), the products parked in this lane's LDS row.  The
+            // sums are the unrolled form's, term for term.
+            wave_sync();  // the volume rows' last reads come first
+            double m[2][N];
 #pragma unroll
-        for (int i = 0; i < N; ++i) {
-            const double* me = P.minv + e * sym_size(N);
-            double a = 0.0;
+            for (int j = 0; j < N; ++j) m[0][j] = me[sym_index(N, 0, j)];
 #pragma unroll
-            for (int j = 0; j < N; ++j) a += me[sym_index(N, i, j)] * acc[j];
-            r[i] = a;
+            for (int i = 0; i < N; ++i) {
+                if (i + 1 < N) {
+#pragma unroll
+                    for (int j = 0; j < N; ++j) m[(i + 1) & 1][j] = me[sym_index(N, i + 1, j)];
+                }
+                double a = 0.0;
+#pragma unroll
+                for (int j = 0; j < N; ++j) a += m[i & 1][j] * acc[j];
+                sW[el * L::SIO + i * 4 + s] = a;
+                asm volatile("" ::: "memory");
+            }
+#pragma unroll
+            for (int i = 0; i < N; ++i) acc[i] = sW[el * L::SIO + i * 4 + s];
+        } else if (live) {
+            double r[N];
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                double a = 0.0;
+#pragma unroll
+                for (int j = 0; j < N; ++j) a += me[sym_index(N, i, j)] * acc[j];
+                r[i] = a;
+            }
+#pragma unroll
+            for (int i = 0; i < N; ++i) acc[i] = r[i];
         }
-#pragma unroll
-        for (int i = 0; i < N; ++i) acc[i] = r[i];
     }
+    if (STAGE && mode == kStageNext && !C::AFF) load_un(rn);
     wave_sync();  // last reads of the volume rows are done; reuse sW for staging
     if (!STAGE) {
         const double osign = mode == kRes ? 1.0 : -1.0;  // rate = -M^-1 R
@@ -1015,21 +1096,30 @@ void k_dg(const nz_problem P, const double* __restrict__ T,
             // The neighbour's face table is read with scalar loads when the
             // wave agrees on it (the usual case on a structured mesh) and
             // from LDS otherwise.
+            // (rolled point loops write each point's trace straight to LDS)
             double tn[NQF];
 #pragma unroll
             for (int q = 0; q < NQF; ++q) tn[q] = 0.0;
             const int key0 = __builtin_amdgcn_readfirstlane(key);
             const double (*cn[1])[N] = {&xn};
-            auto put = [&](int q, const double* a) { tn[q] = a[0]; };
+            auto put = [&](int q, const double* a) {
+                if constexpr (C::ROLL_T) sTn[q * QS + cs] = a[0];
+                else tn[q] = a[0];
+            };
             if (__all(key == key0) && key0 >= 0) {
                 const double* ph = T + P.off.phi_face + key0 * N * NQF;
                 trace_nbr<C, 1>(key0, [&](int k) { return ph[k]; }, cn, put);
             } else if (has) {
                 const double* ph = sPF + key * N * NQF;
                 trace_nbr<C, 1>(key, [&](int k) { return ph[k]; }, cn, put);
-            }
+            } else if constexpr (C::ROLL_T) {
 #pragma unroll
-            for (int q = 0; q < NQF; ++q) sTn[q * QS + cs] = tn[q];
+                for (int q = 0; q < NQF; ++q) sTn[q * QS + cs] = 0.0;
+            }
+            if constexpr (!C::ROLL_T) {
+#pragma unroll
+                for (int q = 0; q < NQF; ++q) sTn[q * QS + cs] = tn[q];
+            }
         }
         wave_sync();
         // (q,e): the flux at edge point q, one item per lane
@@ -1073,14 +1163,18 @@ void k_dg(const nz_problem P, const double* __restrict__ T,
         wave_sync();
         // (e,s): lift, and the element's wave-speed sum
         {
-            double fw[NQF];
-#pragma unroll
-            for (int q = 0; q < NQF; ++q) fw[q] = sTo[q * QS + cs];
             const double* lb = T + lift_off + f * N * NQF;
+            if constexpr (C::ROLL_L) {
+                lift_face_rolled<C>(lb, [&](int q) { return sTo[q * QS + cs]; }, acc);
+            } else {
+                double fw[NQF];
 #pragma unroll
-            for (int i = 0; i < N; ++i) {
+                for (int q = 0; q < NQF; ++q) fw[q] = sTo[q * QS + cs];
 #pragma unroll
-                for (int q = 0; q < NQF; ++q) acc[i] += lb[i * NQF + q] * fw[q];
+                for (int i = 0; i < N; ++i) {
+#pragma unroll
+                    for (int q = 0; q < NQF; ++q) acc[i] += lb[i * NQF + q] * fw[q];
+                }
             }
             if (live) {
                 const double* eg = sEG + (el * NF + f) * L::GF;
@@ -2299,8 +2393,13 @@ void k_vjpw(const nz_problem P, const double* __restrict__ T,
             const int key0 = __builtin_amdgcn_readfirstlane(key);
             const double (*cn[2])[N] = {&xn, &yn};
             auto put = [&](int q, const double* a) {
-                tu[q] = a[0];
-                tl[q] = a[1];
+                if constexpr (C::ROLL_T) {
+                    sUn[q * QS + cs] = a[0];
+                    sLn[q * QS + cs] = a[1];
+                } else {
+                    tu[q] = a[0];
+                    tl[q] = a[1];
+                }
             };
             if (__all(key == key0) && key0 >= 0) {
                 const double* ph = T + P.off.phi_face + key0 * N * NQF;
@@ -2308,11 +2407,16 @@ void k_vjpw(const nz_problem P, const double* __restrict__ T,
             } else if (has) {
                 const double* ph = sPF + key * N * NQF;
                 trace_nbr<C, 2>(key, [&](int k) { return ph[k]; }, cn, put);
-            }
+            } else if constexpr (C::ROLL_T) {
 #pragma unroll
-            for (int q = 0; q < NQF; ++q) {
-                sUn[q * QS + cs] = tu[q];
-                sLn[q * QS + cs] = tl[q];
+                for (int q = 0; q < NQF; ++q) sUn[q * QS + cs] = sLn[q * QS + cs] = 0.0;
+            }
+            if constexpr (!C::ROLL_T) {
+#pragma unroll
+                for (int q = 0; q < NQF; ++q) {
+                    sUn[q * QS + cs] = tu[q];
+                    sLn[q * QS + cs] = tl[q];
+                }
             }
         }
         wave_sync();
@@ -2355,7 +2459,10 @@ void k_vjpw(const nz_problem P, const double* __restrict__ T,
             }
         }
         wave_sync();
-        {
+        if constexpr (C::ROLL_T) {
+            lift_face_rolled<C>(T + P.off.phi_face + f * N * NQF,
+                                [&](int q) { return sUo[q * QS + cs]; }, acc);
+        } else {
             double fw[NQF];
 #pragma unroll
             for (int q = 0; q < NQF; ++q) fw[q] = sUo[q * QS + cs];
