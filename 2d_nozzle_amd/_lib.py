@@ -42,7 +42,7 @@ class Problem(C.Structure):
         ("ca", C.c_double), ("sa", C.c_double), ("p_back", C.c_double),
         ("rho_floor", C.c_double), ("p_floor", C.c_double), ("fraction", C.c_double),
         ("lebesgue", C.c_double),
-        ("bisect_steps", C.c_int32), ("pad0", C.c_int32),
+        ("bisect_steps", C.c_int32), ("bflux_chain", C.c_int32),
         ("wnorm", C.c_double * 4),
         ("tiles", C.c_void_p), ("n_tiles", C.c_int64),
         ("vjp_tiles", C.c_void_p), ("n_vjp_tiles", C.c_int64),

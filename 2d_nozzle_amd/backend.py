@@ -206,6 +206,11 @@ def _least_squares_hessenberg(H: np.ndarray, beta: float) -> tuple[np.ndarray, f
     return y, abs(g[k])
 
 
+# problems with fewer element-basis pairs than this chain their stages'
+# boundary fluxes (HipBackend.chain_bflux)
+CHAIN_MAX_DOFS = 1 << 22
+
+
 class HipBackend:
     """MI355X execution of the DG residual and the pseudo-time march."""
 
@@ -277,6 +282,13 @@ class HipBackend:
         self.limiting = opts.limiter != "none" and rd.order > 0
         self.superbee = self.limiting and opts.limiter == "superbee"
         P.limiter = (2 if self.superbee else 1) if self.limiting else 0
+        # Consecutive stages hand the boundary fluxes of their output to the
+        # next instead of launching the boundary pre-pass (bit-identical).  It
+        # pays where the pre-pass is a sizeable share of a stage (a 4,096-
+        # element P1 channel: 7 % per step); on the 2^20-element P3 channel
+        # the heavier stage epilogue costs more than the 5 us pre-pass it
+        # removes (0.5 % per step), so large problems keep the pre-pass.
+        self.chain_bflux = E * N < CHAIN_MAX_DOFS
         self.tvb = float(getattr(opts, "tvb_constant", 50.0))
         P.rho_t_bf = float(flow.stagnation_density) if getattr(flow, "backflow", False) else 0.0
         P.p_t_bf, P.band_bf = flow.total_pressure, 0.05
@@ -718,11 +730,26 @@ class HipBackend:
             return self.ssprk3_step
         raise ValueError(f"unknown scheme {scheme!r}; choose from {SCHEMES}")
 
-    def _stage(self, uin, un, uout, first: bool, a: float, b: float, fac: float, st):
-        self._chk(self.lib.nz_stage(
-            self._pp, self._p(uin), self._p(un) if un is not None else None, self._p(uout),
-            self._p(self._dt), fac, a, b, int(first), self._p(self._scratch), None,
-            self._p(self._counters), st), "nz_stage")
+    def _stage_chain(self, n_stages: int) -> list:
+        """``nz_problem.bflux_chain`` for a run of ``n_stages`` consecutive
+        stages: each stage after the first finds its input's boundary fluxes
+        left by the one before (bit 0, no boundary pre-pass), each stage but
+        the last leaves its output's for the next (bit 1).  Off with the
+        superbee passes, which change a stage's output after it."""
+        if not self.chain_bflux or self.superbee:
+            return [0] * n_stages
+        return [(1 if i > 0 else 0) | (2 if i < n_stages - 1 else 0) for i in range(n_stages)]
+
+    def _stage(self, uin, un, uout, first: bool, a: float, b: float, fac: float, st,
+               chain: int = 0):
+        self.P.bflux_chain = chain
+        try:
+            self._chk(self.lib.nz_stage(
+                self._pp, self._p(uin), self._p(un) if un is not None else None, self._p(uout),
+                self._p(self._dt), fac, a, b, int(first), self._p(self._scratch), None,
+                self._p(self._counters), st), "nz_stage")
+        finally:
+            self.P.bflux_chain = 0
         if self.superbee:
             self._apply_limiter(uout, st, fused_positivity=True)
 
@@ -736,13 +763,15 @@ class HipBackend:
         st = self._stream()
         fac = self.dt_factor(scheme)
         t1, t2 = self._S[2], self._S[3]
-        for _ in range(n_steps):
+        per = 1 if scheme == "euler" else 3
+        ch = self._stage_chain(n_steps * per)
+        for k in range(0, n_steps * per, per):
             if scheme == "euler":
-                self._stage(cur, None, nxt, True, 1.0, 1.0, fac, st)
+                self._stage(cur, None, nxt, True, 1.0, 1.0, fac, st, ch[k])
             else:
-                self._stage(cur, None, t1, True, 1.0, 1.0, fac, st)
-                self._stage(t1, cur, t2, False, 0.75, 0.25, fac, st)
-                self._stage(t2, cur, nxt, False, 1.0 / 3.0, 2.0 / 3.0, fac, st)
+                self._stage(cur, None, t1, True, 1.0, 1.0, fac, st, ch[k])
+                self._stage(t1, cur, t2, False, 0.75, 0.25, fac, st, ch[k + 1])
+                self._stage(t2, cur, nxt, False, 1.0 / 3.0, 2.0 / 3.0, fac, st, ch[k + 2])
             cur, nxt = nxt, cur
         self._chk(self.lib.nz_norm_finish(self._pp, self._p(self._scratch), self._p(self._res), st),
                   "nz_norm_finish")

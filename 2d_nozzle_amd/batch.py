@@ -229,13 +229,15 @@ class _Family:
         st = bk._stream()
         cur, nxt = self.cur, self.nxt
         t1, t2 = bk._S[2], bk._S[3]
-        for _ in range(n_steps):
+        per = 1 if scheme == "euler" else 3
+        ch = bk._stage_chain(n_steps * per)
+        for k in range(0, n_steps * per, per):
             if scheme == "euler":
-                bk._stage(cur, None, nxt, True, 1.0, 1.0, fac, st)
+                bk._stage(cur, None, nxt, True, 1.0, 1.0, fac, st, ch[k])
             else:
-                bk._stage(cur, None, t1, True, 1.0, 1.0, fac, st)
-                bk._stage(t1, cur, t2, False, 0.75, 0.25, fac, st)
-                bk._stage(t2, cur, nxt, False, 1.0 / 3.0, 2.0 / 3.0, fac, st)
+                bk._stage(cur, None, t1, True, 1.0, 1.0, fac, st, ch[k])
+                bk._stage(t1, cur, t2, False, 0.75, 0.25, fac, st, ch[k + 1])
+                bk._stage(t2, cur, nxt, False, 1.0 / 3.0, 2.0 / 3.0, fac, st, ch[k + 2])
             cur, nxt = nxt, cur
         bk._chk(bk.lib.nz_norm_segments(bk._pp, bk._p(bk._scratch), bk._p(self.seg),
                                         bk._p(self.counts), len(self.active),

@@ -494,7 +494,9 @@ struct Tile {
     // occupancy: the fourth wave hides more of the latency than the few
     // spilled registers cost.
     static constexpr int LDS = 8 * (WORK + NF * N * NQF + EGN + GEN + TE) + 8 * TE * NF;
-    static constexpr int WPE = LDS * 16 <= 160 * 1024 ? 4 : 1;
+    // (also when the LDS allows 14 or 15 tiles per CU, where most SIMDs still
+    // hold four; P3 quadrilaterals need more registers than four allow)
+    static constexpr int WPE = LDS * 14 <= 160 * 1024 && N < 16 ? 4 : 1;
     // a later stage's rows of Un in registers: one double per lane per round
     using Row = double;
     static constexpr int NR = TE * NS / DG_BS;
@@ -816,6 +818,46 @@ __device__ __forceinline__ void dg_volume(const nz_problem& P, const double* __r
 
 }
 
+// The boundary fluxes of a stage's output for the next stage (nz_problem
+// bflux_chain bit 1), from the output rows in LDS: one (boundary face, point)
+// item per lane over the tile's own boundary faces, formed as k_bflux forms
+// them, so the next stage reads the same bits the pre-pass would write.  A
+// boundary face belongs to one element, so the workgroup that reads its entry
+// in this stage is the one that rewrites it, after the read.
+template <class C, class L>
+__device__ __forceinline__ void boundary_next(const nz_problem& P, const double* __restrict__ T,
+                                              const double* sW, const int* sIn, int t, int ne) {
+    constexpr int N = C::N, NQF = C::NQF, NF = C::NF, SL = TE * NF;
+    static_assert(SL <= DG_BS, "one lane per face slot");
+    __shared__ uint8_t sBl[SL];  // the tile's boundary face slots, in slot order
+    const bool bnd = t < SL && t / NF < ne && (sIn[t < SL ? t : 0] >> 3) >= P.n_int;
+    const unsigned long long m = __ballot(bnd);
+    if (m == 0ull) return;
+    if (bnd) sBl[__popcll(m & ((1ull << t) - 1ull))] = static_cast<uint8_t>(t);
+    wave_sync();
+    const int n_it = __popcll(m) * NQF;
+    for (int it = t; it < n_it; it += DG_BS) {
+        const int j = it / NQF, q = it - j * NQF;
+        const int sl = sBl[j], el = sl / NF, f = sl - el * NF;
+        const int64_t k = sIn[sl] >> 3, b = k - P.n_int;
+        const int tag = P.bedge[2 * b + 1] & 7;
+        double uo[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int i = 0; i < N; ++i) {
+            const double w = T[P.off.phi_face + (f * N + i) * NQF + q];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) uo[c] += w * sW[el * L::SIO + i * 4 + c];
+        }
+        const double* eg = C::STR ? P.edge_geo + k * 3 : P.edge_geo + (k * NQF + q) * 3;
+        double fx[4];
+        const double sp = boundary_flux(P, tag, uo, eg[0], eg[1], P.p_back, fx);
+        const double sc = eg[2] * T[P.off.w_face + q];
+        double* out = P.bflux + (b * NQF + q) * 5;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) out[c] = fx[c] * sc;
+        out[4] = sp;
+    }
+}
+
 // The epilogue of an element tile (shared by k_dg and k_dgp): the inverse
 // mass of non-affine elements, then either the residual / rate rows, or the
 // fused Runge-Kutta stage (time step, combination, norm partial, positivity
@@ -824,7 +866,7 @@ template <class C, class L, bool STAGE, class STORE, class LOADN>
 __device__ __forceinline__ void dg_epilogue(
     const nz_problem& P, const double* __restrict__ T, double* sW, const int* sIn,
     double (&u)[C::N], double (&acc)[C::N], double wave, int t, int el, int s, int64_t e, bool live,
-    int64_t slot, double* __restrict__ dt, double dt_factor, double ca, double cb, int mode,
+    int ne, int64_t slot, double* __restrict__ dt, double dt_factor, double ca, double cb, int mode,
     double* __restrict__ partial, uint8_t* __restrict__ flags,
     unsigned long long* __restrict__ counters, STORE store, LOADN load_un) {
     constexpr int N = C::N, NF = C::NF;
@@ -942,6 +984,7 @@ __device__ __forceinline__ void dg_epilogue(
     for (int i = 0; i < N; ++i) sW[el * L::SIO + i * 4 + s] = u[i];
     wave_sync();
     store();
+    if (P.bflux_chain & 2) boundary_next<C, L>(P, T, sW, sIn, t, ne);
 
 
 }
@@ -1194,7 +1237,7 @@ void k_dg(const nz_problem P, const double* __restrict__ T,
 
     dg_volume<C, L>(P, T, sW, sGe, u, acc, t, cs, ne, fold,
                     [&](int pe) -> int64_t { return e0 + pe; });
-    dg_epilogue<C, L, STAGE>(P, T, sW, sIn, u, acc, wave, t, el, s, e, live, slot, dt,
+    dg_epilogue<C, L, STAGE>(P, T, sW, sIn, u, acc, wave, t, el, s, e, live, ne, slot, dt,
                              dt_factor, ca, cb, mode, partial, flags, counters,
                              [&]() { store_tile<L>(Uout + e0 * NS, sW, t, ne); },
                              [&](typename L::Row (&rn)[L::NR]) {  // the tile's rows of Un
@@ -1574,7 +1617,7 @@ void k_dgp(const nz_problem P, const double* __restrict__ T, const double* __res
     dg_volume<C, L>(P, T, sW, sGe, u, acc, t, cs, ne, fold,
                     [&](int pe) -> int64_t { return sId[pe]; });
     dg_epilogue<C, L, STAGE>(
-        P, T, sW, sIn, u, acc, wave, t, el, s, e, live, slot, dt, dt_factor, ca, cb, mode,
+        P, T, sW, sIn, u, acc, wave, t, el, s, e, live, ne, slot, dt, dt_factor, ca, cb, mode,
         partial, flags, counters,
         [&]() {  // the staged rows back, as they were loaded
             const int tq = opaque(t);
@@ -2907,10 +2950,11 @@ struct Launch {
         (void)nblk;  // k_dg tiles are nz::TE elements, or come from the tile table
         const int64_t nblk_dg = P.tiles ? P.n_tiles : (P.n_elem + nz::TE - 1) / nz::TE;
         const int64_t nbq = (P.n_edge - P.n_int) * C::NQF;
-        if (nbq > 0)
+        const bool stage = mode == nz::kStageFirst || mode == nz::kStageNext;
+        // a chained stage finds Uin's boundary fluxes left by the one before
+        if (nbq > 0 && !(stage && (P.bflux_chain & 1)))
             hipLaunchKernelGGL((nz::k_bflux<C>), dim3(static_cast<unsigned>((nbq + nz::BS - 1) / nz::BS)),
                                dim3(nz::BS), 0, st, P, P.tables, Uin);
-        const bool stage = mode == nz::kStageFirst || mode == nz::kStageNext;
         if constexpr (C::AFF) {
             if (P.patches) {
                 if (P.n_patches <= 0) return;
@@ -3065,6 +3109,10 @@ int resolve(const nz_problem* p, Ops* ops) {
         return fail(NZ_E_ARG, "n_tiles out of range");
     if (p->tiles_cover_mesh != 0 && p->tiles_cover_mesh != 1)
         return fail(NZ_E_ARG, "tiles_cover_mesh must be 0 or 1");
+    if (p->bflux_chain < 0 || p->bflux_chain > 3)
+        return fail(NZ_E_ARG, "bflux_chain must be 0..3");
+    if (p->bflux_chain != 0 && p->limiter == 2)
+        return fail(NZ_E_ARG, "bflux_chain needs the stage's output final (limiter 2 runs passes after it)");
     if (p->tiles && p->tiles_cover_mesh && p->n_tiles != (p->n_elem + nz::TE - 1) / nz::TE)
         return fail(NZ_E_ARG, "tiles_cover_mesh needs one tile-table entry per tile of the mesh");
     if (p->vjp_tiles && (p->n_vjp_tiles < 0 || p->n_vjp_tiles > (int64_t{1} << 31)))

@@ -91,6 +91,36 @@ __device__ __forceinline__ void normal_flux(const Prim<T>& w, const N_& nx, cons
     f[3] = w.r * w.H * vn;
 }
 
+// The boundary fluxes run in the boundary pre-pass and, for chained
+// Runge-Kutta stages, in the stage kernels' epilogues.  With FMA contraction
+// left to the compiler, a sum of two products can fuse either product
+// depending on the code around it, so the two places would round apart.
+// These forms and the boundary fluxes below are compiled without
+// contraction: every product and sum rounds on its own, wherever inlined.
+template <class T>
+__device__ __forceinline__ Prim<T> primitives_x(const T U[4], double g) {
+#pragma clang fp contract(off)
+    Prim<T> w;
+    w.r = fmax(U[0], kFloor);
+    const T ir = recip(w.r);
+    w.u = U[1] * ir;
+    w.v = U[2] * ir;
+    w.p = fmax((g - 1.0) * (U[3] - 0.5 * w.r * (w.u * w.u + w.v * w.v)), kFloor);
+    w.H = (U[3] + w.p) * ir;
+    w.ir = ir;
+    return w;
+}
+
+template <class T, class N_>
+__device__ __forceinline__ void normal_flux_x(const Prim<T>& w, const N_& nx, const N_& ny, T f[4]) {
+#pragma clang fp contract(off)
+    const T vn = w.u * nx + w.v * ny;
+    f[0] = w.r * vn;
+    f[1] = w.r * w.u * vn + w.p * nx;
+    f[2] = w.r * w.v * vn + w.p * ny;
+    f[3] = w.r * w.H * vn;
+}
+
 // Roe with the Harten-Hyman entropy fix (fluxes.py:38-103).
 template <class T, class N_>
 __device__ __forceinline__ T flux_roe(const T UL[4], const T UR[4], const N_& nx, const N_& ny,
@@ -230,6 +260,7 @@ __device__ __forceinline__ T flux_ausm(const T UL[4], const T UR[4], const N_& n
 template <class T>
 __device__ __forceinline__ T flux_wall(const T U[4], const T& nx, const T& ny, double g,
                                        T out[4]) {
+#pragma clang fp contract(off)
     const T r = fmax(U[0], kFloor), u = U[1] / r, v = U[2] / r;
     const T vn = u * nx + v * ny;
     const T tx = u - vn * nx, ty = v - vn * ny;
@@ -245,8 +276,9 @@ __device__ __forceinline__ T flux_wall(const T U[4], const T& nx, const T& ny, d
 template <class T>
 __device__ __forceinline__ T flux_inflow(const T U[4], const T& nx, const T& ny, const nz_problem& P,
                                          T out[4]) {
+#pragma clang fp contract(off)
     const double g = P.gamma;
-    const Prim<T> w = primitives(U, g);
+    const Prim<T> w = primitives_x(U, g);
     const T a = sqrt(g * w.p / w.r);
     const T vn = w.u * nx + w.v * ny;
     const T Jp = vn + 2.0 * a / (g - 1.0);
@@ -282,8 +314,9 @@ __device__ __forceinline__ T flux_inflow(const T U[4], const T& nx, const T& ny,
 template <class T>
 __device__ __forceinline__ T flux_outflow(const T U[4], const T& nx, const T& ny,
                                           const nz_problem& P, const T& pb, T out[4]) {
+#pragma clang fp contract(off)
     const double g = P.gamma;
-    const Prim<T> w = primitives(U, g);
+    const Prim<T> w = primitives_x(U, g);
     const T a = sqrt(g * w.p / w.r);
     const T vn = w.u * nx + w.v * ny;
     T Ub[4];
@@ -310,8 +343,8 @@ __device__ __forceinline__ T flux_outflow(const T U[4], const T& nx, const T& ny
             for (int s = 0; s < 4; ++s) Ub[s] = t * Ub[s] + (1.0 - t) * Uf[s];
         }
     }
-    const Prim<T> o = primitives(Ub, g);
-    normal_flux(o, nx, ny, out);
+    const Prim<T> o = primitives_x(Ub, g);
+    normal_flux_x(o, nx, ny, out);
     return fabs(o.u * nx + o.v * ny) + sqrt(g * o.p / o.r);
 }
 

@@ -77,7 +77,17 @@ typedef struct {
     /* limiter (src/spatial/limiter.py:98-243) */
     double rho_floor, p_floor, fraction, lebesgue;
     int32_t bisect_steps;
-    int32_t pad0;
+    /* boundary fluxes carried from one Runge-Kutta stage to the next
+     * (nz_stage only; leave 0 elsewhere).  Bit 0: `bflux` already holds the
+     * boundary fluxes of Uin, written by the previous nz_stage on this
+     * problem, so the boundary pre-pass is skipped.  Bit 1: after Uout is
+     * complete, each workgroup writes the boundary fluxes of Uout for its own
+     * boundary faces into `bflux`, for the next stage (the values the
+     * pre-pass would compute, bit for bit).  Only valid when nothing changes
+     * Uout between the two stages: not with limiter 2, whose passes run
+     * after the stage.  No counterpart in the reference, which evaluates the
+     * boundary faces in every residual (src/backends/_numba_kernels.py). */
+    int32_t bflux_chain;
     double wnorm[4];        /* residual-norm component weights (src/backends/base.py:42-78) */
     /* optional tile table for the element kernel of nz_residual / nz_rate /
      * nz_stage, one workgroup per entry in this order: entry k covers
