@@ -47,7 +47,7 @@ class Problem(C.Structure):
         ("tiles", C.c_void_p), ("n_tiles", C.c_int64),
         ("vjp_tiles", C.c_void_p), ("n_vjp_tiles", C.c_int64),
         ("vjp_kernel", C.c_int32), ("tiles_cover_mesh", C.c_int32),
-        ("patches", C.c_void_p), ("n_patches", C.c_int64),
+        ("patches", C.c_void_p), ("n_patches", C.c_int64), ("bjac", C.c_void_p),
     ]
 
 
@@ -139,6 +139,7 @@ SIGNATURES = {
     "nz_residual_vjp": (C.c_int, [C.POINTER(Problem), P, P, P, P, P, P, P, P]),
     "nz_adjoint_rk4_stage": (C.c_int, [C.POINTER(Problem), P, P, P, P, P, P, P, C.c_int, P, P]),
     "nz_adjoint_defect": (C.c_int, [C.POINTER(Problem), P, P, P, P, P, P, P]),
+    "nz_boundary_jacobian": (C.c_int, [C.POINTER(Problem), P, P, P]),
     "nz_superbee": (C.c_int, [C.POINTER(Problem), P, C.c_double, C.c_double, P, P, P]),
     "nz_last_error": (C.c_char_p, []),
     "nz_build_edges_workspace_bytes": (C.c_int64, [C.c_int64, C.c_int32]),

@@ -568,6 +568,8 @@ def batched_gradients(results: list, flow: FlowConditions, opts: SolverOptions, 
         f.its = [0] * len(idx)
         f.rels = [0.0 if d else float("nan") for d in f.done]
         set_vjp_tiles(f)
+        with torch.cuda.stream(f.stream):  # the group's fixed states: boundary Jacobians once
+            bk._set_boundary_jacobian(bk._p(f.u))
         fams.append(f)
 
     def launch(f, n):
@@ -635,6 +637,7 @@ def batched_gradients(results: list, flow: FlowConditions, opts: SolverOptions, 
         # geometry cotangents of the whole group, split per design
         bk, cat = f.bk, f.cat
         bk.P.vjp_tiles, bk.P.n_vjp_tiles = C.c_void_p(0), 0
+        bk.P.bjac = None
         rd, nqf = cat.ref, cat.ref.n_qface
         dev = bk.device
         gm = torch.zeros((cat.n_elem, rd.n_qvol, 5), dtype=torch.float64, device=dev)

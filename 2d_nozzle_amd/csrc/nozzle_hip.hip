@@ -46,6 +46,7 @@
 namespace nz {
 
 constexpr int BS = 256;
+constexpr int BJAC_DOUBLES = 20;  // per boundary face point of nz_problem.bjac
 
 enum Mode { kRes = 0, kRate = 1, kStageFirst = 2, kStageNext = 3 };
 
@@ -1880,6 +1881,43 @@ __global__ __launch_bounds__(BS) void k_bvjp(const nz_problem P, const double* _
     if (gpb) gpb[b * C::NQF + q] = v[6];
 }
 
+// The boundary-flux Jacobians of U for a march of cotangents at fixed U
+// (nz_problem.bjac): per boundary face point, J[s][d] = dFb_s / du_d at the
+// owner's trace (forward-mode duals, as k_bvjp differentiates the flux) and
+// the point's |dX/dsigma| w_q.  The one-wave VJP kernel applies it to the
+// cotangent's traces, so the march launches no boundary pre-pass.
+template <class C>
+__global__ __launch_bounds__(BS) void k_bjac(const nz_problem P, const double* __restrict__ T,
+                                             const double* __restrict__ U,
+                                             double* __restrict__ bjac) {
+    const int64_t it = static_cast<int64_t>(blockIdx.x) * BS + threadIdx.x;
+    const int64_t nb = P.n_edge - P.n_int;
+    if (it >= nb * C::NQF) return;
+    const int64_t b = it / C::NQF;
+    const int q = static_cast<int>(it - b * C::NQF);
+    const int64_t e = P.bedge[2 * b];
+    const int ft = P.bedge[2 * b + 1];
+    const int f = ft >> 3, tag = ft & 7;
+    double uo[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = 0; i < C::N; ++i) {
+        const double w = T[P.off.phi_face + (f * C::N + i) * C::NQF + q];
+        for (int s = 0; s < 4; ++s) uo[s] += w * U[(e * C::N + i) * 4 + s];
+    }
+    const int64_t k = P.n_int + b;
+    const double* eg = C::STR ? P.edge_geo + k * 3 : P.edge_geo + (k * C::NQF + q) * 3;
+    using D = Dual<4>;
+    D ud[4];
+    for (int s = 0; s < 4; ++s) ud[s] = D::seed(uo[s], s);
+    const D nx(eg[0]), ny(eg[1]), pb(P.p_back);
+    D fx[4];
+    boundary_flux(P, tag, ud, nx, ny, pb, fx);
+    double* out = bjac + (b * C::NQF + q) * BJAC_DOUBLES;
+    for (int s = 0; s < 4; ++s)
+        for (int d = 0; d < 4; ++d) out[s * 4 + d] = fx[s].d[d];
+    out[16] = eg[2] * T[P.off.w_face + q];
+    out[17] = out[18] = out[19] = 0.0;
+}
+
 // The register budget of the reverse-mode Roe instantiation is held to the
 // occupancy the LDS allows (three waves per SIMD at P2: no spills, and 19 %
 // faster than the unconstrained allocation's two); the dual-number
@@ -2492,6 +2530,17 @@ void k_vjpw(const nz_problem P, const double* __restrict__ T,
                     } else {
                         hllc_vjp(uL, uR, eg[0], eg[1], P.gamma, P.low_mach, sd == 0, ob, v);
                     }
+                } else if (P.bjac) {  // boundary faces at a fixed U: J^T (jw lambda)
+                    const double* jb = P.bjac + ((k - P.n_int) * NQF + q) * BJAC_DOUBLES;
+                    const double jw = jb[16];
+#pragma unroll
+                    for (int d = 0; d < 4; ++d) v[d] = 0.0;
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        const double a = jw * sLo[at(qo, c, pe)];
+#pragma unroll
+                        for (int d = 0; d < 4; ++d) v[d] += a * jb[c * 4 + d];
+                    }
                 } else {  // boundary faces: the pre-pass's values
                     const double* b = bv + ((k - P.n_int) * NQF + q) * 4;
 #pragma unroll
@@ -2939,6 +2988,7 @@ struct Ops {
     void (*vjp)(const nz_problem&, const double*, const double*, double*, double*, double*,
                 double*, double*, int, const double*, const double*, const double*, double*,
                 double*, int, int64_t, hipStream_t);
+    void (*bjac)(const nz_problem&, const double*, double*, hipStream_t);
     int te;
 };
 
@@ -3029,11 +3079,13 @@ struct Launch {
                     const double* seed, const double* Lbase, const double* dt, double* acc,
                     double* Lnext, int stage, int64_t nblk, hipStream_t st) {
         const int64_t nbq = (P.n_edge - P.n_int) * C::NQF;
-        if (nbq > 0)
+        const bool geo = gmet || gedge;  // geometry cotangents: the normal is differentiated too
+        // the one-wave kernel applies a fixed U's boundary Jacobians itself
+        const bool fixed = P.bjac && !geo && !gpb && P.vjp_kernel == 0;
+        if (nbq > 0 && !fixed)
             hipLaunchKernelGGL((nz::k_bvjp<C>), dim3(static_cast<unsigned>((nbq + nz::BS - 1) / nz::BS)),
                                dim3(nz::BS), 0, st, P, P.tables, U, L, bv, gedge, gpb);
         if (nblk <= 0) return;  // an empty tile table: nothing to march
-        const bool geo = gmet || gedge;  // geometry cotangents: the normal is differentiated too
         if (!geo && P.vjp_kernel == 0) {  // state cotangents on wave tiles
 #define NZ_VJPW(FL)                                                                               \
     hipLaunchKernelGGL((nz::k_vjpw<C, FL>), dim3(static_cast<unsigned>(nblk)), dim3(nz::DG_BS), 0, \
@@ -3061,8 +3113,15 @@ struct Launch {
         else NZ_VJP(0);
 #undef NZ_VJP
     }
+    static void bjac(const nz_problem& P, const double* U, double* out, hipStream_t st) {
+        const int64_t nbq = (P.n_edge - P.n_int) * C::NQF;
+        if (nbq > 0)
+            hipLaunchKernelGGL((nz::k_bjac<C>), dim3(static_cast<unsigned>((nbq + nz::BS - 1) / nz::BS)),
+                               dim3(nz::BS), 0, st, P, P.tables, U, out);
+    }
     static Ops ops() {
-        return Ops{&dg, &limit, &inverse_mass, &project, &boundary_integrals, &superbee, &vjp, C::TE};
+        return Ops{&dg, &limit, &inverse_mass, &project, &boundary_integrals, &superbee, &vjp,
+                   &bjac, C::TE};
     }
 };
 
@@ -3350,6 +3409,15 @@ int nz_adjoint_defect(const nz_problem* p, const double* U, const double* lam, c
     hipLaunchKernelGGL(k_norm_finish, dim3(1), dim3(FIN_BS), 0, st, scratch, nb,
                        static_cast<double>(p->n_elem) * p->nbf * 4.0, out);
     return check_launch("nz_adjoint_defect");
+}
+
+int nz_boundary_jacobian(const nz_problem* p, const double* U, double* bjac, void* stream) {
+    Ops o;
+    int rc = resolve(p, &o);
+    if (rc) return rc;
+    if (!U || !bjac) return fail(NZ_E_ARG, "null array");
+    o.bjac(*p, U, bjac, static_cast<hipStream_t>(stream));
+    return check_launch("nz_boundary_jacobian");
 }
 
 int64_t nz_superbee_work_doubles(const nz_problem* p) {

@@ -132,6 +132,15 @@ typedef struct {
      * construction, src/backends/_numba_kernels.py:484-583). */
     const int32_t *patches;
     int64_t n_patches;
+    /* optional, for a march of state cotangents at a fixed U (the adjoint):
+     * the boundary-flux Jacobians of that U from nz_boundary_jacobian.  When
+     * set, nz_residual_vjp, nz_adjoint_rk4_stage and nz_adjoint_defect on
+     * one-wave tiles (vjp_kernel 0) without geometry cotangents skip their
+     * boundary pre-pass and apply the Jacobian to the cotangent's traces
+     * instead; every U passed must then be the one it was formed from.  No
+     * counterpart in the reference, whose jax.vjp re-linearises the boundary
+     * fluxes at every call (src/sensitivity.py:289-383). */
+    const double *bjac;
 } nz_problem;
 
 /* Number of scratch doubles nz_* calls need for norm partials. */
@@ -269,6 +278,14 @@ int nz_adjoint_rk4_stage(const nz_problem *p, const double *U, const double *lam
 /* out[0] = sqrt(mean((vjp(lam) - seed)^2)) (src/sensitivity.py:378). */
 int nz_adjoint_defect(const nz_problem *p, const double *U, const double *lam, const double *seed,
                       double *work, double *scratch, double *out, void *stream);
+
+/* The boundary-flux Jacobians of U for nz_problem.bjac: per boundary face
+ * point (Fb, nqf), 20 doubles: J[s][d] = dFb_s / du_d (row-major, 16) at the
+ * owner's trace, the point's |dX/dsigma| w_q, and three zeros.  The Jacobian
+ * is the one the boundary pre-pass of nz_residual_vjp differentiates
+ * (src/spatial/boundary.py:41-214 through jax.vjp, src/sensitivity.py);
+ * no reference function computes it on its own. */
+int nz_boundary_jacobian(const nz_problem *p, const double *U, double *bjac, void *stream);
 
 /* dt_e = factor * 2 A_e / max(wave_e, 1e-10) (src/timestepping/step.py:171-183). */
 int nz_time_step(const nz_problem *p, const double *wave, double factor, double *dt,

@@ -238,6 +238,33 @@ def test_vjp_tile_shapes_agree(name, key, flux):
 
 
 @pytest.mark.parametrize("flux", FLUXES)
+@pytest.mark.parametrize("name,key", [("cfg1", "Ur"), ("quad_q2_p2", "U0"),
+                                      ("tri_p2_q2_backflow", "U")])
+def test_fixed_boundary_jacobian_matches_the_pre_pass(name, key, flux):
+    """At a fixed U the one-wave VJP applies U's boundary-flux Jacobians
+    (``nz_boundary_jacobian``, ``nz_problem.bjac``) to the cotangent's face
+    traces instead of running the boundary pre-pass at every call: the same
+    VJP to round-off (its traces sum over the face's nodes, the pre-pass's
+    over every node), on inflow, outflow, backflow and wall faces."""
+    g = golden(name)
+    ops, flow = build_case(name, dense=False)
+    bk = backend(ops, flow.replace(flux=flux))
+    U = g[key]
+    lam = np.random.default_rng(9).standard_normal(U.shape)
+    bk.P.vjp_kernel = 0
+    v0 = np.asarray(bk.residual_vjp(U, lam))
+    u, _ = bk._in(U, "state", "U")
+    bk._set_boundary_jacobian(bk._p(u))
+    try:
+        v1 = np.asarray(bk.residual_vjp(U, lam))
+    finally:
+        bk.P.bjac = None
+    err = float(np.abs(v1 - v0).max() / np.abs(v0).max())
+    print(name, flux, "fixed Jacobian against the pre-pass", err)
+    assert err <= 1e-13
+
+
+@pytest.mark.parametrize("flux", FLUXES)
 def test_adjoint_march_graph_replay_matches_eager(flux):
     """The adjoint march's 100-step chunks replayed from a captured HIP graph
     (first chunk eager, second captured, then four replays, each followed by
