@@ -3226,6 +3226,7 @@ int nz_residual(const nz_problem* p, const double* U, double* R, double* wave, v
     Ops o;
     int rc = resolve(p, &o);
     if (rc) return rc;
+    if (p->bflux_chain != 0) return fail(NZ_E_ARG, "bflux_chain is for nz_stage only");
     if (!U || !R || !wave) return fail(NZ_E_ARG, "null array");
     o.dg(*p, U, nullptr, R, wave, 0.0, 0.0, 0.0, nz::kRes, nullptr, nullptr, nullptr, n_blocks(p, o),
          static_cast<hipStream_t>(stream));
@@ -3236,6 +3237,7 @@ int nz_rate(const nz_problem* p, const double* U, double* F, double* wave, void*
     Ops o;
     int rc = resolve(p, &o);
     if (rc) return rc;
+    if (p->bflux_chain != 0) return fail(NZ_E_ARG, "bflux_chain is for nz_stage only");
     if (!U || !F || !wave) return fail(NZ_E_ARG, "null array");
     o.dg(*p, U, nullptr, F, wave, 0.0, 0.0, 0.0, nz::kRate, nullptr, nullptr, nullptr,
          n_blocks(p, o), static_cast<hipStream_t>(stream));
@@ -3246,6 +3248,7 @@ int64_t nz_rate_noalloc_max_boundary_points(void) { return noalloc::kNoAllocDoub
 
 int nz_rate_noalloc(const nz_problem* p, const double* U, double* F, double* wave, void* stream) {
     if (!p) return fail(NZ_E_ARG, "null problem");
+    if (p->bflux_chain != 0) return fail(NZ_E_ARG, "bflux_chain is for nz_stage only");
     if (!U || !F || !wave) return fail(NZ_E_ARG, "null array");
     const int64_t nbq = (p->n_edge - p->n_int) * static_cast<int64_t>(p->nqf);
     if (p->n_edge < p->n_int || p->nqf <= 0) return fail(NZ_E_ARG, "n_int out of range");

@@ -78,7 +78,8 @@ typedef struct {
     double rho_floor, p_floor, fraction, lebesgue;
     int32_t bisect_steps;
     /* boundary fluxes carried from one Runge-Kutta stage to the next
-     * (nz_stage only; leave 0 elsewhere).  Bit 0: `bflux` already holds the
+     * (nz_stage only; nz_residual, nz_rate and nz_rate_noalloc refuse
+     * a nonzero value).  Bit 0: `bflux` already holds the
      * boundary fluxes of Uin, written by the previous nz_stage on this
      * problem, so the boundary pre-pass is skipped.  Bit 1: after Uout is
      * complete, each workgroup writes the boundary fluxes of Uout for its own

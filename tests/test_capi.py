@@ -73,6 +73,27 @@ def test_unsupported_configuration_is_an_error():
     assert dll.nz_scratch_doubles(C.byref(P)) == -1
 
 
+@pytest.mark.parametrize("entry", ["nz_residual", "nz_rate", "nz_rate_noalloc"])
+@pytest.mark.parametrize("chain", [1, 2, 3])
+def test_stage_to_stage_boundary_fluxes_are_refused_outside_stages(entry, chain):
+    """``bflux_chain`` belongs to ``nz_stage``: the residual and rate entry
+    points refuse it on the host, before any launch (a rate's epilogue would
+    otherwise write boundary fluxes of the rate into the workspace)."""
+    lib = nz_module("_lib")
+    if not os.path.exists(lib.LIB_PATH):
+        pytest.fail("libnozzle_hip.so is not built")
+    dll = lib.load()
+    P = lib.Problem()
+    P.n_elem, P.nface, P.nbf, P.nqv, P.nqf, P.affine, P.straight = 10, 3, 3, 4, 2, 1, 1
+    P.bflux_chain = chain
+    rc = getattr(dll, entry)(C.byref(P), None, None, None, None)
+    assert rc == -1
+    assert b"bflux_chain is for nz_stage only" in dll.nz_last_error()
+    P.bflux_chain = 0  # the same problem passes the check and stops at the null arrays
+    assert getattr(dll, entry)(C.byref(P), None, None, None, None) == -1
+    assert b"null array" in dll.nz_last_error()
+
+
 def test_face_nodes_match_the_reference_tables():
     """nz_face_node (a host query) against the reference's face tables: for
     every element kind, order and geometry order, and both sides, the rows of
