@@ -985,9 +985,8 @@ __device__ __forceinline__ void dg_epilogue(
     for (int i = 0; i < N; ++i) sW[el * L::SIO + i * 4 + s] = u[i];
     wave_sync();
     store();
-    if (P.bflux_chain & 2) boundary_next<C, L>(P, T, sW, sIn, t, ne);
-
-
+    // (stages only: the residual and rate entry points refuse the flag)
+    if (STAGE && (P.bflux_chain & 2)) boundary_next<C, L>(P, T, sW, sIn, t, ne);
 }
 
 template <class C, int FLUX, bool STAGE>
