@@ -262,8 +262,9 @@ def test_fixed_boundary_jacobian_matches_the_pre_pass(name, key, flux):
     err = float(np.abs(v1 - v0).max() / np.abs(v0).max())
     print(name, flux, "fixed Jacobian against the pre-pass", err)
     assert err <= 1e-13
-    # the four-wave kernel keeps its boundary pre-pass: the Jacobians, when
-    # set, leave its VJP unchanged bit for bit
+    # the four-wave kernel and the launch with geometry cotangents keep their
+    # boundary pre-pass: the Jacobians, when set, leave them unchanged bit
+    # for bit
     bk.P.vjp_kernel = 1
     try:
         w0 = np.asarray(bk.residual_vjp(U, lam))
@@ -273,6 +274,14 @@ def test_fixed_boundary_jacobian_matches_the_pre_pass(name, key, flux):
         bk.P.bjac = None
         bk.P.vjp_kernel = 0
     assert np.array_equal(w0, w1)
+    g0 = bk.residual_vjp(U, lam, geometry=True)
+    bk._set_boundary_jacobian(bk._p(u))
+    try:
+        g1 = bk.residual_vjp(U, lam, geometry=True)
+    finally:
+        bk.P.bjac = None
+    for a, b in zip(g0, g1):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
 
 
 @pytest.mark.parametrize("flux", FLUXES)
